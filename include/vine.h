@@ -1,0 +1,158 @@
+/*
+ * vine.h -- C ABI of the MI355X-native exact constrained-shortest-path (CSP)
+ * solver.  Plain pointers and sizes only; no C++ or torch types cross it.
+ *
+ * Reference interfaces each entry point replaces (the reference exposes a C++
+ * template API, not a C ABI):
+ *   vine_graph_create   Graph<Edge_t>::adj_to_csr + make_gpu_graph
+ *                       (include/graph.hpp:136-195) and create_edge_list
+ *                       (main.cu:28-36): host CSR -> device CSR.
+ *   vine_solver_create  the allocation half of RUN_CSPP
+ *                       (algorithm/cspp.cuh:82-123), done once per handle
+ *                       instead of once per call.
+ *   vine_solve          cspp<c_label,c_edge>::solve() -> RUN_CSPP
+ *                       (algorithm/cspp.hpp:36-53, algorithm/cspp.cuh:69-325)
+ *                       for the c_label / c_constraint / c_edge instantiation
+ *                       (include/label.hpp:170-247, include/constraint.hpp:81-105,
+ *                       include/edge.hpp:16-26): one query per (source, sink,
+ *                       delay bound), answers written to caller buffers.
+ *   vine_solve_device   the same with query and answer arrays already in HBM.
+ *
+ * Semantics.  For each query (s, t, bound) the solver returns the
+ * lexicographically smallest (len, delay) over s->t walks whose total delay
+ * is <= bound (c_constraint::pass_constraints_check, include/constraint.hpp:99-101),
+ * i.e. the min-d label of t's Pareto set under the non-strict dominance of
+ * c_label::dominate (include/label.hpp:234-236), together with one path that
+ * realises it.  cost(s, s) = 0 (the source sits in its own Pareto set, as
+ * ck_init does, algorithm/cspp.cuh:63-67).  Weights are integer-valued float32;
+ * results are exact while every path sum stays below 2^24.
+ *
+ * Errors: every entry point returns a vine_status; vine_last_error() gives a
+ * thread-local message.  Threading: separate solver handles are independent;
+ * one handle must not be used by two threads at once.
+ */
+#ifndef VINE_H
+#define VINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  VINE_OK = 0,
+  VINE_E_ARG = -1,      /* invalid argument (bad id, size, pointer, option) */
+  VINE_E_OOM = -2,      /* device or host allocation failed */
+  VINE_E_HIP = -3,      /* HIP runtime error */
+  VINE_E_OVERFLOW = -4, /* a device capacity (labels, candidates, pool) was exceeded */
+  VINE_E_RANGE = -5     /* a path sum reached 2^24, exactness no longer guaranteed */
+} vine_status;
+
+/* Pruning modes.  All return the same (cost, delay, feasible). */
+enum {
+  VINE_PRUNE_NONE = 0,   /* reference one-to-all search: full Pareto sets at every vertex */
+  VINE_PRUNE_TARGET = 1, /* drop labels lexicographically no better than t's best */
+  VINE_PRUNE_BOUNDS = 2  /* lower bounds on remaining len/delay to t (reverse SSSP on the GPU) */
+};
+
+/* Bucket-width (frontier window) policies.  Scheduling only; never change answers. */
+enum {
+  VINE_DELTA_INF = 0,   /* expand the whole frontier every round */
+  VINE_DELTA_FIXED = 1, /* expand labels with key <= min key + delta_fixed */
+  VINE_DELTA_AR = 2,    /* the reference's DOP rule driven by an AR(3) model of log2 frontier size */
+  VINE_DELTA_MLP = 3    /* per-query window predicted by the on-device MLP (vine_solver_set_mlp) */
+};
+
+typedef struct vine_graph vine_graph;
+typedef struct vine_solver vine_solver;
+
+typedef struct {
+  int32_t max_batch;        /* query slots solved together; 1 = plain single-query solver */
+  int32_t prune;            /* VINE_PRUNE_* */
+  int32_t delta_policy;     /* VINE_DELTA_* */
+  float delta_fixed;        /* window for VINE_DELTA_FIXED */
+  int32_t use_graph;        /* capture rounds in hipGraphs (batched serving) */
+  int32_t rounds_per_sync;  /* rounds between host checks of the frontier size */
+  int32_t block_size;       /* threads per workgroup (multiple of 64) */
+  int32_t grid_blocks;      /* workgroups per round kernel (0 = auto) */
+  int32_t hash_slots;       /* LDS hash slots per workgroup for candidate pre-merge (0 = off) */
+  int64_t label_capacity;   /* label log entries (0 = auto from free memory) */
+  int64_t cand_capacity;    /* candidate entries per round (0 = auto) */
+  int64_t pool_capacity;    /* Pareto-set pool entries (0 = auto) */
+  int64_t frontier_capacity;/* carried-frontier entries (0 = auto) */
+  double ar_max_labels;     /* MAX for the DOP rule (0 = free device memory / 12 B, as the reference) */
+  int32_t record_trace;     /* keep per-round frontier statistics (for vine_solver_trace) */
+  int32_t time_kernels;     /* bracket every round kernel with HIP events (vine_stats.ms_*) */
+  int32_t reserved[6];
+} vine_options;
+
+typedef struct {
+  int64_t rounds;           /* CSP rounds, summed over batches */
+  int64_t sssp_rounds;      /* bound-computation rounds (VINE_PRUNE_BOUNDS) */
+  int64_t expanded;         /* labels expanded (X) */
+  int64_t relaxed;          /* arcs relaxed (E) */
+  int64_t candidates;       /* candidates that survived the read-only dominance check */
+  int64_t inserted;         /* labels inserted into Pareto sets (I) */
+  int64_t sssp_relaxed;     /* arcs relaxed by the bound computation */
+  int64_t max_frontier;     /* largest frontier seen */
+  int64_t batches;
+  double ms_total;          /* wall time of the solve (host clock) */
+  double ms_kernels;        /* summed HIP-event time of the round kernels */
+  double ms_expand;         /* HIP-event time of the expand kernels */
+  double ms_merge;          /* HIP-event time of the merge kernels */
+  double ms_sssp;           /* HIP-event time of the bound kernels */
+  int64_t expand_launches;
+  int64_t reserved[8];
+} vine_stats;
+
+void vine_options_default(vine_options* opt);
+const char* vine_last_error(void);
+/* Number of devices visible; negative vine_status on runtime error. */
+int32_t vine_device_count(void);
+
+vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, const int32_t* dst,
+                              const float* len, const float* delay, vine_graph** out);
+vine_status vine_graph_destroy(vine_graph* g);
+
+vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vine_solver** out);
+vine_status vine_solver_destroy(vine_solver* s);
+
+/* Host buffers.  out_path_off has nq+1 entries; path i is
+ * out_path[out_path_off[i] .. out_path_off[i+1]) (vertex ids s..t; empty if
+ * infeasible).  Paths are skipped when out_path is NULL; if path_cap is too
+ * small VINE_E_OVERFLOW is returned after costs are written. */
+vine_status vine_solve(vine_solver* s, int32_t nq, const int32_t* src, const int32_t* dst,
+                       const float* bound, float* out_cost, float* out_delay,
+                       int32_t* out_feasible, int64_t* out_path_off, int32_t* out_path,
+                       int64_t path_cap, vine_stats* out_stats);
+
+/* Device buffers (HBM), no paths.  Costs/delays are +inf for infeasible queries. */
+vine_status vine_solve_device(vine_solver* s, int32_t nq, const int32_t* d_src, const int32_t* d_dst,
+                              const float* d_bound, float* d_cost, float* d_delay,
+                              int32_t* d_feasible, vine_stats* out_stats);
+
+/* Per-query finish time in microseconds from the start of its batch (last
+ * solve), measured with the device wall clock. */
+vine_status vine_solver_latency(const vine_solver* s, int32_t nq, double* out_us);
+
+/* Per-round trace of the last solve (record_trace = 1): for each round r,
+ * rows[r*8 .. r*8+8) = {query slot, round, frontier size, expanded, min key,
+ * max key, delta used, candidates}.  Returns the number of rows written. */
+int64_t vine_solver_trace(const vine_solver* s, double* rows, int64_t max_rows);
+
+/* MLP Delta policy weights (8 -> 32 -> 32 -> 2, fp32, row-major W[out][in]):
+ * w1[32*8], b1[32], w2[32*32], b2[32], w3[2*32], b3[2], plus input mean/std
+ * normalisation xmu[8], xsd[8].  Copied to the device. */
+vine_status vine_solver_set_mlp(vine_solver* s, const float* params, int64_t nparams);
+
+/* Device-resident Pareto set of vertex v for batch slot q after the last
+ * batch of the last solve (VINE_PRUNE_NONE gives the reference's full sets).
+ * Writes up to cap (d, delay) pairs, returns the set size or a negative status. */
+int64_t vine_solver_pareto(const vine_solver* s, int32_t q, int32_t v, float* out_pairs, int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VINE_H */

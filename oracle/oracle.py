@@ -1,0 +1,155 @@
+"""Python bindings for the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker (or the timed CPU
+baseline), never by the product path.
+
+oracle_cspp restates RUN_CSPP_CPU (algorithm/cspp_cpu.hpp:44-125) with the
+additive expand of c_label::expand (include/label.hpp:213-222).
+"""
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_AR = os.path.join(HERE, "_ref", "libar_ref.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        vp, i32, f32, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_double
+        L.oracle_cspp.restype = i32
+        L.oracle_cspp.argtypes = [i32, vp, vp, vp, vp, i32, i32, f32, i32, vp, vp, vp, i32, vp, vp, i32,
+                                  vp, vp, vp, i32]
+        L.oracle_dijkstra_lex.restype = i32
+        L.oracle_dijkstra_lex.argtypes = [i32, vp, vp, vp, vp, i32, vp, vp]
+        L.oracle_resource_dp.restype = i32
+        L.oracle_resource_dp.argtypes = [i32, vp, vp, vp, vp, i32, i32, i32, vp, vp]
+        L.oracle_bruteforce.restype = i32
+        L.oracle_bruteforce.argtypes = [i32, vp, vp, vp, vp, i32, i32, f32, vp, vp]
+        L.oracle_ar_burg.restype = i32
+        L.oracle_ar_burg.argtypes = [vp, i32, i32, vp]
+        L.oracle_ar_lsq.restype = i32
+        L.oracle_ar_lsq.argtypes = [vp, i32, i32, vp]
+        _lib = L
+    return _lib
+
+
+@dataclass
+class Result:
+    feasible: bool
+    cost: float
+    delay: float
+    path: np.ndarray
+    pareto: np.ndarray  # (k, 2) float32, sorted by d
+    counters: dict
+    trace: np.ndarray = None
+
+
+COUNTER_NAMES = ("expanded", "relaxed", "inserted", "rounds", "max_frontier", "sum_frontier")
+
+
+def _g(g):
+    return (g.n, g.row_ptr.ctypes.data, g.dst.ctypes.data, g.len.ctypes.data, g.delay.ctypes.data)
+
+
+def cspp(g, s, t, bound, prune_target=False, path_cap=None, pareto_cap=4096, trace_cap=0):
+    """Solve one query; returns Result."""
+    path_cap = g.n if path_cap is None else path_cap
+    cost = ctypes.c_float()
+    dly = ctypes.c_float()
+    path = np.zeros(max(path_cap, 1), np.int32)
+    plen = ctypes.c_int32()
+    par = np.zeros(2 * pareto_cap, np.float32)
+    parl = ctypes.c_int32()
+    cnt = np.zeros(6, np.int64)
+    trace = np.zeros(max(trace_cap, 1), np.int32)
+    rc = lib().oracle_cspp(*_g(g), int(s), int(t), float(bound), 1 if prune_target else 0,
+                           ctypes.byref(cost), ctypes.byref(dly), path.ctypes.data, path_cap,
+                           ctypes.byref(plen), par.ctypes.data, pareto_cap, ctypes.byref(parl),
+                           cnt.ctypes.data, trace.ctypes.data if trace_cap else None, trace_cap)
+    if rc < 0:
+        raise RuntimeError("oracle_cspp failed (%d)" % rc)
+    k = min(parl.value, pareto_cap)
+    counters = dict(zip(COUNTER_NAMES, (int(x) for x in cnt)))
+    tr = trace[:min(counters["rounds"], trace_cap)].copy() if trace_cap else None
+    return Result(rc == 1, cost.value, dly.value,
+                  path[:plen.value].copy() if plen.value <= path_cap else None,
+                  par[:2 * k].reshape(k, 2).copy(), counters, tr)
+
+
+def dijkstra_lex(g, s, w1="len", w2="delay"):
+    a = getattr(g, w1) if isinstance(w1, str) else w1
+    b = getattr(g, w2) if isinstance(w2, str) else w2
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    d1 = np.zeros(g.n, np.float64)
+    d2 = np.zeros(g.n, np.float64)
+    rc = lib().oracle_dijkstra_lex(g.n, g.row_ptr.ctypes.data, g.dst.ctypes.data, a.ctypes.data,
+                                   b.ctypes.data, int(s), d1.ctypes.data, d2.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("oracle_dijkstra_lex failed")
+    return d1, d2
+
+
+def resource_dp(g, s, t, bound):
+    c = ctypes.c_double()
+    d = ctypes.c_double()
+    rc = lib().oracle_resource_dp(*_g(g), int(s), int(t), int(bound), ctypes.byref(c), ctypes.byref(d))
+    if rc < 0:
+        raise RuntimeError("oracle_resource_dp failed (%d)" % rc)
+    return rc == 1, c.value, d.value
+
+
+def bruteforce(g, s, t, bound):
+    c = ctypes.c_double()
+    d = ctypes.c_double()
+    rc = lib().oracle_bruteforce(*_g(g), int(s), int(t), float(bound), ctypes.byref(c), ctypes.byref(d))
+    if rc < 0:
+        raise RuntimeError("oracle_bruteforce failed (%d)" % rc)
+    return rc == 1, c.value, d.value
+
+
+def ar_burg(series, degree=3):
+    x = np.ascontiguousarray(series, np.float64)
+    c = np.zeros(degree, np.float64)
+    if lib().oracle_ar_burg(x.ctypes.data, x.shape[0], degree, c.ctypes.data) != 0:
+        raise ValueError("oracle_ar_burg: bad arguments")
+    return c
+
+
+def ar_lsq(series, degree=3):
+    x = np.ascontiguousarray(series, np.float64)
+    c = np.zeros(degree, np.float64)
+    if lib().oracle_ar_lsq(x.ctypes.data, x.shape[0], degree, c.ctypes.data) != 0:
+        raise ValueError("oracle_ar_lsq failed")
+    return c
+
+
+def ref_ar(series, degree=3, method=0):
+    """The reference's own AR::AutoRegression, compiled from include/AR.cpp into
+    oracle/_ref/libar_ref.so (None if that build is absent)."""
+    if not os.path.exists(REF_AR):
+        return None
+    L = ctypes.CDLL(REF_AR)
+    fn = getattr(L, "_ZN2AR14AutoRegressionEPdiiS0_i")
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+    x = np.ascontiguousarray(series, np.float64).copy()
+    c = np.zeros(degree, np.float64)
+    fn(x.ctypes.data, x.shape[0], degree, c.ctypes.data, method)
+    return c

@@ -1,0 +1,133 @@
+"""Parity of the HIP solver with the CPU oracle (bit-exact costs, delays,
+feasibility; valid paths; in the reference's one-to-all mode also the full
+Pareto set at t)."""
+import numpy as np
+import pytest
+
+from helpers import check_path
+from oracle import oracle as O
+from vine_amd import solver as S
+from vine_amd import workload as W
+
+
+@pytest.fixture(scope="module")
+def grid_case():
+    g, q = W.preset("grid64")
+    ref = [O.cspp(g, q.src[i], q.dst[i], q.bound[i]) for i in range(len(q))]
+    return g, q, ref
+
+
+def _compare(g, q, ref, ans, idx=None):
+    idx = range(len(q)) if idx is None else idx
+    for k, i in enumerate(idx):
+        r = ref[i]
+        assert bool(ans.feasible[k]) == r.feasible, (i, ans.feasible[k], r.feasible)
+        if r.feasible:
+            assert ans.cost[k] == np.float32(r.cost), (i, ans.cost[k], r.cost)
+            assert ans.delay[k] == np.float32(r.delay), (i, ans.delay[k], r.delay)
+            check_path(g, ans.path_of(k), q.src[i], q.dst[i], ans.cost[k], ans.delay[k], q.bound[i])
+        else:
+            assert np.isinf(ans.cost[k])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prune", [0, 1, 2])
+@pytest.mark.parametrize("batch", [1, 100])
+def test_grid_parity(gpu, grid_case, prune, batch):
+    g, q, ref = grid_case
+    print("grid64 parity prune=%d batch=%d" % (prune, batch), flush=True)
+    with S.Solver(g, max_batch=batch, prune=prune) as s:
+        ans = s.solve(q.src, q.dst, q.bound)
+        _compare(g, q, ref, ans)
+        if prune == 0 and batch == 100:
+            # reference semantics: the full Pareto set at t is the oracle's
+            for i in range(len(q)):
+                got = s.pareto(i, q.dst[i])
+                exp = ref[i].pareto
+                assert got.shape == exp.shape, (i, got.shape, exp.shape)
+                assert np.array_equal(got, exp), i
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("policy,delta", [(S.DELTA_FIXED, 0.0), (S.DELTA_FIXED, 2000.0), (S.DELTA_AR, 0.0)])
+def test_grid_windows(gpu, grid_case, policy, delta):
+    g, q, ref = grid_case
+    print("grid64 windows policy=%d delta=%g" % (policy, delta), flush=True)
+    with S.Solver(g, max_batch=25, prune=2, delta_policy=policy, delta_fixed=delta) as s:
+        ans = s.solve(q.src, q.dst, q.bound)
+    _compare(g, q, ref, ans)
+
+
+def _tiny_graphs(count, seed):
+    rng = np.random.default_rng(seed)
+    for _ in range(count):
+        n = int(rng.integers(2, 9))
+        m = int(rng.integers(1, 3 * n))
+        u = rng.integers(0, n, m)
+        v = rng.integers(0, n, m)
+        keep = u != v
+        u, v = u[keep], v[keep]
+        order = np.argsort(u, kind="stable")
+        u, v = u[order], v[order]
+        row = np.zeros(n + 1, np.int32)
+        np.add.at(row, u + 1, 1)
+        row = np.cumsum(row).astype(np.int32)
+        ln = rng.integers(0, 10, u.shape[0]).astype(np.float32)
+        dl = rng.integers(1, 10, u.shape[0]).astype(np.float32)
+        yield W.Graph(row, v.astype(np.int32), ln, dl, "tiny")
+
+
+@pytest.mark.gpu
+def test_tiny_bruteforce(gpu):
+    """Random tiny graphs (parallel arcs, zero lengths, cycles): every prune mode
+    against exhaustive enumeration of simple paths."""
+    rng = np.random.default_rng(1)
+    for gi, g in enumerate(_tiny_graphs(40, 7)):
+        nq = 12
+        src = rng.integers(0, g.n, nq).astype(np.int32)
+        dst = rng.integers(0, g.n, nq).astype(np.int32)
+        bound = rng.integers(0, 25, nq).astype(np.float32)
+        exp = [O.bruteforce(g, src[i], dst[i], bound[i]) for i in range(nq)]
+        for prune in (0, 1, 2):
+            with S.Solver(g, max_batch=5, prune=prune) as s:
+                ans = s.solve(src, dst, bound)
+            for i in range(nq):
+                f, c, d = exp[i]
+                assert bool(ans.feasible[i]) == f, (gi, prune, i)
+                if f:
+                    assert (float(ans.cost[i]), float(ans.delay[i])) == (c, d), (gi, prune, i)
+                    check_path(g, ans.path_of(i), src[i], dst[i], ans.cost[i], ans.delay[i], bound[i])
+
+
+@pytest.fixture(scope="module")
+def rmat_case():
+    g = W.rmat(scale=12, edge_factor=8)
+    q = W.random_queries(g, 40, (13, 10), lcc_only=True, seed=3)
+    ref = [O.cspp(g, q.src[i], q.dst[i], q.bound[i], prune_target=2) for i in range(len(q))]
+    return g, q, ref
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prune", [0, 2])
+def test_rmat_hubs(gpu, rmat_case, prune):
+    """Power-law graph: hub vertices with hundreds of arcs (the reference's
+    `break` on an infeasible arc, algorithm/bucket.cuh:76-78, would skip
+    strided arcs here)."""
+    g, q, ref = rmat_case
+    assert np.diff(g.row_ptr).max() > 64
+    with S.Solver(g, max_batch=8, prune=prune) as s:
+        ans = s.solve(q.src, q.dst, q.bound)
+    _compare(g, q, ref, ans)
+
+
+@pytest.mark.gpu
+def test_knn_sample_bounds(gpu):
+    """Geometric k-NN graph (the road-like generator) at 20k vertices with the
+    lower-bound pruning, whole-frontier and windowed schedules."""
+    g = W.knn(20000, seed=5)
+    q = W.hop_queries(g, 30, hop_lo=5, hop_hi=80, f_permille=(1050, 1500), seed=5)
+    ref = [O.cspp(g, q.src[i], q.dst[i], q.bound[i], prune_target=2) for i in range(len(q))]
+    for pol, dlt in ((S.DELTA_INF, 0.0), (S.DELTA_FIXED, 500.0)):
+        with S.Solver(g, max_batch=16, prune=2, delta_policy=pol, delta_fixed=dlt) as s:
+            ans = s.solve(q.src, q.dst, q.bound)
+        _compare(g, q, ref, ans)

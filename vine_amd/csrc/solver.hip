@@ -1,0 +1,902 @@
+// Host orchestration and C ABI of the exact CSP solver (libvine.so).
+//
+// Counterpart of the reference's RUN_CSPP host driver (algorithm/cspp.cuh:69-325)
+// and solver facade (algorithm/cspp.hpp:11-69), re-designed for one MI355X:
+//  - device memory is owned by the solver handle and sized once, not
+//    cudaMallocManaged per call (algorithm/cspp.cuh:82-114);
+//  - per-batch reset touches only per-(query, vertex) headers, not O(N*128)
+//    label arrays (algorithm/cspp.cuh:142-154);
+//  - many queries share every round launch (max_batch query slots), with
+//    per-query bounds (the reference uses C_l[0] for all, cspp.cuh:196,234);
+//  - the frontier size stays on the device; the host polls it every
+//    rounds_per_sync rounds (the reference syncs 5-6 times per round).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sched.hpp"
+#include "solver_kernels.hpp"
+#include "vine.h"
+
+using namespace vine;
+using vine::dev::Counters;
+using vine::dev::Ctx;
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_err("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return VINE_E_HIP;                                                           \
+    }                                                                              \
+  } while (0)
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct vine_graph {
+  int32_t n = 0;
+  int64_t m = 0;
+  int device = 0;
+  int32_t* d_row = nullptr;
+  int4* d_edges = nullptr;
+  int32_t* d_rrow = nullptr;
+  int4* d_redges = nullptr;
+};
+
+struct vine_solver {
+  const vine_graph* g = nullptr;
+  vine_options opt{};
+  int B = 1;
+  int grid = 1024;
+  int block = 256;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+
+  // per query slot
+  int32_t *d_qsrc = nullptr, *d_qdst = nullptr;
+  float *d_qbound = nullptr, *d_qdelta = nullptr;
+  float2* d_qbest = nullptr;
+  int32_t* d_qbest_id = nullptr;
+  uint32_t* d_qfmin = nullptr;
+  int32_t* d_qlast = nullptr;
+  float* d_qfmax = nullptr;
+  int32_t* d_qnfront = nullptr;
+  // per (query, vertex)
+  int2* d_pset = nullptr;
+  int32_t* d_cand_head = nullptr;
+  float *d_lbl = nullptr, *d_lbd = nullptr;
+  int32_t* d_stamp = nullptr;
+  // pools
+  float2* d_pool_cd = nullptr;
+  int32_t* d_pool_id = nullptr;
+  int4* d_labs = nullptr;
+  int2* d_lmeta = nullptr;
+  uint8_t* d_dead = nullptr;
+  float2* d_cand = nullptr;
+  int32_t *d_cand_par = nullptr, *d_cand_next = nullptr;
+  long long* d_touched = nullptr;
+  int32_t* d_carry[2] = {nullptr, nullptr};
+  long long* d_sfr[2] = {nullptr, nullptr};
+  unsigned long long* d_ts = nullptr;
+  Counters* d_ct = nullptr;
+  Counters* h_ct = nullptr;  // pinned
+  // outputs for the host-buffer path
+  float *d_out_cost = nullptr, *d_out_delay = nullptr;
+  int32_t *d_out_feas = nullptr, *d_out_plen = nullptr;
+  int32_t* d_in_q = nullptr;  // staging for host query arrays (src, dst)
+  float* d_in_bound = nullptr;
+  int32_t in_cap = 0;
+  // MLP parameters (device)
+  float* d_mlp = nullptr;
+  int64_t mlp_nparams = 0;
+
+  Ctx ctx{};
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+
+  // last-solve records
+  std::vector<double> latency_us;
+  std::vector<double> trace;
+  int last_batch_q = 0;
+};
+
+static vine_status dev_alloc(vine_solver* s, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    set_err("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    (void)hipGetLastError();
+    return VINE_E_OOM;
+  }
+  s->allocs.push_back(*p);
+  return VINE_OK;
+}
+
+#define ALLOC(ptr, bytes)                                                 \
+  do {                                                                    \
+    vine_status st_ = dev_alloc(s, (void**)&(ptr), (size_t)(bytes));       \
+    if (st_ != VINE_OK) {                                                 \
+      vine_solver_destroy(s);                                             \
+      return st_;                                                         \
+    }                                                                     \
+  } while (0)
+
+extern "C" {
+
+void vine_options_default(vine_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->max_batch = 1;
+  o->prune = VINE_PRUNE_BOUNDS;
+  o->delta_policy = VINE_DELTA_INF;
+  o->delta_fixed = 0.f;
+  o->use_graph = 0;
+  o->rounds_per_sync = 1;
+  o->block_size = 256;
+  o->grid_blocks = 0;
+  o->hash_slots = 0;
+  o->ar_max_labels = 0.0;
+}
+
+const char* vine_last_error(void) { return g_err.c_str(); }
+
+int32_t vine_device_count(void) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    set_err("hipGetDeviceCount: %s", hipGetErrorString(e));
+    return VINE_E_HIP;
+  }
+  return n;
+}
+
+vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, const int32_t* dst,
+                              const float* len, const float* delay, vine_graph** out) {
+  if (!out || n <= 0 || m < 0 || !row_ptr || (m > 0 && (!dst || !len || !delay))) {
+    set_err("vine_graph_create: invalid arguments");
+    return VINE_E_ARG;
+  }
+  if (m >= (int64_t)INT32_MAX) {
+    set_err("vine_graph_create: %lld arcs exceed the 32-bit CSR offsets", (long long)m);
+    return VINE_E_ARG;
+  }
+  if (row_ptr[0] != 0 || row_ptr[n] != m) {
+    set_err("vine_graph_create: row_ptr must start at 0 and end at m");
+    return VINE_E_ARG;
+  }
+  for (int32_t i = 0; i < n; i++)
+    if (row_ptr[i + 1] < row_ptr[i]) {
+      set_err("vine_graph_create: row_ptr not monotone at %d", i);
+      return VINE_E_ARG;
+    }
+  // packed records {dst, len, delay, 0}; transposed copy for the bound search
+  std::vector<int4> e(std::max<int64_t>(m, 1)), re(std::max<int64_t>(m, 1));
+  std::vector<int32_t> rrow(n + 1, 0);
+  for (int64_t p = 0; p < m; p++) {
+    if (dst[p] < 0 || dst[p] >= n) {
+      set_err("vine_graph_create: arc %lld has invalid head %d", (long long)p, dst[p]);
+      return VINE_E_ARG;
+    }
+    if (!(len[p] >= 0.f) || !(delay[p] >= 0.f)) {
+      set_err("vine_graph_create: arc %lld has a negative or NaN weight", (long long)p);
+      return VINE_E_ARG;
+    }
+    rrow[dst[p] + 1]++;
+  }
+  for (int32_t v = 0; v < n; v++) rrow[v + 1] += rrow[v];
+  std::vector<int32_t> fill(rrow.begin(), rrow.end() - 1);
+  for (int32_t u = 0; u < n; u++)
+    for (int32_t p = row_ptr[u]; p < row_ptr[u + 1]; p++) {
+      int4 rec;
+      rec.x = dst[p];
+      std::memcpy(&rec.y, &len[p], 4);
+      std::memcpy(&rec.z, &delay[p], 4);
+      rec.w = 0;
+      e[p] = rec;
+      int4 rr = rec;
+      rr.x = u;
+      re[fill[dst[p]]++] = rr;
+    }
+  auto* g = new vine_graph();
+  g->n = n;
+  g->m = m;
+  hipError_t err = hipGetDevice(&g->device);
+  size_t eb = sizeof(int4) * std::max<int64_t>(m, 1);
+  if (err == hipSuccess) err = hipMalloc(&g->d_row, sizeof(int32_t) * (n + 1));
+  if (err == hipSuccess) err = hipMalloc(&g->d_rrow, sizeof(int32_t) * (n + 1));
+  if (err == hipSuccess) err = hipMalloc(&g->d_edges, eb);
+  if (err == hipSuccess) err = hipMalloc(&g->d_redges, eb);
+  if (err == hipSuccess) err = hipMemcpy(g->d_row, row_ptr, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(g->d_rrow, rrow.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(g->d_edges, e.data(), eb, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(g->d_redges, re.data(), eb, hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    set_err("vine_graph_create: %s", hipGetErrorString(err));
+    vine_graph_destroy(g);
+    return err == hipErrorOutOfMemory ? VINE_E_OOM : VINE_E_HIP;
+  }
+  *out = g;
+  return VINE_OK;
+}
+
+vine_status vine_graph_destroy(vine_graph* g) {
+  if (!g) return VINE_OK;
+  if (g->d_row) (void)hipFree(g->d_row);
+  if (g->d_rrow) (void)hipFree(g->d_rrow);
+  if (g->d_edges) (void)hipFree(g->d_edges);
+  if (g->d_redges) (void)hipFree(g->d_redges);
+  delete g;
+  return VINE_OK;
+}
+
+vine_status vine_solver_destroy(vine_solver* s) {
+  if (!s) return VINE_OK;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (void* p : s->allocs) (void)hipFree(p);
+  if (s->h_ct) (void)hipHostFree(s->h_ct);
+  for (auto& e : s->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return VINE_OK;
+}
+
+vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vine_solver** out) {
+  if (!g || !out) {
+    set_err("vine_solver_create: null argument");
+    return VINE_E_ARG;
+  }
+  vine_options o;
+  if (opt) o = *opt;
+  else vine_options_default(&o);
+  if (o.max_batch < 1 || o.prune < 0 || o.prune > 2 || o.delta_policy < 0 || o.delta_policy > 3 ||
+      o.block_size < 64 || o.block_size > 1024 || o.block_size % 64 || o.rounds_per_sync < 1 ||
+      o.grid_blocks < 0 || o.hash_slots < 0 || !(o.delta_fixed >= 0.f)) {
+    set_err("vine_solver_create: invalid options");
+    return VINE_E_ARG;
+  }
+  auto* s = new vine_solver();
+  s->g = g;
+  s->opt = o;
+  s->B = o.max_batch;
+  s->block = o.block_size;
+  s->grid = o.grid_blocks ? o.grid_blocks : 1024;
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    set_err("hipStreamCreate: %s", hipGetErrorString(e));
+    delete s;
+    return VINE_E_HIP;
+  }
+  for (auto& ev : s->ev)
+    if (hipEventCreate(&ev) != hipSuccess) {
+      set_err("hipEventCreate failed");
+      vine_solver_destroy(s);
+      return VINE_E_HIP;
+    }
+  const long long N = g->n, B = s->B, BN = N * B;
+  size_t freeb = 0, totalb = 0;
+  if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
+    set_err("hipMemGetInfo failed");
+    vine_solver_destroy(s);
+    return VINE_E_HIP;
+  }
+  // per-(query, vertex) state
+  size_t per_qv = sizeof(int2) + sizeof(int32_t) + (o.prune == 2 ? 3 * sizeof(float) : 0);
+  double remaining = (double)freeb - (double)per_qv * BN - 64.0 * (1 << 20);
+  if (remaining <= 0) {
+    set_err("vine_solver_create: %lld query slots x %lld vertices need %.1f GB, %.1f GB free",
+            B, N, (double)per_qv * BN / 1e9, (double)freeb / 1e9);
+    vine_solver_destroy(s);
+    return VINE_E_OOM;
+  }
+  // auto capacities: share the remaining memory (labels 25 B, pool 12 B x 2,
+  // candidates 24 B, carry 8 B, bound frontier 16 B per entry)
+  long long lab_cap = o.label_capacity;
+  if (!lab_cap) {
+    // sized to the problem (64 labels per (query, vertex) pair plus slack),
+    // capped by 60% of the free memory
+    double per = 25.0 + 2 * 12.0 + 0.5 * 24.0 + 0.5 * 8.0 + 0.25 * 16.0;
+    double want = 64.0 * (double)BN + (double)(1 << 22);
+    lab_cap = (long long)std::min(std::min(remaining * 0.6 / per, want), (double)INT32_MAX - 1);
+    lab_cap = std::max<long long>(lab_cap, 4 * B + 1024);
+  }
+  if (lab_cap > INT32_MAX - 1 || lab_cap < B + 1) {
+    set_err("vine_solver_create: label_capacity must be in [max_batch+1, 2^31-1)");
+    vine_solver_destroy(s);
+    return VINE_E_ARG;
+  }
+  long long pool_cap = o.pool_capacity ? o.pool_capacity : std::min<long long>(2 * lab_cap + 4 * B, 0xFFFFFFFFll);
+  long long cand_cap = o.cand_capacity ? o.cand_capacity : std::min<long long>(lab_cap / 2 + 1024, INT32_MAX - 1);
+  long long carry_cap = o.frontier_capacity ? o.frontier_capacity : std::min<long long>(lab_cap / 2 + 1024, INT32_MAX - 1);
+  long long sf_cap = std::min<long long>(std::max<long long>(BN, 1024), std::min<long long>(lab_cap / 4 + 1024, INT32_MAX - 1));
+  if (pool_cap < 4 * B || pool_cap > 0xFFFFFFFFll || cand_cap < 64 || cand_cap > INT32_MAX - 1 ||
+      carry_cap < 64 || carry_cap > INT32_MAX - 1) {
+    set_err("vine_solver_create: capacities out of range");
+    vine_solver_destroy(s);
+    return VINE_E_ARG;
+  }
+  ALLOC(s->d_qsrc, sizeof(int32_t) * B);
+  ALLOC(s->d_qdst, sizeof(int32_t) * B);
+  ALLOC(s->d_qbound, sizeof(float) * B);
+  ALLOC(s->d_qdelta, sizeof(float) * B);
+  ALLOC(s->d_qbest, sizeof(float2) * B);
+  ALLOC(s->d_qbest_id, sizeof(int32_t) * B);
+  ALLOC(s->d_qfmin, sizeof(uint32_t) * 2 * B);
+  ALLOC(s->d_qlast, sizeof(int32_t) * B);
+  ALLOC(s->d_qfmax, sizeof(float) * B);
+  ALLOC(s->d_qnfront, sizeof(int32_t) * B);
+  ALLOC(s->d_pset, sizeof(int2) * BN);
+  ALLOC(s->d_cand_head, sizeof(int32_t) * BN);
+  if (o.prune == 2) {
+    ALLOC(s->d_lbl, sizeof(float) * BN);
+    ALLOC(s->d_lbd, sizeof(float) * BN);
+    ALLOC(s->d_stamp, sizeof(int32_t) * BN);
+    ALLOC(s->d_sfr[0], sizeof(long long) * sf_cap);
+    ALLOC(s->d_sfr[1], sizeof(long long) * sf_cap);
+  }
+  ALLOC(s->d_pool_cd, sizeof(float2) * pool_cap);
+  ALLOC(s->d_pool_id, sizeof(int32_t) * pool_cap);
+  ALLOC(s->d_labs, sizeof(int4) * lab_cap);
+  ALLOC(s->d_lmeta, sizeof(int2) * lab_cap);
+  ALLOC(s->d_dead, lab_cap);
+  ALLOC(s->d_cand, sizeof(float2) * cand_cap);
+  ALLOC(s->d_cand_par, sizeof(int32_t) * cand_cap);
+  ALLOC(s->d_cand_next, sizeof(int32_t) * cand_cap);
+  ALLOC(s->d_touched, sizeof(long long) * cand_cap);
+  ALLOC(s->d_carry[0], sizeof(int32_t) * carry_cap);
+  ALLOC(s->d_carry[1], sizeof(int32_t) * carry_cap);
+  ALLOC(s->d_ts, sizeof(unsigned long long) * dev::TS_CAP);
+  ALLOC(s->d_ct, sizeof(Counters));
+  if (hipHostMalloc((void**)&s->h_ct, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
+    set_err("hipHostMalloc failed");
+    vine_solver_destroy(s);
+    return VINE_E_OOM;
+  }
+  Ctx& c = s->ctx;
+  c.n = g->n;
+  c.B = s->B;
+  c.row_ptr = g->d_row;
+  c.edges = g->d_edges;
+  c.rrow_ptr = g->d_rrow;
+  c.redges = g->d_redges;
+  c.q_src = s->d_qsrc;
+  c.q_dst = s->d_qdst;
+  c.q_bound = s->d_qbound;
+  c.q_delta = s->d_qdelta;
+  c.q_best = s->d_qbest;
+  c.q_best_id = s->d_qbest_id;
+  c.q_fmin = s->d_qfmin;
+  c.q_last = s->d_qlast;
+  c.q_fmax = s->d_qfmax;
+  c.q_nfront = s->d_qnfront;
+  c.pset = s->d_pset;
+  c.cand_head = s->d_cand_head;
+  c.lbl = s->d_lbl;
+  c.lbd = s->d_lbd;
+  c.stamp = s->d_stamp;
+  c.pool_cd = s->d_pool_cd;
+  c.pool_id = s->d_pool_id;
+  c.pool_cap = (unsigned long long)pool_cap;
+  c.labs = s->d_labs;
+  c.lmeta = s->d_lmeta;
+  c.dead = s->d_dead;
+  c.label_cap = lab_cap;
+  c.cand = s->d_cand;
+  c.cand_par = s->d_cand_par;
+  c.cand_next = s->d_cand_next;
+  c.touched = s->d_touched;
+  c.cand_cap = (int32_t)cand_cap;
+  c.carry[0] = s->d_carry[0];
+  c.carry[1] = s->d_carry[1];
+  c.carry_cap = (int32_t)carry_cap;
+  c.sfr[0] = s->d_sfr[0];
+  c.sfr[1] = s->d_sfr[1];
+  c.sf_cap = (int32_t)sf_cap;
+  c.ts = s->d_ts;
+  c.ct = s->d_ct;
+  c.hash_slots = o.hash_slots;
+  *out = s;
+  return VINE_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Timer {
+  vine_solver* s;
+  bool on;
+  double* acc;
+  Timer(vine_solver* s_, double* acc_) : s(s_), on(s_->opt.time_kernels != 0), acc(acc_) {
+    if (on) (void)hipEventRecord(s->ev[0], s->stream);
+  }
+  hipError_t stop() {
+    if (!on) return hipSuccess;
+    hipError_t e = hipEventRecord(s->ev[1], s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(s->ev[1]);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, s->ev[0], s->ev[1]);
+    *acc += ms;
+    return e;
+  }
+};
+
+vine_status poll(vine_solver* s) {
+  HIP_TRY(hipMemcpyAsync(s->h_ct, s->d_ct, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->h_ct->overflow) {
+    set_err("device capacity exceeded (flags 0x%x: 1 labels, 2 candidates, 4 touched, 8 pool, "
+            "16 carry, 32 bound frontier); raise the capacities in vine_options",
+            s->h_ct->overflow);
+    return VINE_E_OVERFLOW;
+  }
+  return VINE_OK;
+}
+
+template <int PRUNE>
+vine_status launch_round(vine_solver* s, long long dop, vine_stats* st) {
+  Ctx& c = s->ctx;
+  {
+    Timer t(s, &st->ms_expand);
+    hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, PRUNE>), dim3(s->grid),
+                       dim3(s->block), 0, s->stream, c, dop);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(t.stop());
+  }
+  {
+    Timer t(s, &st->ms_merge);
+    hipLaunchKernelGGL((dev::k_merge<c_label, PRUNE>), dim3(s->grid), dim3(s->block), 0, s->stream, c);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(t.stop());
+  }
+  st->expand_launches++;
+  return VINE_OK;
+}
+
+vine_status launch_round_any(vine_solver* s, long long dop, vine_stats* st) {
+  switch (s->opt.prune) {
+    case 0:
+      return launch_round<0>(s, dop, st);
+    case 1:
+      return launch_round<1>(s, dop, st);
+    default:
+      return launch_round<2>(s, dop, st);
+  }
+}
+
+vine_status fill_f32(vine_solver* s, float* p, float v, long long n) {
+  uint32_t bits;
+  std::memcpy(&bits, &v, 4);
+  int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(dev::k_fill_u32, dim3(blocks), dim3(256), 0, s->stream, (uint32_t*)p, bits, n);
+  HIP_TRY(hipGetLastError());
+  return VINE_OK;
+}
+
+// Bound computation: label-correcting SSSP on the transposed graph.
+vine_status run_sssp(vine_solver* s, int nb, int which, vine_stats* st) {
+  Ctx& c = s->ctx;
+  Timer t(s, &st->ms_sssp);
+  hipLaunchKernelGGL(dev::k_sssp_init, dim3((nb + 255) / 256), dim3(256), 0, s->stream, c, nb, which);
+  HIP_TRY(hipGetLastError());
+  const int K = std::max(1, s->opt.rounds_per_sync);
+  for (;;) {
+    for (int k = 0; k < K; k++) {
+      if (which == 0)
+        hipLaunchKernelGGL(dev::k_sssp<0>, dim3(s->grid), dim3(s->block), 0, s->stream, c);
+      else
+        hipLaunchKernelGGL(dev::k_sssp<1>, dim3(s->grid), dim3(s->block), 0, s->stream, c);
+      HIP_TRY(hipGetLastError());
+      st->sssp_rounds++;
+    }
+    vine_status ps = poll(s);
+    if (ps != VINE_OK) return ps;
+    if (s->h_ct->sf_n[s->h_ct->sround % 3] == 0) break;
+    // label-correcting SSSP settles within n rounds per query; more means a bug
+    if (s->h_ct->sround > 4LL * s->g->n + 256) {
+      set_err("bound computation made no progress after %d rounds", s->h_ct->sround);
+      return VINE_E_OVERFLOW;
+    }
+  }
+  HIP_TRY(t.stop());
+  return VINE_OK;
+}
+
+// One batch of nb <= B queries whose arrays are already in the slot buffers.
+vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
+  Ctx& c = s->ctx;
+  const long long BN = (long long)s->B * c.n;
+  // reset per-(query, vertex) headers and counters
+  HIP_TRY(hipMemsetAsync(s->d_pset, 0, sizeof(int2) * BN, s->stream));
+  HIP_TRY(hipMemsetAsync(s->d_cand_head, 0xFF, sizeof(int32_t) * BN, s->stream));
+  std::memset(s->h_ct, 0, sizeof(Counters));
+  s->h_ct->label_ctr = (unsigned long long)s->B;
+  s->h_ct->pool_ctr = (unsigned long long)s->B * dev::INIT_CAP;
+  HIP_TRY(hipMemcpyAsync(s->d_ct, s->h_ct, sizeof(Counters), hipMemcpyHostToDevice, s->stream));
+  // window per query
+  {
+    float dv = dev::FINF;
+    if (s->opt.delta_policy == VINE_DELTA_FIXED) dv = s->opt.delta_fixed;
+    vine_status fs = fill_f32(s, s->d_qdelta, dv, s->B);
+    if (fs != VINE_OK) return fs;
+  }
+  if (s->opt.prune == 2) {
+    vine_status fs = fill_f32(s, s->d_lbd, dev::FINF, BN);
+    if (fs == VINE_OK) fs = fill_f32(s, s->d_lbl, dev::FINF, BN);
+    if (fs != VINE_OK) return fs;
+    HIP_TRY(hipMemsetAsync(s->d_stamp, 0xFF, sizeof(int32_t) * BN, s->stream));
+    vine_status ss = run_sssp(s, nb, 0, st);
+    if (ss == VINE_OK) ss = run_sssp(s, nb, 1, st);
+    if (ss != VINE_OK) return ss;
+  }
+  switch (s->opt.prune) {
+    case 0:
+      hipLaunchKernelGGL(dev::k_init_batch<0>, dim3((s->B + 255) / 256), dim3(256), 0, s->stream, c, nb);
+      break;
+    case 1:
+      hipLaunchKernelGGL(dev::k_init_batch<1>, dim3((s->B + 255) / 256), dim3(256), 0, s->stream, c, nb);
+      break;
+    default:
+      hipLaunchKernelGGL(dev::k_init_batch<2>, dim3((s->B + 255) / 256), dim3(256), 0, s->stream, c, nb);
+  }
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(dev::k_stamp_start, dim3(1), dim3(1), 0, s->stream, c);
+  HIP_TRY(hipGetLastError());
+
+  // scheduling
+  const bool ar = s->opt.delta_policy == VINE_DELTA_AR;
+  DopRule rule;
+  if (ar) {
+    double mx = s->opt.ar_max_labels;
+    if (mx <= 0) {
+      size_t fb = 0, tb = 0;
+      HIP_TRY(hipMemGetInfo(&fb, &tb));
+      mx = (double)fb / 12.0;  // free memory / sizeof(c_label), algorithm/cspp.cuh:43-47
+    }
+    rule.reset(mx);
+  }
+  const int K = ar ? 1 : std::max(1, s->opt.rounds_per_sync);
+  int64_t rounds = 0;
+  for (;;) {
+    for (int k = 0; k < K; k++) {
+      long long dop = ar ? (long long)rule.dop() : 0;
+      vine_status ls = launch_round_any(s, dop, st);
+      if (ls != VINE_OK) return ls;
+      rounds++;
+    }
+    vine_status ps = poll(s);
+    if (ps != VINE_OK) return ps;
+    const Counters& h = *s->h_ct;
+    int par = h.round_e & 1;
+    long long fn = (long long)h.carry_n[par] + ((long long)h.label_ctr - h.lstart[par]);
+    st->max_frontier = std::max<int64_t>(st->max_frontier, fn);
+    if (s->opt.record_trace) {
+      double row[8] = {-1, (double)rounds, (double)fn, (double)h.expanded, 0, 0,
+                       ar ? (double)rule.dop() : (double)s->opt.delta_fixed, (double)h.cands};
+      s->trace.insert(s->trace.end(), row, row + 8);
+    }
+    if (fn == 0) break;
+    // every round expands at least the minimum-key label of each live query
+    // (or re-keys carried labels once), so rounds are bounded by twice the
+    // number of labels ever created
+    if (rounds > 2 * (int64_t)h.label_ctr + 64) {
+      set_err("solver made no progress after %lld rounds", (long long)rounds);
+      return VINE_E_OVERFLOW;
+    }
+    if (ar) rule.update((int64_t)fn);
+  }
+  if (s->h_ct->range_flag) {
+    set_err("a label sum reached 2^24; float32 sums are no longer exact");
+    return VINE_E_RANGE;
+  }
+  st->rounds += rounds;
+  st->expanded += (int64_t)s->h_ct->expanded;
+  st->relaxed += (int64_t)s->h_ct->relaxed;
+  st->candidates += (int64_t)s->h_ct->cands;
+  st->inserted += (int64_t)s->h_ct->inserted;
+  st->sssp_relaxed += (int64_t)s->h_ct->sssp_relaxed;
+  st->batches++;
+  return VINE_OK;
+}
+
+vine_status check_queries_host(const vine_solver* s, int32_t nq, const int32_t* src, const int32_t* dst,
+                               const float* bound) {
+  for (int32_t i = 0; i < nq; i++) {
+    if (src[i] < 0 || src[i] >= s->g->n || dst[i] < 0 || dst[i] >= s->g->n) {
+      set_err("query %d: vertex id out of range [0, %d)", i, s->g->n);
+      return VINE_E_ARG;
+    }
+    if (!(bound[i] >= 0.f) || !(bound[i] < dev::EXACT_LIMIT)) {
+      set_err("query %d: bound %g must be in [0, 2^24)", i, (double)bound[i]);
+      return VINE_E_ARG;
+    }
+  }
+  return VINE_OK;
+}
+
+__global__ void k_check_queries(const int32_t* src, const int32_t* dst, const float* bound, int nq, int n,
+                                int* bad) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  if (src[i] < 0 || src[i] >= n || dst[i] < 0 || dst[i] >= n || !(bound[i] >= 0.f) ||
+      !(bound[i] < dev::EXACT_LIMIT))
+    atomicMin(bad, i);
+}
+
+// Solve nq queries whose arrays are in device memory; answers to device memory.
+vine_status solve_impl(vine_solver* s, int32_t nq, const int32_t* d_src, const int32_t* d_dst,
+                       const float* d_bound, float* d_cost, float* d_delay, int32_t* d_feas,
+                       int32_t* d_plen, vine_stats* st) {
+  Ctx& c = s->ctx;
+  s->latency_us.assign(nq, 0.0);
+  s->trace.clear();
+  for (int32_t q0 = 0; q0 < nq; q0 += s->B) {
+    int nb = std::min<int32_t>(s->B, nq - q0);
+    HIP_TRY(hipMemcpyAsync(s->d_qsrc, d_src + q0, sizeof(int32_t) * nb, hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_qdst, d_dst + q0, sizeof(int32_t) * nb, hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_qbound, d_bound + q0, sizeof(float) * nb, hipMemcpyDeviceToDevice, s->stream));
+    if (nb < s->B) {
+      // idle slots: harmless query 0 -> 0 with bound 0 (never initialised as live)
+      HIP_TRY(hipMemsetAsync(s->d_qsrc + nb, 0, sizeof(int32_t) * (s->B - nb), s->stream));
+      HIP_TRY(hipMemsetAsync(s->d_qdst + nb, 0, sizeof(int32_t) * (s->B - nb), s->stream));
+      HIP_TRY(hipMemsetAsync(s->d_qbound + nb, 0, sizeof(float) * (s->B - nb), s->stream));
+    }
+    vine_status bs = solve_batch(s, nb, st);
+    if (bs != VINE_OK) return bs;
+    // per-query finish times: merge-end clock of the last round with live labels
+    std::vector<int32_t> last(nb);
+    HIP_TRY(hipMemcpyAsync(last.data(), s->d_qlast, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s->stream));
+    std::vector<unsigned long long> ts(dev::TS_CAP);
+    int64_t nr = std::min<int64_t>(s->h_ct->round_e + 1, dev::TS_CAP);
+    HIP_TRY(hipMemcpyAsync(ts.data(), s->d_ts, sizeof(unsigned long long) * nr, hipMemcpyDeviceToHost, s->stream));
+    unsigned long long tstart = 0;
+    HIP_TRY(hipMemcpyAsync(&tstart, &s->d_ct->t_start, sizeof(tstart), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    int rate_khz = 100000;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, s->g->device) != hipSuccess ||
+        rate_khz <= 0)
+      rate_khz = 100000;
+    for (int i = 0; i < nb; i++) {
+      int r = last[i];
+      double us = 0.0;
+      if (r >= 0 && r < nr && ts[r] >= tstart) us = (double)(ts[r] - tstart) * 1000.0 / rate_khz;
+      if (q0 + i < (int32_t)s->latency_us.size()) s->latency_us[q0 + i] = us;
+    }
+    hipLaunchKernelGGL(dev::k_answers, dim3((nb + 255) / 256), dim3(256), 0, s->stream, c, nb, d_cost + q0,
+                       d_delay + q0, d_feas + q0, d_plen ? d_plen + q0 : nullptr);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return VINE_OK;
+}
+
+vine_status ensure_io(vine_solver* s, int32_t nq) {
+  if (nq <= s->in_cap) return VINE_OK;
+  for (void* p : {(void*)s->d_in_q, (void*)s->d_in_bound, (void*)s->d_out_cost, (void*)s->d_out_delay,
+                  (void*)s->d_out_feas, (void*)s->d_out_plen}) {
+    if (!p) continue;
+    auto it = std::find(s->allocs.begin(), s->allocs.end(), p);
+    if (it != s->allocs.end()) s->allocs.erase(it);
+    (void)hipFree(p);
+  }
+  s->in_cap = 0;
+  vine_status st;
+  if ((st = dev_alloc(s, (void**)&s->d_in_q, sizeof(int32_t) * 2 * nq)) != VINE_OK) return st;
+  if ((st = dev_alloc(s, (void**)&s->d_in_bound, sizeof(float) * nq)) != VINE_OK) return st;
+  if ((st = dev_alloc(s, (void**)&s->d_out_cost, sizeof(float) * nq)) != VINE_OK) return st;
+  if ((st = dev_alloc(s, (void**)&s->d_out_delay, sizeof(float) * nq)) != VINE_OK) return st;
+  if ((st = dev_alloc(s, (void**)&s->d_out_feas, sizeof(int32_t) * nq)) != VINE_OK) return st;
+  if ((st = dev_alloc(s, (void**)&s->d_out_plen, sizeof(int32_t) * nq)) != VINE_OK) return st;
+  s->in_cap = nq;
+  return VINE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+vine_status vine_solve_device(vine_solver* s, int32_t nq, const int32_t* d_src, const int32_t* d_dst,
+                              const float* d_bound, float* d_cost, float* d_delay, int32_t* d_feasible,
+                              vine_stats* out_stats) {
+  if (!s || nq < 0 || (nq > 0 && (!d_src || !d_dst || !d_bound || !d_cost || !d_delay || !d_feasible))) {
+    set_err("vine_solve_device: invalid arguments");
+    return VINE_E_ARG;
+  }
+  vine_stats st{};
+  double t0 = now_ms();
+  if (nq > 0) {
+    // validate ids and bounds on the device (no host copy of the query set)
+    vine_status es = ensure_io(s, 1);
+    if (es != VINE_OK) return es;
+    int big = INT32_MAX;
+    HIP_TRY(hipMemcpyAsync(s->d_out_plen, &big, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_check_queries, dim3((nq + 255) / 256), dim3(256), 0, s->stream, d_src, d_dst, d_bound,
+                       nq, s->g->n, s->d_out_plen);
+    int bad = 0;
+    HIP_TRY(hipMemcpyAsync(&bad, s->d_out_plen, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (bad != INT32_MAX) {
+      set_err("query %d: vertex id out of range or bound not in [0, 2^24)", bad);
+      return VINE_E_ARG;
+    }
+  }
+  vine_status r = solve_impl(s, nq, d_src, d_dst, d_bound, d_cost, d_delay, d_feasible, nullptr, &st);
+  st.ms_total = now_ms() - t0;
+  st.ms_kernels = st.ms_expand + st.ms_merge + st.ms_sssp;
+  if (out_stats) *out_stats = st;
+  return r;
+}
+
+vine_status vine_solve(vine_solver* s, int32_t nq, const int32_t* src, const int32_t* dst, const float* bound,
+                       float* out_cost, float* out_delay, int32_t* out_feasible, int64_t* out_path_off,
+                       int32_t* out_path, int64_t path_cap, vine_stats* out_stats) {
+  if (!s || nq < 0 || (nq > 0 && (!src || !dst || !bound || !out_cost || !out_delay || !out_feasible)) ||
+      (out_path && (!out_path_off || path_cap < 0))) {
+    set_err("vine_solve: invalid arguments");
+    return VINE_E_ARG;
+  }
+  vine_status v = check_queries_host(s, nq, src, dst, bound);
+  if (v != VINE_OK) return v;
+  vine_stats st{};
+  double t0 = now_ms();
+  if (nq == 0) {
+    if (out_path_off) out_path_off[0] = 0;
+    if (out_stats) *out_stats = st;
+    return VINE_OK;
+  }
+  v = ensure_io(s, nq);
+  if (v != VINE_OK) return v;
+  HIP_TRY(hipMemcpyAsync(s->d_in_q, src, sizeof(int32_t) * nq, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->d_in_q + nq, dst, sizeof(int32_t) * nq, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->d_in_bound, bound, sizeof(float) * nq, hipMemcpyHostToDevice, s->stream));
+  const bool want_path = out_path != nullptr;
+  // paths must be read before the next batch overwrites the label log: solve
+  // batch by batch and extract paths per batch
+  std::vector<int32_t> plen(nq, 0);
+  if (out_path_off) out_path_off[0] = 0;
+  int64_t written = 0;
+  bool path_overflow = false;
+  for (int32_t q0 = 0; q0 < nq; q0 += s->B) {
+    int nb = std::min<int32_t>(s->B, nq - q0);
+    v = solve_impl(s, nb, s->d_in_q + q0, s->d_in_q + nq + q0, s->d_in_bound + q0, s->d_out_cost + q0,
+                   s->d_out_delay + q0, s->d_out_feas + q0, s->d_out_plen + q0, &st);
+    if (v != VINE_OK) return v;
+    if (want_path || out_path_off) {
+      HIP_TRY(hipMemcpy(plen.data() + q0, s->d_out_plen + q0, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
+      for (int i = 0; i < nb; i++) out_path_off[q0 + i + 1] = out_path_off[q0 + i] + plen[q0 + i];
+    }
+    if (want_path) {
+      int64_t need = out_path_off[q0 + nb] - out_path_off[q0];
+      if (out_path_off[q0 + nb] > path_cap) {
+        path_overflow = true;
+      } else if (need > 0) {
+        long long* d_off = nullptr;
+        int32_t* d_path = nullptr;
+        HIP_TRY(hipMalloc(&d_off, sizeof(long long) * (nb + 1)));
+        hipError_t e = hipMalloc(&d_path, sizeof(int32_t) * need);
+        if (e != hipSuccess) {
+          (void)hipFree(d_off);
+          set_err("hipMalloc for paths: %s", hipGetErrorString(e));
+          return VINE_E_OOM;
+        }
+        std::vector<long long> off(nb + 1);
+        for (int i = 0; i <= nb; i++) off[i] = out_path_off[q0 + i] - out_path_off[q0];
+        e = hipMemcpy(d_off, off.data(), sizeof(long long) * (nb + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+          hipLaunchKernelGGL(dev::k_paths, dim3((nb + 255) / 256), dim3(256), 0, s->stream, s->ctx, nb, d_off,
+                             d_path);
+          e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e == hipSuccess)
+          e = hipMemcpy(out_path + out_path_off[q0], d_path, sizeof(int32_t) * need, hipMemcpyDeviceToHost);
+        (void)hipFree(d_off);
+        (void)hipFree(d_path);
+        if (e != hipSuccess) {
+          set_err("path extraction: %s", hipGetErrorString(e));
+          return VINE_E_HIP;
+        }
+      }
+      written = out_path_off[q0 + nb];
+    }
+  }
+  (void)written;
+  HIP_TRY(hipMemcpy(out_cost, s->d_out_cost, sizeof(float) * nq, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out_delay, s->d_out_delay, sizeof(float) * nq, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out_feasible, s->d_out_feas, sizeof(int32_t) * nq, hipMemcpyDeviceToHost));
+  st.ms_total = now_ms() - t0;
+  st.ms_kernels = st.ms_expand + st.ms_merge + st.ms_sssp;
+  if (out_stats) *out_stats = st;
+  if (path_overflow) {
+    set_err("vine_solve: path_cap %lld too small (need %lld)", (long long)path_cap,
+            (long long)out_path_off[nq]);
+    return VINE_E_OVERFLOW;
+  }
+  return VINE_OK;
+}
+
+vine_status vine_solver_latency(const vine_solver* s, int32_t nq, double* out_us) {
+  if (!s || !out_us || nq < 0) return VINE_E_ARG;
+  for (int32_t i = 0; i < nq; i++) out_us[i] = i < (int32_t)s->latency_us.size() ? s->latency_us[i] : 0.0;
+  return VINE_OK;
+}
+
+int64_t vine_solver_trace(const vine_solver* s, double* rows, int64_t max_rows) {
+  if (!s) return VINE_E_ARG;
+  int64_t n = (int64_t)s->trace.size() / 8;
+  if (!rows) return n;
+  int64_t k = std::min(n, max_rows);
+  std::memcpy(rows, s->trace.data(), sizeof(double) * 8 * k);
+  return k;
+}
+
+vine_status vine_solver_set_mlp(vine_solver* s, const float* params, int64_t nparams) {
+  if (!s || !params || nparams <= 0) {
+    set_err("vine_solver_set_mlp: invalid arguments");
+    return VINE_E_ARG;
+  }
+  if (!s->d_mlp || s->mlp_nparams != nparams) {
+    if (s->d_mlp) {
+      auto it = std::find(s->allocs.begin(), s->allocs.end(), (void*)s->d_mlp);
+      if (it != s->allocs.end()) s->allocs.erase(it);
+      (void)hipFree(s->d_mlp);
+      s->d_mlp = nullptr;
+    }
+    vine_status st = dev_alloc(s, (void**)&s->d_mlp, sizeof(float) * nparams);
+    if (st != VINE_OK) return st;
+    s->mlp_nparams = nparams;
+  }
+  HIP_TRY(hipMemcpy(s->d_mlp, params, sizeof(float) * nparams, hipMemcpyHostToDevice));
+  return VINE_OK;
+}
+
+int64_t vine_solver_pareto(const vine_solver* s, int32_t q, int32_t v, float* out_pairs, int64_t cap) {
+  if (!s || q < 0 || q >= s->B || v < 0 || v >= s->g->n || (cap > 0 && !out_pairs)) {
+    set_err("vine_solver_pareto: invalid arguments");
+    return VINE_E_ARG;
+  }
+  int2 ps;
+  hipError_t e = hipMemcpy(&ps, s->d_pset + (long long)q * s->g->n + v, sizeof(int2), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    set_err("vine_solver_pareto: %s", hipGetErrorString(e));
+    return VINE_E_HIP;
+  }
+  int64_t cnt = ps.y & ((1 << 26) - 1);
+  int64_t k = std::min(cnt, cap);
+  if (k > 0) {
+    e = hipMemcpy(out_pairs, s->d_pool_cd + (uint32_t)ps.x, sizeof(float2) * k, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      set_err("vine_solver_pareto: %s", hipGetErrorString(e));
+      return VINE_E_HIP;
+    }
+  }
+  return cnt;
+}
+
+}  // extern "C"

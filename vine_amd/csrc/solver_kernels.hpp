@@ -1,0 +1,647 @@
+// Device side of the exact CSP solver for gfx950 (CDNA4, wave64).
+//
+// A round is two kernels:
+//   expand  (k_expand)  -- counterpart of ck_expand_bucket_with_filter
+//                          (algorithm/bucket.cuh:15-195) and ck_recheck_bucket
+//                          (:217-245): every live frontier label inside the
+//                          window is extended along its out-arcs; a candidate is
+//                          filtered by the constraint, by the pruning bounds and
+//                          by a read-only dominance check against the target
+//                          vertex's Pareto set, pre-merged in an LDS hash, and
+//                          pushed onto a per-(query, vertex) candidate list.
+//                          Labels outside the window are carried.
+//   merge   (k_merge)   -- counterpart of second_level_pruning
+//                          (algorithm/dominance.cuh:15-153): one lane owns each
+//                          touched (query, vertex) and folds its candidates into
+//                          the Pareto set (drop dominated, evict dominated
+//                          entries, append survivors to the label log, which is
+//                          the next frontier).  No locks: the two kernels never
+//                          run concurrently and each set has a single writer.
+// The thrust scan + D2H size + ck_fill_buckets sequence of the reference
+// (algorithm/cspp.cuh:245-292, bucket.cuh:197-215) is replaced by wave-level
+// ballot/mbcnt compaction into the label log; the frontier size never leaves
+// the device unless the host polls it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vine/label.hpp"
+
+namespace vine {
+namespace dev {
+
+constexpr int WAVE = 64;
+constexpr float FINF = __builtin_huge_valf();
+constexpr float EXACT_LIMIT = 16777216.0f;  // 2^24
+constexpr int INIT_CAP = 4;                 // first Pareto-set allocation (entries)
+constexpr int TS_CAP = 1 << 20;             // per-round timestamps kept
+
+enum : int { OVF_LABEL = 1, OVF_CAND = 2, OVF_TOUCH = 4, OVF_POOL = 8, OVF_CARRY = 16, OVF_SSSP = 32 };
+
+struct Counters {
+  unsigned long long label_ctr;  // labels in the log
+  unsigned long long pool_ctr;   // Pareto pool entries handed out
+  int cand_ctr[2];
+  int touched_n[2];
+  int carry_n[2];
+  long long lstart[2];   // first label of the frontier's "new" range, per parity
+  int round_e;           // round number read by expand
+  int round_m;           // round number read by merge (written by expand)
+  int sf_n[3];           // bound-computation frontier sizes (triple buffered)
+  int sround;            // bound-computation round
+  unsigned int blocks_done;
+  int overflow;
+  int range_flag;
+  int pad;
+  unsigned long long t_start;  // device wall clock at batch start
+  unsigned long long expanded, relaxed, cands, inserted, sssp_relaxed;
+};
+
+struct Ctx {
+  int32_t n;
+  int32_t B;
+  // graph
+  const int32_t* row_ptr;
+  const int4* edges;  // {dst, len bits, delay bits, 0}
+  const int32_t* rrow_ptr;
+  const int4* redges;
+  // per query slot
+  const int32_t* q_src;
+  const int32_t* q_dst;
+  const float* q_bound;
+  const float* q_delta;
+  float2* q_best;
+  int32_t* q_best_id;
+  uint32_t* q_fmin;  // [2][B] float bits
+  int32_t* q_last;   // last round with live frontier labels
+  float* q_fmax;     // [B] max key among expanded labels this round (stats)
+  int32_t* q_nfront; // [B] live frontier labels this round (stats)
+  // per (query, vertex)
+  int2* pset;        // {pool offset, cnt | capcode << 26}
+  int32_t* cand_head;
+  float* lbl;
+  float* lbd;
+  int32_t* stamp;
+  // pools
+  float2* pool_cd;
+  int32_t* pool_id;
+  unsigned long long pool_cap;
+  int4* labs;        // {vid, d bits, delay bits, parent}
+  int2* lmeta;       // {query slot, key bits}
+  uint8_t* dead;
+  long long label_cap;
+  float2* cand;
+  int32_t* cand_par;
+  int32_t* cand_next;
+  long long* touched;
+  int32_t cand_cap;
+  int32_t* carry[2];
+  int32_t carry_cap;
+  long long* sfr[2];
+  int32_t sf_cap;
+  unsigned long long* ts;  // [TS_CAP] per-round merge-end wall clock
+  Counters* ct;
+  int32_t hash_slots;
+};
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ int mbcnt64(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Wave-aggregated slot allocation: one atomic per wave, ranks by mbcnt.
+__device__ __forceinline__ int wave_alloc32(int* ctr, bool pred) {
+  unsigned long long m = __ballot(pred);
+  if (m == 0ull) return -1;
+  int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane_id() == leader) base = atomicAdd(ctr, __popcll(m));
+  base = __shfl(base, leader, WAVE);
+  return pred ? base + mbcnt64(m) : -1;
+}
+
+__device__ __forceinline__ long long wave_alloc64(unsigned long long* ctr, bool pred) {
+  unsigned long long m = __ballot(pred);
+  if (m == 0ull) return -1;
+  int leader = __ffsll((long long)m) - 1;
+  unsigned long long base = 0;
+  if (lane_id() == leader) base = atomicAdd(ctr, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader, WAVE);
+  return pred ? (long long)base + mbcnt64(m) : -1;
+}
+
+// Variable-size wave allocation: returns this lane's offset for `size` items.
+__device__ __forceinline__ long long wave_alloc64_n(unsigned long long* ctr, int size) {
+  int lane = lane_id();
+  int incl = size;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    int t = __shfl_up(incl, o, WAVE);
+    if (lane >= o) incl += t;
+  }
+  int total = __shfl(incl, WAVE - 1, WAVE);
+  if (total == 0) return -1;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(ctr, (unsigned long long)total);
+  base = __shfl(base, 0, WAVE);
+  return (long long)base + (incl - size);
+}
+
+__device__ __forceinline__ void wave_add(unsigned long long* ctr, int v) {
+  int lane = lane_id();
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  if (lane == 0 && v) atomicAdd(ctr, (unsigned long long)v);
+}
+
+// Visit every (item, sub-item) pair of the wave, 64 pairs per step: lane i
+// holds an item with `deg` sub-items starting at `row`.  The wave scans the
+// degrees, then each lane finds its owner by binary search over the exclusive
+// offsets (all through cross-lane permutes, no LDS).  body(owner, pos, active)
+// runs with wave-uniform control flow.
+template <class F>
+__device__ __forceinline__ void wave_flat(int deg, int row, F&& body) {
+  int lane = lane_id();
+  int incl = deg;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    int t = __shfl_up(incl, o, WAVE);
+    if (lane >= o) incl += t;
+  }
+  int excl = incl - deg;
+  int total = __shfl(incl, WAVE - 1, WAVE);
+  for (int k = 0; k < total; k += WAVE) {
+    int e = k + lane;
+    bool act = e < total;
+    int ec = act ? e : total - 1;
+    int lo = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+      int v = __shfl(excl, lo + step, WAVE);
+      if (v <= ec) lo += step;
+    }
+    int pos = __shfl(row, lo, WAVE) + (ec - __shfl(excl, lo, WAVE));
+    body(lo, pos, act);
+  }
+}
+
+__device__ __forceinline__ int pset_cnt(int2 p) { return p.y & ((1 << 26) - 1); }
+__device__ __forceinline__ int pset_cap(int2 p) {
+  int code = (int)((uint32_t)p.y >> 26);
+  return code ? (1 << (code - 1)) : 0;
+}
+__device__ __forceinline__ int2 pset_make(uint32_t off, int cnt, int cap) {
+  int code = 0;
+  if (cap) code = 32 - __clz(cap);  // cap is a power of two: log2(cap) + 1
+  return make_int2((int)off, cnt | (code << 26));
+}
+
+__device__ __forceinline__ bool lex_ge(float a, float b, float c, float d) {
+  // (a, b) >= (c, d) lexicographically
+  return a > c || (a == c && b >= d);
+}
+
+// ---------------------------------------------------------------------------
+// Batch initialisation: one source label per query slot (the ck_init of
+// algorithm/cspp.cuh:63-67), answers reset.
+template <int PRUNE>
+__global__ void k_init_batch(Ctx c, int nq) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= c.B) return;
+  if (q < nq) {
+    int s = c.q_src[q];
+    long long qs = (long long)q * c.n + s;
+    float key = 0.f;
+    if (PRUNE == 2) key = c.lbl[qs];
+    c.labs[q] = make_int4(s, __float_as_int(0.f), __float_as_int(0.f), -1);
+    c.lmeta[q] = make_int2(q, __float_as_int(key));
+    c.dead[q] = 0;
+    // source in its own Pareto set, storage [q*INIT_CAP, +INIT_CAP)
+    c.pool_cd[(long long)q * INIT_CAP] = make_float2(0.f, 0.f);
+    c.pool_id[(long long)q * INIT_CAP] = q;
+    c.pset[qs] = pset_make((uint32_t)(q * INIT_CAP), 1, INIT_CAP);
+    bool st = (s == c.q_dst[q]);
+    c.q_best[q] = st ? make_float2(0.f, 0.f) : make_float2(FINF, FINF);
+    c.q_best_id[q] = st ? q : -1;
+    c.q_fmin[q] = __float_as_uint(key);
+  } else {
+    // unused slot: a dead label so that the log stays dense
+    c.labs[q] = make_int4(0, 0, 0, -1);
+    c.lmeta[q] = make_int2(q, __float_as_int(FINF));
+    c.dead[q] = 1;
+    c.q_best[q] = make_float2(FINF, FINF);
+    c.q_best_id[q] = -1;
+    c.q_fmin[q] = __float_as_uint(FINF);
+  }
+  c.q_fmin[c.B + q] = __float_as_uint(FINF);
+  c.q_last[q] = -1;
+  c.q_fmax[q] = 0.f;
+  c.q_nfront[q] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Lower bounds to t on the transposed graph (VINE_PRUNE_BOUNDS), label-
+// correcting SSSP with a deduplicated frontier.  WHICH = 0: delay, limited to
+// radius bound.  WHICH = 1: len, restricted to vertices with lbd <= bound (only
+// those can lie on a feasible completion, so the restricted distance is still
+// a valid lower bound for every label that survives the delay test).
+__global__ void k_sssp_init(Ctx c, int nq, int which) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  long long qt = (long long)q * c.n + c.q_dst[q];
+  float* dist = which ? c.lbl : c.lbd;
+  bool ok = which ? (c.lbd[qt] <= c.q_bound[q]) : true;
+  if (!ok) return;
+  dist[qt] = 0.f;
+  int slot = atomicAdd(&c.ct->sf_n[c.ct->sround % 3], 1);
+  if (slot < c.sf_cap) c.sfr[c.ct->sround & 1][slot] = qt;
+  else atomicOr(&c.ct->overflow, OVF_SSSP);
+}
+
+template <int WHICH>
+__global__ void __launch_bounds__(256) k_sssp(Ctx c) {
+  const int r = c.ct->sround;
+  const int cur = r & 1, nxt = cur ^ 1;
+  const int n_in = min(c.ct->sf_n[r % 3], c.sf_cap);
+  if (blockIdx.x == 0 && threadIdx.x == 0) c.ct->sf_n[(r + 2) % 3] = 0;
+  float* dist = WHICH ? c.lbl : c.lbd;
+  const int lane = lane_id();
+  const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
+  int relaxed = 0;
+  for (long long base = ((long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * WAVE;
+       base < n_in; base += nwaves * WAVE) {
+    long long i = base + lane;
+    int deg = 0, row = 0, q = 0;
+    float du = 0.f;
+    if (i < n_in) {
+      long long qv = c.sfr[cur][i];
+      q = (int)(qv / c.n);
+      int u = (int)(qv - (long long)q * c.n);
+      du = dist[qv];
+      row = c.rrow_ptr[u];
+      deg = c.rrow_ptr[u + 1] - row;
+    }
+    wave_flat(deg, row, [&](int owner, int pos, bool act) {
+      float odu = __shfl(du, owner, WAVE);
+      int oq = __shfl(q, owner, WAVE);
+      bool push = false;
+      long long wv = 0;
+      if (act) {
+        relaxed++;
+        int4 e = c.redges[pos];
+        float wt = WHICH ? __int_as_float(e.y) : __int_as_float(e.z);
+        float nd = odu + wt;
+        float bnd = c.q_bound[oq];
+        wv = (long long)oq * c.n + e.x;
+        bool ok = WHICH ? (c.lbd[wv] <= bnd) : (nd <= bnd);
+        if (ok && nd < dist[wv]) {
+          unsigned old = atomicMin((unsigned*)&dist[wv], __float_as_uint(nd));
+          if (__float_as_uint(nd) < old) push = (atomicExch(&c.stamp[wv], r) != r);
+        }
+      }
+      int slot = wave_alloc32(&c.ct->sf_n[(r + 1) % 3], push);
+      if (push) {
+        if (slot < c.sf_cap) c.sfr[nxt][slot] = wv;
+        else atomicOr(&c.ct->overflow, OVF_SSSP);
+      }
+    });
+  }
+  wave_add(&c.ct->sssp_relaxed, relaxed);
+  // last block advances the round
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned prev = atomicAdd(&c.ct->blocks_done, 1u);
+    if (prev == gridDim.x - 1) {
+      c.ct->blocks_done = 0;
+      c.ct->sround = r + 1;
+      __threadfence();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Expand.  PRUNE: VINE_PRUNE_* ; policy_count > 0 applies the reference's DOP
+// count window (expand the last DOP frontier entries, carry the rest,
+// algorithm/cspp.cuh:179-216).
+template <typename Label_t, typename Constraint_t, typename Edge_t, int PRUNE>
+__global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
+  using LT = label_traits<Label_t>;
+  Counters* ct = c.ct;
+  const int r = ct->round_e;
+  const int cur = r & 1, nxt = cur ^ 1;
+  const int ncarry = min(ct->carry_n[cur], c.carry_cap);
+  const long long lstart = ct->lstart[cur];
+  const long long lend = min((long long)ct->label_ctr, c.label_cap);
+  const long long fn = (long long)ncarry + (lend - lstart);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ct->lstart[nxt] = lend;
+    ct->round_m = r;
+  }
+  const int lane = lane_id();
+  const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
+  int n_exp = 0, n_rel = 0, n_cand = 0;
+  const uint32_t* fmin_cur = c.q_fmin + (size_t)cur * c.B;
+  uint32_t* fmin_nxt = c.q_fmin + (size_t)nxt * c.B;
+
+  for (long long base = ((long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * WAVE;
+       base < fn; base += nwaves * WAVE) {
+    long long i = base + lane;
+    int id = -1, deg = 0, row = 0, q = 0;
+    Label_t lab;
+    bool carry = false;
+    float key = 0.f;
+    if (i < fn) {
+      id = (i < ncarry) ? c.carry[cur][i] : (int)(lstart + (i - ncarry));
+      if (!c.dead[id]) {
+        int4 L = c.labs[id];
+        int2 meta = c.lmeta[id];
+        q = meta.x;
+        key = __int_as_float(meta.y);
+        lab = LT::unpack(L.x, make_float2(__int_as_float(L.y), __int_as_float(L.z)));
+        bool live = true;
+        if (PRUNE != 0 && lab.vid == c.q_dst[q]) live = false;  // answers are not extended
+        if (live && PRUNE == 1) {
+          float2 b = c.q_best[q];
+          if (lex_ge(lab.d, lab.delay, b.x, b.y)) live = false;
+        }
+        if (live && PRUNE == 2) {
+          float2 b = c.q_best[q];
+          long long qv = (long long)q * c.n + lab.vid;
+          float rd = lab.delay + c.lbd[qv];
+          if (rd > c.q_bound[q] || lex_ge(key, rd, b.x, b.y)) live = false;
+        }
+        if (live) {
+          c.q_last[q] = r;
+          bool in_window;
+          if (dop > 0) in_window = (i >= fn - dop);
+          else in_window = !(key > __uint_as_float(fmin_cur[q]) + c.q_delta[q]);
+          if (in_window) {
+            row = c.row_ptr[lab.vid];
+            deg = c.row_ptr[lab.vid + 1] - row;
+            n_exp++;
+          } else {
+            carry = true;
+          }
+        }
+      }
+    }
+    // carried labels keep their place in the next frontier
+    int cslot = wave_alloc32(&ct->carry_n[nxt], carry);
+    if (carry) {
+      if (cslot < c.carry_cap) {
+        c.carry[nxt][cslot] = id;
+        if (__float_as_uint(key) < fmin_nxt[q]) atomicMin(&fmin_nxt[q], __float_as_uint(key));
+      } else {
+        atomicOr(&ct->overflow, OVF_CARRY);
+      }
+    }
+    float ld = lab.d, ldl = lab.delay;
+    wave_flat(deg, row, [&](int owner, int pos, bool act) {
+      Label_t src = LT::unpack(0, make_float2(__shfl(ld, owner, WAVE), __shfl(ldl, owner, WAVE)));
+      const int oq = __shfl(q, owner, WAVE);
+      const int oid = __shfl(id, owner, WAVE);
+      bool ok = false;
+      Label_t son;
+      long long qv = 0;
+      if (act) {
+        n_rel++;
+        int4 e = c.edges[pos];
+        Edge_t ed(__int_as_float(e.y), __int_as_float(e.z));
+        src.expand(e.x, ed, &son);
+        Constraint_t C(c.q_bound[oq]);
+        ok = C.pass_constraints_check(son);
+        qv = (long long)oq * c.n + son.vid;
+        if (ok && PRUNE == 1 && son.vid != c.q_dst[oq]) {
+          float2 b = c.q_best[oq];
+          if (lex_ge(son.d, son.delay, b.x, b.y)) ok = false;
+        }
+        if (ok && PRUNE == 2) {
+          float rd = son.delay + c.lbd[qv];
+          if (rd > C.delay_L) {
+            ok = false;
+          } else {
+            float2 b = c.q_best[oq];
+            if (lex_ge(son.d + c.lbl[qv], rd, b.x, b.y)) ok = false;
+          }
+        }
+        if (ok) {
+          // read-only dominance check against the Pareto set (no writer runs now)
+          int2 ps = c.pset[qv];
+          int cnt = pset_cnt(ps);
+          const float2* ent = c.pool_cd + (uint32_t)ps.x;
+          for (int j = 0; j < cnt; j++) {
+            if (Label_t::dominate(LT::unpack(son.vid, ent[j]), son)) {
+              ok = false;
+              break;
+            }
+          }
+        }
+      }
+      int cid = wave_alloc32(&ct->cand_ctr[cur], ok);
+      bool first = false;
+      if (ok) {
+        if (cid < c.cand_cap) {
+          n_cand++;
+          c.cand[cid] = LT::pack(son);
+          c.cand_par[cid] = oid;
+          int old = atomicExch(&c.cand_head[qv], cid);
+          c.cand_next[cid] = old;
+          first = (old < 0);
+        } else {
+          atomicOr(&ct->overflow, OVF_CAND);
+        }
+      }
+      int tix = wave_alloc32(&ct->touched_n[cur], first);
+      if (first) {
+        if (tix < c.cand_cap) c.touched[tix] = qv;
+        else atomicOr(&ct->overflow, OVF_TOUCH);
+      }
+    });
+  }
+  wave_add(&ct->expanded, n_exp);
+  wave_add(&ct->relaxed, n_rel);
+  wave_add(&ct->cands, n_cand);
+}
+
+// ---------------------------------------------------------------------------
+// Merge: fold each touched (query, vertex)'s candidates into its Pareto set.
+template <typename Label_t, int PRUNE>
+__global__ void __launch_bounds__(256) k_merge(Ctx c) {
+  using LT = label_traits<Label_t>;
+  Counters* ct = c.ct;
+  const int r = ct->round_m;
+  const int cur = r & 1, nxt = cur ^ 1;
+  const int tn = min(ct->touched_n[cur], c.cand_cap);
+  // a candidate list never holds more than this round's candidates; the walk
+  // is cut there so that state left by an overflowed round cannot loop
+  const int max_steps = min(ct->cand_ctr[cur], c.cand_cap);
+  const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gtid == 0) {
+    ct->touched_n[nxt] = 0;
+    ct->cand_ctr[nxt] = 0;
+    ct->carry_n[cur] = 0;
+  }
+  for (long long q = gtid; q < c.B; q += (long long)gridDim.x * blockDim.x)
+    c.q_fmin[(size_t)cur * c.B + q] = __float_as_uint(FINF);
+  uint32_t* fmin_nxt = c.q_fmin + (size_t)nxt * c.B;
+  const int lane = lane_id();
+  const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
+  int n_ins = 0;
+  for (long long base = ((long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * WAVE;
+       base < tn; base += nwaves * WAVE) {
+    long long i = base + lane;
+    long long qv = 0;
+    int q = 0, v = 0, h = -1, cnt = 0, cap = 0, tq = -1;
+    uint32_t off = 0;
+    float lb = 0.f;
+    if (i < tn) {
+      qv = c.touched[i];
+      q = (int)(qv / c.n);
+      v = (int)(qv - (long long)q * c.n);
+      h = c.cand_head[qv];
+      c.cand_head[qv] = -1;
+      int2 ps = c.pset[qv];
+      off = (uint32_t)ps.x;
+      cnt = pset_cnt(ps);
+      cap = pset_cap(ps);
+      tq = c.q_dst[q];
+      if (PRUNE == 2) lb = c.lbl[qv];
+    }
+    int steps = 0;
+    while (__ballot(h >= 0)) {
+      if (h >= 0 && ++steps > max_steps) {
+        atomicOr(&ct->overflow, OVF_CAND);
+        h = -1;
+      }
+      bool acc = false;
+      float2 cv = make_float2(0.f, 0.f);
+      int par = -1;
+      if (h >= 0) {
+        cv = c.cand[h];
+        par = c.cand_par[h];
+        Label_t cl = LT::unpack(v, cv);
+        bool dom = false;
+        for (int j = 0; j < cnt; j++) {
+          if (Label_t::dominate(LT::unpack(v, c.pool_cd[off + j]), cl)) {
+            dom = true;
+            break;
+          }
+        }
+        if (!dom) {
+          for (int j = 0; j < cnt;) {
+            if (Label_t::dominate(cl, LT::unpack(v, c.pool_cd[off + j]))) {
+              c.dead[c.pool_id[off + j]] = 1;
+              cnt--;
+              c.pool_cd[off + j] = c.pool_cd[off + cnt];
+              c.pool_id[off + j] = c.pool_id[off + cnt];
+            } else {
+              j++;
+            }
+          }
+          acc = true;
+        }
+      }
+      bool grow = acc && cnt == cap;
+      int newcap = grow ? (cap ? cap * 2 : INIT_CAP) : 0;
+      long long poff = wave_alloc64_n(&ct->pool_ctr, newcap);
+      long long lid = wave_alloc64(&ct->label_ctr, acc);
+      if (acc) {
+        bool fits = true;
+        if (grow) {
+          if ((unsigned long long)(poff + newcap) > c.pool_cap || poff + newcap > 0xFFFFFFFFll) {
+            atomicOr(&ct->overflow, OVF_POOL);
+            fits = false;
+          } else {
+            for (int j = 0; j < cnt; j++) {
+              c.pool_cd[poff + j] = c.pool_cd[off + j];
+              c.pool_id[poff + j] = c.pool_id[off + j];
+            }
+            off = (uint32_t)poff;
+            cap = newcap;
+          }
+        }
+        if (lid >= c.label_cap) {
+          atomicOr(&ct->overflow, OVF_LABEL);
+          fits = false;
+        }
+        if (fits) {
+          n_ins++;
+          c.pool_cd[off + cnt] = cv;
+          c.pool_id[off + cnt] = (int)lid;
+          cnt++;
+          Label_t cl = LT::unpack(v, cv);
+          float key = LT::key(cl) + lb;
+          c.labs[lid] = make_int4(v, __float_as_int(cv.x), __float_as_int(cv.y), par);
+          c.lmeta[lid] = make_int2(q, __float_as_int(key));
+          c.dead[lid] = 0;
+          if (!(cv.x < EXACT_LIMIT) || !(cv.y < EXACT_LIMIT)) ct->range_flag = 1;
+          if (v == tq) {
+            float2 b = c.q_best[q];
+            if (cv.x < b.x || (cv.x == b.x && cv.y < b.y)) {
+              c.q_best[q] = cv;
+              c.q_best_id[q] = (int)lid;
+            }
+          }
+          if (!(PRUNE != 0 && v == tq) && __float_as_uint(key) < fmin_nxt[q])
+            atomicMin(&fmin_nxt[q], __float_as_uint(key));
+        }
+      }
+      if (h >= 0) h = c.cand_next[h];
+    }
+    if (i < tn) c.pset[qv] = pset_make(off, cnt, cap);
+  }
+  wave_add(&ct->inserted, n_ins);
+  // last block stamps the end of the round and advances the round counter
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned prev = atomicAdd(&ct->blocks_done, 1u);
+    if (prev == gridDim.x - 1) {
+      ct->blocks_done = 0;
+      c.ts[r & (TS_CAP - 1)] = wall_clock64();
+      ct->round_e = r + 1;
+      __threadfence();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Answers and paths.
+__global__ void k_answers(Ctx c, int nq, float* cost, float* delay, int32_t* feasible,
+                          int32_t* plen) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  int id = c.q_best_id[q];
+  float2 b = c.q_best[q];
+  cost[q] = id >= 0 ? b.x : FINF;
+  delay[q] = id >= 0 ? b.y : FINF;
+  feasible[q] = id >= 0 ? 1 : 0;
+  if (plen) {
+    int h = 0;
+    for (int x = id; x >= 0; x = c.labs[x].w) h++;
+    plen[q] = h;
+  }
+}
+
+__global__ void k_paths(Ctx c, int nq, const long long* off, int32_t* path) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  long long o = off[q], e = off[q + 1];
+  long long k = e - 1;
+  for (int x = c.q_best_id[q]; x >= 0 && k >= o; x = c.labs[x].w) path[k--] = c.labs[x].x;
+}
+
+__global__ void k_stamp_start(Ctx c) { c.ct->t_start = wall_clock64(); }
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+}  // namespace dev
+}  // namespace vine
