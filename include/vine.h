@@ -103,7 +103,9 @@ typedef struct {
   double ms_merge;          /* HIP-event time of the merge kernels */
   double ms_sssp;           /* HIP-event time of the bound kernels */
   int64_t expand_launches;
-  int64_t reserved[8];
+  int64_t merge_launches;
+  int64_t sssp_launches;
+  int64_t reserved[6];
 } vine_stats;
 
 void vine_options_default(vine_options* opt);
@@ -150,6 +152,15 @@ vine_status vine_solver_set_mlp(vine_solver* s, const float* params, int64_t npa
  * batch of the last solve (VINE_PRUNE_NONE gives the reference's full sets).
  * Writes up to cap (d, delay) pairs, returns the set size or a negative status. */
 int64_t vine_solver_pareto(const vine_solver* s, int32_t q, int32_t v, float* out_pairs, int64_t cap);
+
+/* Host scheduling helpers (also used by the solver's VINE_DELTA_AR policy).
+ * vine_ar_burg: Burg AR fit, the reference's AR::AutoRegression with the
+ *   default MAXENTROPY method (include/AR.cpp:6-147); 0 on success.
+ * vine_dop_sequence: the reference's DOP rule (algorithm/cspp.cuh:254-284 with
+ *   calculate_p :33-41), fed one frontier size per round; writes the DOP after
+ *   each round and returns n. */
+int vine_ar_burg(const double* x, int n, int degree, double* coef);
+int64_t vine_dop_sequence(const int64_t* sizes, int n, double max_labels, int64_t* out_dop);
 
 #ifdef __cplusplus
 }

@@ -45,6 +45,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 typedef struct {
   int32_t vid;
@@ -97,13 +98,14 @@ enum { OC_X = 0, OC_E, OC_I, OC_ROUNDS, OC_MAXF, OC_SUMF, OC_NCOUNT };
  *      when (d + lbl, delay + lbd) is lexicographically >= (bd, bdl).
  * out_pareto receives t's Pareto set sorted by d as (d, delay) pairs.
  * frontier_trace (optional, length trace_cap) receives the frontier size after
- * every round. */
+ * every round.  time_limit_s > 0 aborts with -4 once that much CPU time is spent
+ * (checked once per round). */
 int oracle_cspp(int32_t n, const int32_t* row_ptr, const int32_t* dst, const float* len,
                 const float* delay, int32_t s, int32_t t, float bound, int32_t prune_target,
                 float* out_cost, float* out_delay, int32_t* out_path, int32_t path_cap,
                 int32_t* out_path_len, float* out_pareto, int32_t pareto_cap,
                 int32_t* out_pareto_len, int64_t* counters, int32_t* frontier_trace,
-                int32_t trace_cap) {
+                int32_t trace_cap, double time_limit_s) {
   if (n <= 0 || s < 0 || s >= n || t < 0 || t >= n) return -1;
   int64_t cnt[OC_NCOUNT] = {0};
   ivec* dom = (ivec*)calloc((size_t)n, sizeof(ivec));
@@ -156,8 +158,13 @@ int oracle_cspp(int32_t n, const int32_t* row_ptr, const int32_t* dst, const flo
     bdl = 0.0f;
   }
   int32_t rounds = 0;
+  clock_t t_begin = clock();
   while (frontier.n > 0) {
     rounds++;
+    if (time_limit_s > 0 && (double)(clock() - t_begin) / CLOCKS_PER_SEC > time_limit_s) {
+      rc = -4; /* time limit reached: no answer */
+      goto done;
+    }
     for (int32_t ff = 0; ff < frontier.n; ff++) {
       int32_t fid = frontier.a[ff];
       olabel f = log[fid];

@@ -34,7 +34,7 @@ def lib():
         vp, i32, f32, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_double
         L.oracle_cspp.restype = i32
         L.oracle_cspp.argtypes = [i32, vp, vp, vp, vp, i32, i32, f32, i32, vp, vp, vp, i32, vp, vp, i32,
-                                  vp, vp, vp, i32]
+                                  vp, vp, vp, i32, f64]
         L.oracle_dijkstra_lex.restype = i32
         L.oracle_dijkstra_lex.argtypes = [i32, vp, vp, vp, vp, i32, vp, vp]
         L.oracle_resource_dp.restype = i32
@@ -67,8 +67,13 @@ def _g(g):
     return (g.n, g.row_ptr.ctypes.data, g.dst.ctypes.data, g.len.ctypes.data, g.delay.ctypes.data)
 
 
-def cspp(g, s, t, bound, prune_target=False, path_cap=None, pareto_cap=4096, trace_cap=0):
-    """Solve one query; returns Result."""
+class TimeLimit(RuntimeError):
+    pass
+
+
+def cspp(g, s, t, bound, prune_target=False, path_cap=None, pareto_cap=4096, trace_cap=0, time_limit=0.0):
+    """Solve one query; returns Result.  prune_target: 0/False reference
+    one-to-all search, 1 target pruning, 2 lower-bound pruning."""
     path_cap = g.n if path_cap is None else path_cap
     cost = ctypes.c_float()
     dly = ctypes.c_float()
@@ -78,10 +83,13 @@ def cspp(g, s, t, bound, prune_target=False, path_cap=None, pareto_cap=4096, tra
     parl = ctypes.c_int32()
     cnt = np.zeros(6, np.int64)
     trace = np.zeros(max(trace_cap, 1), np.int32)
-    rc = lib().oracle_cspp(*_g(g), int(s), int(t), float(bound), 1 if prune_target else 0,
+    rc = lib().oracle_cspp(*_g(g), int(s), int(t), float(bound), int(prune_target),
                            ctypes.byref(cost), ctypes.byref(dly), path.ctypes.data, path_cap,
                            ctypes.byref(plen), par.ctypes.data, pareto_cap, ctypes.byref(parl),
-                           cnt.ctypes.data, trace.ctypes.data if trace_cap else None, trace_cap)
+                           cnt.ctypes.data, trace.ctypes.data if trace_cap else None, trace_cap,
+                           float(time_limit))
+    if rc == -4:
+        raise TimeLimit("oracle_cspp exceeded %.1f s" % time_limit)
     if rc < 0:
         raise RuntimeError("oracle_cspp failed (%d)" % rc)
     k = min(parl.value, pareto_cap)

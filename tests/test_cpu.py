@@ -1,0 +1,146 @@
+"""CPU-side checks of the product library: the C ABI loads and exports every
+symbol the headers declare, the workload generators are deterministic and have
+the documented shape, and the host scheduling rule (AR / DOP) matches the
+reference's values (golden vectors from the compiled include/AR.cpp)."""
+import ctypes
+import hashlib
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from vine_amd import _lib
+from vine_amd import solver as S
+from vine_amd import workload as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden", "ar_golden.json")
+
+
+def _declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(vine_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = _declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_options_layout_matches_header():
+    o = S.default_options()
+    assert o.max_batch == 1 and o.prune == S.PRUNE_BOUNDS and o.block_size == 256
+    assert ctypes.sizeof(S.Options) == 112
+    assert ctypes.sizeof(S.Stats) == 8 * 9 + 8 * 5 + 8 * 3 + 8 * 6
+
+
+def test_errors_without_device():
+    # argument errors are reported before any device work
+    L = S._L()
+    h = ctypes.c_void_p()
+    rc = L.vine_graph_create(0, 0, None, None, None, None, ctypes.byref(h))
+    assert rc == S.VINE_E_ARG
+    assert b"invalid" in L.vine_last_error()
+
+
+def _h(*arrs):
+    m = hashlib.sha256()
+    for a in arrs:
+        m.update(np.ascontiguousarray(a).tobytes())
+    return m.hexdigest()[:16]
+
+
+def test_generators_deterministic():
+    a = W.grid(seed=0)
+    b = W.grid(seed=0)
+    c = W.grid(seed=1)
+    assert _h(a.row_ptr, a.dst, a.len, a.delay) == _h(b.row_ptr, b.dst, b.len, b.delay)
+    assert _h(a.row_ptr, a.dst, a.len, a.delay) != _h(c.row_ptr, c.dst, c.len, c.delay)
+    k1 = W.knn(5000, seed=3)
+    k2 = W.knn(5000, seed=3)
+    assert _h(k1.dst, k1.len) == _h(k2.dst, k2.len)
+    q1 = W.hop_queries(k1, 50, hop_lo=3, hop_hi=40, seed=2, nthreads=1)
+    q2 = W.hop_queries(k1, 50, hop_lo=3, hop_hi=40, seed=2, nthreads=4)
+    assert np.array_equal(q1.src, q2.src) and np.array_equal(q1.bound, q2.bound)
+
+
+def _symmetric(g):
+    u = np.repeat(np.arange(g.n), np.diff(g.row_ptr))
+    fwd = set(zip(u.tolist(), g.dst.tolist(), g.len.tolist(), g.delay.tolist()))
+    return all((v, x, l, d) in fwd for (x, v, l, d) in fwd)
+
+
+def test_grid_shape():
+    g = W.grid()
+    assert g.n == 4096 and g.m == 14516
+    assert g.len.min() >= 1 and g.len.max() <= 1000 and g.delay.min() >= 1 and g.delay.max() <= 100
+    assert np.all(g.len == np.round(g.len)) and np.all(g.delay == np.round(g.delay))
+    assert _symmetric(g)
+
+
+def test_knn_shape():
+    g = W.knn(20000, seed=0)
+    deg = np.diff(g.row_ptr)
+    assert deg.min() >= 3  # every point keeps its 3 nearest neighbours
+    assert g.len.min() >= 1 and np.all(g.delay >= 1)
+    ratio = g.delay / g.len
+    assert ratio.min() >= 0.5 - 0.5 / g.len.min() and ratio.max() <= 2.0 + 0.5
+    assert _symmetric(g)
+
+
+def test_rmat_shape():
+    g = W.rmat(scale=10, edge_factor=8)
+    u = np.repeat(np.arange(g.n), np.diff(g.row_ptr))
+    assert not np.any(u == g.dst)  # no self loops
+    pairs = set(zip(u.tolist(), g.dst.tolist()))
+    assert len(pairs) == g.m  # no duplicates
+    assert g.len.min() >= 1 and g.len.max() <= 100
+    assert _symmetric(g)
+
+
+def test_query_bounds_are_integer_floors():
+    g = W.grid()
+    q = W.random_queries(g, 40, (13, 10), seed=4)
+    reach = q.mindelay >= 0
+    assert np.all(q.bound[reach] == (q.mindelay[reach] * 13 // 10).astype(np.float32))
+    assert np.all(q.src != q.dst)
+
+
+def test_product_ar_matches_reference_golden():
+    with open(GOLDEN) as f:
+        gold = json.load(f)
+    L = _lib.lib()
+    fn = L.vine_ar_burg
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    for case in gold["ar"]:
+        x = np.asarray(case["series"], np.float64)
+        c = np.zeros(3)
+        assert fn(x.ctypes.data, x.shape[0], 3, c.ctypes.data) == 0
+        assert c.tolist() == case["burg"]
+
+
+def test_product_dop_rule_matches_reference_golden():
+    with open(GOLDEN) as f:
+        gold = json.load(f)
+    L = _lib.lib()
+    fn = L.vine_dop_sequence
+    fn.restype = ctypes.c_int64
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
+    for case in gold["dop"]:
+        sizes = np.asarray(case["sizes"], np.int64)
+        out = np.zeros(sizes.shape[0], np.int64)
+        assert fn(sizes.ctypes.data, sizes.shape[0], case["max_labels"], out.ctypes.data) == sizes.shape[0]
+        assert out.tolist() == case["dop"]

@@ -4,11 +4,7 @@
 #include <algorithm>
 #include <cmath>
 
-extern "C" {
-// C-ABI access for tests: AR coefficients and a DOP sequence for a size trace.
-int vine_ar_burg(const double* x, int n, int degree, double* coef);
-int64_t vine_dop_sequence(const int64_t* sizes, int n, double max_labels, int64_t* out_dop);
-}
+#include "vine.h"
 
 namespace vine {
 

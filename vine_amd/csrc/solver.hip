@@ -117,6 +117,14 @@ struct vine_solver {
 
   Ctx ctx{};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // HIP-event brackets of timed launches, resolved at the end of each batch
+  struct EvRec {
+    hipEvent_t a, b;
+    int kind;  // 0 expand, 1 merge, 2 bound search
+  };
+  std::vector<hipEvent_t> evpool;
+  size_t ev_used = 0;
+  std::vector<EvRec> evrec;
 
   // last-solve records
   std::vector<double> latency_us;
@@ -260,6 +268,7 @@ vine_status vine_solver_destroy(vine_solver* s) {
   if (s->h_ct) (void)hipHostFree(s->h_ct);
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : s->evpool) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return VINE_OK;
@@ -428,23 +437,52 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
 // ---------------------------------------------------------------------------
 namespace {
 
+// Brackets a launch with HIP events on the solver stream (time_kernels = 1);
+// no synchronisation here, the pairs are resolved by resolve_events().
 struct Timer {
   vine_solver* s;
   bool on;
-  double* acc;
-  Timer(vine_solver* s_, double* acc_) : s(s_), on(s_->opt.time_kernels != 0), acc(acc_) {
-    if (on) (void)hipEventRecord(s->ev[0], s->stream);
+  hipEvent_t a = nullptr;
+  int kind;
+  Timer(vine_solver* s_, int kind_) : s(s_), on(s_->opt.time_kernels != 0), kind(kind_) {
+    if (on) {
+      a = next_event(s);
+      if (a) (void)hipEventRecord(a, s->stream);
+      else on = false;
+    }
+  }
+  static hipEvent_t next_event(vine_solver* s) {
+    if (s->ev_used == s->evpool.size()) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      s->evpool.push_back(e);
+    }
+    return s->evpool[s->ev_used++];
   }
   hipError_t stop() {
     if (!on) return hipSuccess;
-    hipError_t e = hipEventRecord(s->ev[1], s->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(s->ev[1]);
-    float ms = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, s->ev[0], s->ev[1]);
-    *acc += ms;
+    hipEvent_t b = next_event(s);
+    if (!b) return hipErrorOutOfMemory;
+    hipError_t e = hipEventRecord(b, s->stream);
+    s->evrec.push_back({a, b, kind});
     return e;
   }
 };
+
+vine_status resolve_events(vine_solver* s, vine_stats* st) {
+  if (s->evrec.empty()) return VINE_OK;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (const auto& r : s->evrec) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+    if (r.kind == 0) st->ms_expand += ms;
+    else if (r.kind == 1) st->ms_merge += ms;
+    else st->ms_sssp += ms;
+  }
+  s->evrec.clear();
+  s->ev_used = 0;
+  return VINE_OK;
+}
 
 vine_status poll(vine_solver* s) {
   HIP_TRY(hipMemcpyAsync(s->h_ct, s->d_ct, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
@@ -462,19 +500,20 @@ template <int PRUNE>
 vine_status launch_round(vine_solver* s, long long dop, vine_stats* st) {
   Ctx& c = s->ctx;
   {
-    Timer t(s, &st->ms_expand);
+    Timer t(s, 0);
     hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, PRUNE>), dim3(s->grid),
                        dim3(s->block), 0, s->stream, c, dop);
     HIP_TRY(hipGetLastError());
     HIP_TRY(t.stop());
   }
   {
-    Timer t(s, &st->ms_merge);
+    Timer t(s, 1);
     hipLaunchKernelGGL((dev::k_merge<c_label, PRUNE>), dim3(s->grid), dim3(s->block), 0, s->stream, c);
     HIP_TRY(hipGetLastError());
     HIP_TRY(t.stop());
   }
   st->expand_launches++;
+  st->merge_launches++;
   return VINE_OK;
 }
 
@@ -502,18 +541,20 @@ vine_status fill_f32(vine_solver* s, float* p, float v, long long n) {
 // Bound computation: label-correcting SSSP on the transposed graph.
 vine_status run_sssp(vine_solver* s, int nb, int which, vine_stats* st) {
   Ctx& c = s->ctx;
-  Timer t(s, &st->ms_sssp);
   hipLaunchKernelGGL(dev::k_sssp_init, dim3((nb + 255) / 256), dim3(256), 0, s->stream, c, nb, which);
   HIP_TRY(hipGetLastError());
   const int K = std::max(1, s->opt.rounds_per_sync);
   for (;;) {
     for (int k = 0; k < K; k++) {
+      Timer t(s, 2);
       if (which == 0)
         hipLaunchKernelGGL(dev::k_sssp<0>, dim3(s->grid), dim3(s->block), 0, s->stream, c);
       else
         hipLaunchKernelGGL(dev::k_sssp<1>, dim3(s->grid), dim3(s->block), 0, s->stream, c);
       HIP_TRY(hipGetLastError());
+      HIP_TRY(t.stop());
       st->sssp_rounds++;
+      st->sssp_launches++;
     }
     vine_status ps = poll(s);
     if (ps != VINE_OK) return ps;
@@ -524,7 +565,6 @@ vine_status run_sssp(vine_solver* s, int nb, int which, vine_stats* st) {
       return VINE_E_OVERFLOW;
     }
   }
-  HIP_TRY(t.stop());
   return VINE_OK;
 }
 
@@ -615,6 +655,8 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
     set_err("a label sum reached 2^24; float32 sums are no longer exact");
     return VINE_E_RANGE;
   }
+  vine_status es = resolve_events(s, st);
+  if (es != VINE_OK) return es;
   st->rounds += rounds;
   st->expanded += (int64_t)s->h_ct->expanded;
   st->relaxed += (int64_t)s->h_ct->relaxed;

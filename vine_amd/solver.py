@@ -66,7 +66,9 @@ class Stats(ctypes.Structure):
         ("ms_merge", ctypes.c_double),
         ("ms_sssp", ctypes.c_double),
         ("expand_launches", ctypes.c_int64),
-        ("reserved", ctypes.c_int64 * 8),
+        ("merge_launches", ctypes.c_int64),
+        ("sssp_launches", ctypes.c_int64),
+        ("reserved", ctypes.c_int64 * 6),
     ]
 
     def as_dict(self):
