@@ -1,0 +1,219 @@
+"""Benchmark: exact CSP queries/sec on the NY-scale synthetic road graph.
+
+Workload (one "step"): 4,096 queries on the 264,346-vertex 3-NN geometric
+graph, hop distances log-uniform in [10, 2000] (truncated at the graph's hop
+diameter), bounds floor(U[1.05, 2.0] x min delay), solved by the batched
+solver in one pass (all queries share every round launch).  Inputs are in HBM
+before the timed region; answers stay in HBM.  Beside it: the plain
+single-query solver on the random-pair sweeps (bound 1.1x and 1.5x), the
+roofline of the dominant kernel from HIP-event durations, and the CPU oracle
+timed on a bounded sample.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--queries", type=int, default=4096)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--policy", default="fixed", choices=["fixed", "ar", "mlp", "inf"])
+    ap.add_argument("--delta", type=float, default=2000.0)
+    ap.add_argument("--plain-queries", type=int, default=100, help="per bound factor")
+    ap.add_argument("--cpu-queries", type=int, default=32)
+    ap.add_argument("--cpu-limit", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(queries would be sharded with a replicated graph; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from vine_amd import solver as S
+    from vine_amd import workload as W
+
+    if S.device_count() < 1 or not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+
+    t0 = time.time()
+    g = W.knn(264346, seed=0)
+    log("graph n=%d m=%d (%.1fs)" % (g.n, g.m, time.time() - t0))
+    t0 = time.time()
+    qb = W.hop_queries(g, args.queries, seed=0)
+    log("batched workload: %d queries, hops p50=%d p99=%d (%.1fs)"
+        % (len(qb), np.percentile(qb.hops, 50), np.percentile(qb.hops, 99), time.time() - t0))
+
+    policy = {"fixed": S.DELTA_FIXED, "ar": S.DELTA_AR, "mlp": S.DELTA_MLP, "inf": S.DELTA_INF}[args.policy]
+    dg = S.DeviceGraph(g)
+    solver = S.Solver(dg, max_batch=args.batch, prune=S.PRUNE_BOUNDS, delta_policy=policy,
+                      delta_fixed=args.delta, rounds_per_sync=16, time_kernels=1,
+                      label_capacity=1_000_000_000, pool_capacity=2_000_000_000,
+                      cand_capacity=400_000_000, frontier_capacity=400_000_000)
+    log("solver ready (batch %d)" % args.batch)
+    nq = len(qb)
+    d_src = torch.from_numpy(qb.src).to(dev)
+    d_dst = torch.from_numpy(qb.dst).to(dev)
+    d_bnd = torch.from_numpy(qb.bound).to(dev)
+    d_cost = torch.empty(nq, dtype=torch.float32, device=dev)
+    d_dly = torch.empty(nq, dtype=torch.float32, device=dev)
+    d_feas = torch.empty(nq, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        return solver.solve_device(d_src.data_ptr(), d_dst.data_ptr(), d_bnd.data_ptr(), d_cost.data_ptr(),
+                                   d_dly.data_ptr(), d_feas.data_ptr(), nq)
+
+    for w in range(args.warmup):
+        st = step()
+        log("warmup %d: %.1f ms, rounds %d, expanded %d" % (w, st["ms_total"], st["rounds"], st["expanded"]))
+    torch.cuda.synchronize()
+    stats = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        stats.append(step())
+        log("step %d: %.1f ms" % (k, stats[-1]["ms_total"]))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    qps = nq * args.steps / elapsed
+    ms_step = elapsed * 1000.0 / args.steps
+    cost_b = d_cost.cpu().numpy()
+    dly_b = d_dly.cpu().numpy()
+    feas_b = d_feas.cpu().numpy()
+    lat = solver.latency_us(nq)
+    log("batched: %.1f queries/s, %.1f ms/step, feasible %d/%d" % (qps, ms_step, int(feas_b.sum()), nq))
+
+    # roofline of the dominant kernel (algorithmic bytes / HIP-event duration)
+    agg = {k: sum(s[k] for s in stats) for k in stats[0]}
+    kern = {
+        "expand": (agg["ms_expand"], agg["expand_launches"], 20.0 * agg["expanded"] + 12.0 * agg["relaxed"]),
+        "merge": (agg["ms_merge"], agg["merge_launches"], 24.0 * agg["inserted"]),
+        "bound_sssp": (agg["ms_sssp"], agg["sssp_launches"], 20.0 * agg["sssp_relaxed"]),
+    }
+    dom = max(kern, key=lambda k: kern[k][0])
+    ms_k, launches, nbytes = kern[dom]
+    avg_s = ms_k / 1000.0 / max(launches, 1)
+    per_launch = nbytes / max(launches, 1)
+    achieved = per_launch / avg_s if avg_s > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None, "avg_launch_us": avg_s * 1e6,
+                "launches_per_step": launches / args.steps,
+                "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()}}
+
+    # plain single-query solver (reference-style: one query at a time)
+    plain = None
+    if not args.no_plain and args.plain_queries > 0:
+        q11 = W.random_queries(g, 1000, (11, 10), seed=0)
+        q15 = W.random_queries(g, 1000, (15, 10), seed=0)
+        sub = np.arange(args.plain_queries)
+        with S.Solver(dg, max_batch=1, prune=S.PRUNE_BOUNDS, delta_policy=policy, delta_fixed=args.delta,
+                      rounds_per_sync=1, label_capacity=200_000_000) as ps:
+            res = {}
+            for name, qq in (("1.1x", q11.subset(sub)), ("1.5x", q15.subset(sub))):
+                t1 = time.perf_counter()
+                a = ps.solve(qq.src, qq.dst, qq.bound, paths=False)
+                dt = time.perf_counter() - t1
+                res[name] = {"queries": len(qq), "qps": len(qq) / dt, "feasible": int(a.feasible.sum())}
+                log("plain %s: %.1f queries/s" % (name, len(qq) / dt))
+            # same answers as the batched pass on a sample of its queries
+            chk = np.arange(min(64, nq))
+            qs = qb.subset(chk)
+            t1 = time.perf_counter()
+            a = ps.solve(qs.src, qs.dst, qs.bound, paths=False)
+            dt = time.perf_counter() - t1
+            same = bool(np.array_equal(a.cost, cost_b[chk]) and np.array_equal(a.delay, dly_b[chk])
+                        and np.array_equal(a.feasible.astype(np.int32), feas_b[chk]))
+            res["batched_workload_sample"] = {"queries": len(chk), "qps": len(chk) / dt, "same_answers": same}
+            log("plain on batched sample: %.1f queries/s, same answers %s" % (len(chk) / dt, same))
+        plain = res
+
+    # CPU oracle on a bounded sample of the same workload
+    cpu = None
+    if not args.no_cpu and args.cpu_queries > 0:
+        from concurrent.futures import ThreadPoolExecutor
+
+        from oracle import oracle as O
+
+        rng = np.random.default_rng(1)
+        idx = rng.choice(nq, min(args.cpu_queries, nq), replace=False)
+        threads = min(16, os.cpu_count() or 1)
+
+        def one(i):
+            t1 = time.perf_counter()
+            try:
+                r = O.cspp(g, qb.src[i], qb.dst[i], qb.bound[i], prune_target=2, time_limit=args.cpu_limit)
+                ok = (r.feasible == bool(feas_b[i])) and (not r.feasible or (
+                    np.float32(r.cost) == cost_b[i] and np.float32(r.delay) == dly_b[i]))
+                return time.perf_counter() - t1, True, ok
+            except O.TimeLimit:
+                return max(time.perf_counter() - t1, args.cpu_limit), False, True
+
+        t1 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            out = list(ex.map(one, idx))
+        wall = time.perf_counter() - t1
+        done = sum(1 for o in out if o[1])
+        agree = all(o[2] for o in out)
+        cpu = {"value": len(idx) / wall, "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": "%d random queries of the batched workload, oracle (restated cspp_cpu with the same "
+                         "lower-bound pruning, whole-frontier rounds) on %d threads; %d finished, %d stopped at "
+                         "%.0f s CPU each and counted at that time (so the CPU rate is an upper bound); "
+                         "finished answers agree with the GPU: %s"
+                         % (len(idx), threads, done, len(idx) - done, args.cpu_limit, agree)}
+        log("cpu: %.2f queries/s (%d/%d finished)" % (cpu["value"], done, len(idx)))
+
+    line = {
+        "metric": "exact CSP queries/sec, NY-scale road graph: batched+learned vs plain GPU",
+        "value": qps,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, seed 0)",
+        "config": {"workload": "ny_batch: 264,346-vertex 3-NN graph (%d arcs), %d queries, hop-distance "
+                               "log-uniform [10,2000], bound U[1.05,2.0] x min delay" % (g.m, nq),
+                   "batch": args.batch, "policy": args.policy, "delta": args.delta, "prune": "bounds",
+                   "parallelism": "single GPU"},
+        "latency_us": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "work_per_step": {k: agg[k] / args.steps for k in ("rounds", "sssp_rounds", "expanded", "relaxed",
+                                                           "inserted", "sssp_relaxed")},
+        "roofline": roofline,
+        "plain": plain,
+        "vs_plain": (qps / plain["1.1x"]["qps"]) if plain else None,
+        "cpu_baseline": cpu,
+    }
+    solver.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

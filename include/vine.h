@@ -84,7 +84,8 @@ typedef struct {
   double ar_max_labels;     /* MAX for the DOP rule (0 = free device memory / 12 B, as the reference) */
   int32_t record_trace;     /* keep per-round frontier statistics (for vine_solver_trace) */
   int32_t time_kernels;     /* bracket every round kernel with HIP events (vine_stats.ms_*) */
-  int32_t reserved[6];
+  float bound_delta;        /* distance window of the bound search (0 = 2 x mean arc weight) */
+  int32_t reserved[5];
 } vine_options;
 
 typedef struct {
@@ -105,7 +106,10 @@ typedef struct {
   int64_t expand_launches;
   int64_t merge_launches;
   int64_t sssp_launches;
-  int64_t reserved[6];
+  int64_t touched;          /* (query, vertex) Pareto sets merged into */
+  int64_t merge_scanned;    /* Pareto entries compared by the merge kernel */
+  int64_t merge_chain_max;  /* longest candidate list folded by one lane */
+  int64_t reserved[3];
 } vine_stats;
 
 void vine_options_default(vine_options* opt);

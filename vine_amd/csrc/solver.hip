@@ -61,6 +61,8 @@ double now_ms() {
 struct vine_graph {
   int32_t n = 0;
   int64_t m = 0;
+  float mean_len = 1.f;
+  float mean_delay = 1.f;
   int device = 0;
   int32_t* d_row = nullptr;
   int4* d_edges = nullptr;
@@ -90,7 +92,8 @@ struct vine_solver {
   int2* d_pset = nullptr;
   int32_t* d_cand_head = nullptr;
   float *d_lbl = nullptr, *d_lbd = nullptr;
-  int32_t* d_stamp = nullptr;
+  uint32_t* d_inq = nullptr;
+  uint32_t* d_qdmin = nullptr;
   // pools
   float2* d_pool_cd = nullptr;
   int32_t* d_pool_id = nullptr;
@@ -100,7 +103,7 @@ struct vine_solver {
   float2* d_cand = nullptr;
   int32_t *d_cand_par = nullptr, *d_cand_next = nullptr;
   long long* d_touched = nullptr;
-  int32_t* d_carry[2] = {nullptr, nullptr};
+  int4* d_carry[2] = {nullptr, nullptr};
   long long* d_sfr[2] = {nullptr, nullptr};
   unsigned long long* d_ts = nullptr;
   Counters* d_ct = nullptr;
@@ -232,6 +235,15 @@ vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, cons
   auto* g = new vine_graph();
   g->n = n;
   g->m = m;
+  if (m > 0) {
+    double sl = 0, sd = 0;
+    for (int64_t p = 0; p < m; p++) {
+      sl += len[p];
+      sd += delay[p];
+    }
+    g->mean_len = (float)std::max(sl / m, 1e-3);
+    g->mean_delay = (float)std::max(sd / m, 1e-3);
+  }
   hipError_t err = hipGetDevice(&g->device);
   size_t eb = sizeof(int4) * std::max<int64_t>(m, 1);
   if (err == hipSuccess) err = hipMalloc(&g->d_row, sizeof(int32_t) * (n + 1));
@@ -323,12 +335,12 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
     return VINE_E_OOM;
   }
   // auto capacities: share the remaining memory (labels 25 B, pool 12 B x 2,
-  // candidates 24 B, carry 8 B, bound frontier 16 B per entry)
+  // candidates 24 B, carry 32 B, bound frontier 16 B per entry)
   long long lab_cap = o.label_capacity;
   if (!lab_cap) {
     // sized to the problem (64 labels per (query, vertex) pair plus slack),
     // capped by 60% of the free memory
-    double per = 25.0 + 2 * 12.0 + 0.5 * 24.0 + 0.5 * 8.0 + 0.25 * 16.0;
+    double per = 25.0 + 2 * 12.0 + 0.5 * 24.0 + 0.5 * 32.0 + 0.25 * 16.0;
     double want = 64.0 * (double)BN + (double)(1 << 22);
     lab_cap = (long long)std::min(std::min(remaining * 0.6 / per, want), (double)INT32_MAX - 1);
     lab_cap = std::max<long long>(lab_cap, 4 * B + 1024);
@@ -363,7 +375,8 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   if (o.prune == 2) {
     ALLOC(s->d_lbl, sizeof(float) * BN);
     ALLOC(s->d_lbd, sizeof(float) * BN);
-    ALLOC(s->d_stamp, sizeof(int32_t) * BN);
+    ALLOC(s->d_inq, sizeof(uint32_t) * BN);
+    ALLOC(s->d_qdmin, sizeof(uint32_t) * 3 * B);
     ALLOC(s->d_sfr[0], sizeof(long long) * sf_cap);
     ALLOC(s->d_sfr[1], sizeof(long long) * sf_cap);
   }
@@ -376,8 +389,8 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   ALLOC(s->d_cand_par, sizeof(int32_t) * cand_cap);
   ALLOC(s->d_cand_next, sizeof(int32_t) * cand_cap);
   ALLOC(s->d_touched, sizeof(long long) * cand_cap);
-  ALLOC(s->d_carry[0], sizeof(int32_t) * carry_cap);
-  ALLOC(s->d_carry[1], sizeof(int32_t) * carry_cap);
+  ALLOC(s->d_carry[0], sizeof(int4) * carry_cap);
+  ALLOC(s->d_carry[1], sizeof(int4) * carry_cap);
   ALLOC(s->d_ts, sizeof(unsigned long long) * dev::TS_CAP);
   ALLOC(s->d_ct, sizeof(Counters));
   if (hipHostMalloc((void**)&s->h_ct, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
@@ -406,7 +419,8 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   c.cand_head = s->d_cand_head;
   c.lbl = s->d_lbl;
   c.lbd = s->d_lbd;
-  c.stamp = s->d_stamp;
+  c.inq = s->d_inq;
+  c.q_dmin = s->d_qdmin;
   c.pool_cd = s->d_pool_cd;
   c.pool_id = s->d_pool_id;
   c.pool_cap = (unsigned long long)pool_cap;
@@ -497,18 +511,18 @@ vine_status poll(vine_solver* s) {
 }
 
 template <int PRUNE>
-vine_status launch_round(vine_solver* s, long long dop, vine_stats* st) {
+vine_status launch_round(vine_solver* s, long long dop, int gsz, vine_stats* st) {
   Ctx& c = s->ctx;
   {
     Timer t(s, 0);
-    hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, PRUNE>), dim3(s->grid),
+    hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, PRUNE>), dim3(gsz),
                        dim3(s->block), 0, s->stream, c, dop);
     HIP_TRY(hipGetLastError());
     HIP_TRY(t.stop());
   }
   {
     Timer t(s, 1);
-    hipLaunchKernelGGL((dev::k_merge<c_label, PRUNE>), dim3(s->grid), dim3(s->block), 0, s->stream, c);
+    hipLaunchKernelGGL((dev::k_merge<c_label, PRUNE>), dim3(gsz), dim3(s->block), 0, s->stream, c);
     HIP_TRY(hipGetLastError());
     HIP_TRY(t.stop());
   }
@@ -517,14 +531,14 @@ vine_status launch_round(vine_solver* s, long long dop, vine_stats* st) {
   return VINE_OK;
 }
 
-vine_status launch_round_any(vine_solver* s, long long dop, vine_stats* st) {
+vine_status launch_round_any(vine_solver* s, long long dop, int gsz, vine_stats* st) {
   switch (s->opt.prune) {
     case 0:
-      return launch_round<0>(s, dop, st);
+      return launch_round<0>(s, dop, gsz, st);
     case 1:
-      return launch_round<1>(s, dop, st);
+      return launch_round<1>(s, dop, gsz, st);
     default:
-      return launch_round<2>(s, dop, st);
+      return launch_round<2>(s, dop, gsz, st);
   }
 }
 
@@ -538,30 +552,45 @@ vine_status fill_f32(vine_solver* s, float* p, float v, long long n) {
   return VINE_OK;
 }
 
-// Bound computation: label-correcting SSSP on the transposed graph.
-vine_status run_sssp(vine_solver* s, int nb, int which, vine_stats* st) {
+// Bound computation: delta-stepping SSSP on the transposed graph.  The host
+// launches every round, so the round number travels as a kernel argument.
+int grid_for(const vine_solver* s, long long items) {
+  long long want = 2 * items / s->block + 4;
+  return (int)std::max<long long>(4, std::min<long long>(want, s->grid));
+}
+
+vine_status run_sssp(vine_solver* s, int nb, int which, int& sr, vine_stats* st) {
   Ctx& c = s->ctx;
-  hipLaunchKernelGGL(dev::k_sssp_init, dim3((nb + 255) / 256), dim3(256), 0, s->stream, c, nb, which);
+  vine_status fs = fill_f32(s, (float*)s->d_qdmin, dev::FINF, 3LL * s->B);
+  if (fs != VINE_OK) return fs;
+  hipLaunchKernelGGL(dev::k_sssp_init, dim3((nb + 255) / 256), dim3(256), 0, s->stream, c, nb, which, sr);
   HIP_TRY(hipGetLastError());
-  const int K = std::max(1, s->opt.rounds_per_sync);
+  float delta = s->opt.bound_delta > 0 ? s->opt.bound_delta
+                                       : 2.0f * (which ? s->g->mean_len : s->g->mean_delay);
+  const int K = std::max(1, std::min(4, s->opt.rounds_per_sync));
+  long long last = nb;
+  const int r0 = sr;
   for (;;) {
     for (int k = 0; k < K; k++) {
+      int gsz = grid_for(s, last * 4);
       Timer t(s, 2);
       if (which == 0)
-        hipLaunchKernelGGL(dev::k_sssp<0>, dim3(s->grid), dim3(s->block), 0, s->stream, c);
+        hipLaunchKernelGGL(dev::k_sssp<0>, dim3(gsz), dim3(s->block), 0, s->stream, c, sr, delta);
       else
-        hipLaunchKernelGGL(dev::k_sssp<1>, dim3(s->grid), dim3(s->block), 0, s->stream, c);
+        hipLaunchKernelGGL(dev::k_sssp<1>, dim3(gsz), dim3(s->block), 0, s->stream, c, sr, delta);
       HIP_TRY(hipGetLastError());
       HIP_TRY(t.stop());
+      sr++;
       st->sssp_rounds++;
       st->sssp_launches++;
     }
     vine_status ps = poll(s);
     if (ps != VINE_OK) return ps;
-    if (s->h_ct->sf_n[s->h_ct->sround % 3] == 0) break;
-    // label-correcting SSSP settles within n rounds per query; more means a bug
-    if (s->h_ct->sround > 4LL * s->g->n + 256) {
-      set_err("bound computation made no progress after %d rounds", s->h_ct->sround);
+    last = s->h_ct->sf_n[sr % 3];
+    if (last == 0) break;
+    // every round settles at least the closest frontier vertex of each query
+    if ((long long)(sr - r0) > 4LL * s->g->n * 8 + 256) {
+      set_err("bound computation made no progress after %d rounds", sr - r0);
       return VINE_E_OVERFLOW;
     }
   }
@@ -590,9 +619,10 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
     vine_status fs = fill_f32(s, s->d_lbd, dev::FINF, BN);
     if (fs == VINE_OK) fs = fill_f32(s, s->d_lbl, dev::FINF, BN);
     if (fs != VINE_OK) return fs;
-    HIP_TRY(hipMemsetAsync(s->d_stamp, 0xFF, sizeof(int32_t) * BN, s->stream));
-    vine_status ss = run_sssp(s, nb, 0, st);
-    if (ss == VINE_OK) ss = run_sssp(s, nb, 1, st);
+    HIP_TRY(hipMemsetAsync(s->d_inq, 0, sizeof(uint32_t) * BN, s->stream));
+    int sr = 0;
+    vine_status ss = run_sssp(s, nb, 0, sr, st);
+    if (ss == VINE_OK) ss = run_sssp(s, nb, 1, sr, st);
     if (ss != VINE_OK) return ss;
   }
   switch (s->opt.prune) {
@@ -623,10 +653,13 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
   }
   const int K = ar ? 1 : std::max(1, s->opt.rounds_per_sync);
   int64_t rounds = 0;
+  long long fn_last = nb;
   for (;;) {
     for (int k = 0; k < K; k++) {
       long long dop = ar ? (long long)rule.dop() : 0;
-      vine_status ls = launch_round_any(s, dop, st);
+      // the frontier may grow between polls: size the grid for 4x growth
+      // (grid-stride loops keep any size correct)
+      vine_status ls = launch_round_any(s, dop, grid_for(s, fn_last * 4), st);
       if (ls != VINE_OK) return ls;
       rounds++;
     }
@@ -636,6 +669,7 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
     int par = h.round_e & 1;
     long long fn = (long long)h.carry_n[par] + ((long long)h.label_ctr - h.lstart[par]);
     st->max_frontier = std::max<int64_t>(st->max_frontier, fn);
+    fn_last = std::max<long long>(fn, 1);
     if (s->opt.record_trace) {
       double row[8] = {-1, (double)rounds, (double)fn, (double)h.expanded, 0, 0,
                        ar ? (double)rule.dop() : (double)s->opt.delta_fixed, (double)h.cands};
@@ -651,6 +685,8 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
     }
     if (ar) rule.update((int64_t)fn);
   }
+  hipLaunchKernelGGL(dev::k_stamp_round, dim3(1), dim3(1), 0, s->stream, c, (int)rounds);
+  HIP_TRY(hipGetLastError());
   if (s->h_ct->range_flag) {
     set_err("a label sum reached 2^24; float32 sums are no longer exact");
     return VINE_E_RANGE;
@@ -663,6 +699,9 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
   st->candidates += (int64_t)s->h_ct->cands;
   st->inserted += (int64_t)s->h_ct->inserted;
   st->sssp_relaxed += (int64_t)s->h_ct->sssp_relaxed;
+  st->touched += (int64_t)s->h_ct->touched_total;
+  st->merge_scanned += (int64_t)s->h_ct->merge_scanned;
+  st->merge_chain_max = std::max<int64_t>(st->merge_chain_max, s->h_ct->merge_chain_max);
   st->batches++;
   return VINE_OK;
 }
@@ -715,7 +754,7 @@ vine_status solve_impl(vine_solver* s, int32_t nq, const int32_t* d_src, const i
     std::vector<int32_t> last(nb);
     HIP_TRY(hipMemcpyAsync(last.data(), s->d_qlast, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s->stream));
     std::vector<unsigned long long> ts(dev::TS_CAP);
-    int64_t nr = std::min<int64_t>(s->h_ct->round_e + 1, dev::TS_CAP);
+    int64_t nr = std::min<int64_t>(s->h_ct->round_e + 2, dev::TS_CAP);
     HIP_TRY(hipMemcpyAsync(ts.data(), s->d_ts, sizeof(unsigned long long) * nr, hipMemcpyDeviceToHost, s->stream));
     unsigned long long tstart = 0;
     HIP_TRY(hipMemcpyAsync(&tstart, &s->d_ct->t_start, sizeof(tstart), hipMemcpyDeviceToHost, s->stream));
@@ -725,9 +764,9 @@ vine_status solve_impl(vine_solver* s, int32_t nq, const int32_t* d_src, const i
         rate_khz <= 0)
       rate_khz = 100000;
     for (int i = 0; i < nb; i++) {
-      int r = last[i];
+      int r = last[i] + 1;  // the query's last active round ends when the next one starts
       double us = 0.0;
-      if (r >= 0 && r < nr && ts[r] >= tstart) us = (double)(ts[r] - tstart) * 1000.0 / rate_khz;
+      if (r >= 1 && r < nr && ts[r] >= tstart) us = (double)(ts[r] - tstart) * 1000.0 / rate_khz;
       if (q0 + i < (int32_t)s->latency_us.size()) s->latency_us[q0 + i] = us;
     }
     hipLaunchKernelGGL(dev::k_answers, dim3((nb + 255) / 256), dim3(256), 0, s->stream, c, nb, d_cost + q0,

@@ -56,6 +56,10 @@ struct Counters {
   int pad;
   unsigned long long t_start;  // device wall clock at batch start
   unsigned long long expanded, relaxed, cands, inserted, sssp_relaxed;
+  unsigned long long touched_total;  // (query, vertex) merges
+  unsigned long long merge_scanned;  // Pareto entries compared in merge
+  int merge_chain_max;               // longest candidate list merged by one lane
+  int pad2;
 };
 
 struct Ctx {
@@ -82,7 +86,8 @@ struct Ctx {
   int32_t* cand_head;
   float* lbl;
   float* lbd;
-  int32_t* stamp;
+  uint32_t* inq;     // (query, vertex) in the bound-search frontier
+  uint32_t* q_dmin;  // [3][B] smallest frontier distance per query (bound search)
   // pools
   float2* pool_cd;
   int32_t* pool_id;
@@ -96,7 +101,7 @@ struct Ctx {
   int32_t* cand_next;
   long long* touched;
   int32_t cand_cap;
-  int32_t* carry[2];
+  int4* carry[2];    // carried frontier entries {label id, query slot, key bits, pruning delay bits}
   int32_t carry_cap;
   long long* sfr[2];
   int32_t sf_cap;
@@ -188,6 +193,15 @@ __device__ __forceinline__ void wave_flat(int deg, int row, F&& body) {
   }
 }
 
+// Loads that bypass the (non-coherent) vector L1, for values other CUs update
+// with atomics during the same kernel.
+__device__ __forceinline__ float load_agent(const float* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t load_agent_u32(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ int pset_cnt(int2 p) { return p.y & ((1 << 26) - 1); }
 __device__ __forceinline__ int pset_cap(int2 p) {
   int code = (int)((uint32_t)p.y >> 26);
@@ -202,6 +216,59 @@ __device__ __forceinline__ int2 pset_make(uint32_t off, int cnt, int cap) {
 __device__ __forceinline__ bool lex_ge(float a, float b, float c, float d) {
   // (a, b) >= (c, d) lexicographically
   return a > c || (a == c && b >= d);
+}
+
+// Is `cand` dominated by one of the cnt entries at ent?  Entries are read
+// SCAN_CHUNK at a time with all loads issued before any compare, so a set of S
+// entries costs about S / SCAN_CHUNK memory round trips per lane.
+constexpr int SCAN_CHUNK = 8;
+
+template <typename Label_t>
+__device__ __forceinline__ bool set_dominates(const float2* ent, int cnt, int vid, const Label_t& cand) {
+  using LT = label_traits<Label_t>;
+  for (int j = 0; j < cnt; j += SCAN_CHUNK) {
+    float2 e[SCAN_CHUNK];
+#pragma unroll
+    for (int k = 0; k < SCAN_CHUNK; k++) e[k] = (j + k < cnt) ? ent[j + k] : make_float2(FINF, FINF);
+    bool dom = false;
+#pragma unroll
+    for (int k = 0; k < SCAN_CHUNK; k++)
+      dom |= (j + k < cnt) && Label_t::dominate(LT::unpack(vid, e[k]), cand);
+    if (dom) return true;
+  }
+  return false;
+}
+
+// Remove (and mark dead) every entry that `cand` dominates, compacting the
+// survivors in place; returns the new count.
+template <typename Label_t>
+__device__ __forceinline__ int set_evict(float2* ent, int32_t* ids, int cnt, int vid, const Label_t& cand,
+                                         uint8_t* dead) {
+  using LT = label_traits<Label_t>;
+  int w = 0;
+  for (int j = 0; j < cnt; j += SCAN_CHUNK) {
+    float2 e[SCAN_CHUNK];
+    int32_t id[SCAN_CHUNK];
+#pragma unroll
+    for (int k = 0; k < SCAN_CHUNK; k++) {
+      e[k] = (j + k < cnt) ? ent[j + k] : make_float2(0.f, 0.f);
+      id[k] = (j + k < cnt) ? ids[j + k] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < SCAN_CHUNK; k++) {
+      if (j + k >= cnt) break;
+      if (Label_t::dominate(cand, LT::unpack(vid, e[k]))) {
+        dead[id[k]] = 1;
+      } else {
+        if (w != j + k) {
+          ent[w] = e[k];
+          ids[w] = id[k];
+        }
+        w++;
+      }
+    }
+  }
+  return w;
 }
 
 // ---------------------------------------------------------------------------
@@ -243,30 +310,43 @@ __global__ void k_init_batch(Ctx c, int nq) {
 }
 
 // ---------------------------------------------------------------------------
-// Lower bounds to t on the transposed graph (VINE_PRUNE_BOUNDS), label-
-// correcting SSSP with a deduplicated frontier.  WHICH = 0: delay, limited to
-// radius bound.  WHICH = 1: len, restricted to vertices with lbd <= bound (only
-// those can lie on a feasible completion, so the restricted distance is still
-// a valid lower bound for every label that survives the delay test).
-__global__ void k_sssp_init(Ctx c, int nq, int which) {
+// Lower bounds to t on the transposed graph (VINE_PRUNE_BOUNDS): a batched
+// delta-stepping SSSP.  Each round relaxes, per query, the frontier vertices
+// whose distance is within `delta` of that query's smallest frontier distance
+// and carries the rest, which keeps re-relaxation close to Dijkstra's while a
+// round still covers every query of the batch.  inq marks (query, vertex)
+// pairs present in the frontier (exactly one copy each).
+//   WHICH = 0: delay, limited to radius `bound`.
+//   WHICH = 1: len, restricted to vertices with lbd <= bound: only those can
+//   lie on a feasible completion, so the restricted distance is a valid lower
+//   bound for every label that passes the delay test.
+// Round r reads frontier r&1 and counter sf_n[r%3], appends to (r+1), clears
+// counter (r+2)%3; q_dmin is triple-buffered the same way.
+
+__global__ void k_sssp_init(Ctx c, int nq, int which, int r) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
   long long qt = (long long)q * c.n + c.q_dst[q];
   float* dist = which ? c.lbl : c.lbd;
-  bool ok = which ? (c.lbd[qt] <= c.q_bound[q]) : true;
-  if (!ok) return;
   dist[qt] = 0.f;
-  int slot = atomicAdd(&c.ct->sf_n[c.ct->sround % 3], 1);
-  if (slot < c.sf_cap) c.sfr[c.ct->sround & 1][slot] = qt;
+  c.inq[qt] = 1;
+  c.q_dmin[(size_t)(r % 3) * c.B + q] = __float_as_uint(0.f);
+  int slot = atomicAdd(&c.ct->sf_n[r % 3], 1);
+  if (slot < c.sf_cap) c.sfr[r & 1][slot] = qt;
   else atomicOr(&c.ct->overflow, OVF_SSSP);
 }
 
 template <int WHICH>
-__global__ void __launch_bounds__(256) k_sssp(Ctx c) {
-  const int r = c.ct->sround;
+__global__ void __launch_bounds__(256) k_sssp(Ctx c, int r, float delta) {
   const int cur = r & 1, nxt = cur ^ 1;
   const int n_in = min(c.ct->sf_n[r % 3], c.sf_cap);
-  if (blockIdx.x == 0 && threadIdx.x == 0) c.ct->sf_n[(r + 2) % 3] = 0;
+  const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gtid == 0) c.ct->sf_n[(r + 2) % 3] = 0;
+  const uint32_t* dmin_cur = c.q_dmin + (size_t)(r % 3) * c.B;
+  uint32_t* dmin_nxt = c.q_dmin + (size_t)((r + 1) % 3) * c.B;
+  uint32_t* dmin_old = c.q_dmin + (size_t)((r + 2) % 3) * c.B;
+  for (long long q = gtid; q < c.B; q += (long long)gridDim.x * blockDim.x)
+    dmin_old[q] = __float_as_uint(FINF);
   float* dist = WHICH ? c.lbl : c.lbd;
   const int lane = lane_id();
   const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
@@ -276,30 +356,54 @@ __global__ void __launch_bounds__(256) k_sssp(Ctx c) {
     long long i = base + lane;
     int deg = 0, row = 0, q = 0;
     float du = 0.f;
+    bool carry = false;
+    long long qv = 0;
     if (i < n_in) {
-      long long qv = c.sfr[cur][i];
+      qv = c.sfr[cur][i];
       q = (int)(qv / c.n);
       int u = (int)(qv - (long long)q * c.n);
-      du = dist[qv];
-      row = c.rrow_ptr[u];
-      deg = c.rrow_ptr[u + 1] - row;
+      du = load_agent(&dist[qv]);
+      if (du > __uint_as_float(dmin_cur[q]) + delta) {
+        carry = true;
+      } else {
+        // leave the frontier first, then read the final distance: a later
+        // decrease by another lane sees inq == 0 and re-queues the vertex
+        c.inq[qv] = 0;
+        __threadfence();
+        du = load_agent(&dist[qv]);
+        row = c.rrow_ptr[u];
+        deg = c.rrow_ptr[u + 1] - row;
+      }
+    }
+    int cslot = wave_alloc32(&c.ct->sf_n[(r + 1) % 3], carry);
+    if (carry) {
+      if (cslot < c.sf_cap) {
+        c.sfr[nxt][cslot] = qv;
+        atomicMin(&dmin_nxt[q], __float_as_uint(du));
+      } else {
+        atomicOr(&c.ct->overflow, OVF_SSSP);
+      }
     }
     wave_flat(deg, row, [&](int owner, int pos, bool act) {
       float odu = __shfl(du, owner, WAVE);
       int oq = __shfl(q, owner, WAVE);
       bool push = false;
       long long wv = 0;
+      float nd = 0.f;
       if (act) {
         relaxed++;
         int4 e = c.redges[pos];
         float wt = WHICH ? __int_as_float(e.y) : __int_as_float(e.z);
-        float nd = odu + wt;
+        nd = odu + wt;
         float bnd = c.q_bound[oq];
         wv = (long long)oq * c.n + e.x;
         bool ok = WHICH ? (c.lbd[wv] <= bnd) : (nd <= bnd);
         if (ok && nd < dist[wv]) {
           unsigned old = atomicMin((unsigned*)&dist[wv], __float_as_uint(nd));
-          if (__float_as_uint(nd) < old) push = (atomicExch(&c.stamp[wv], r) != r);
+          if (__float_as_uint(nd) < old) {
+            if (__float_as_uint(nd) < load_agent_u32(&dmin_nxt[oq])) atomicMin(&dmin_nxt[oq], __float_as_uint(nd));
+            push = (atomicExch(&c.inq[wv], 1u) == 0u);
+          }
         }
       }
       int slot = wave_alloc32(&c.ct->sf_n[(r + 1) % 3], push);
@@ -310,17 +414,6 @@ __global__ void __launch_bounds__(256) k_sssp(Ctx c) {
     });
   }
   wave_add(&c.ct->sssp_relaxed, relaxed);
-  // last block advances the round
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned prev = atomicAdd(&c.ct->blocks_done, 1u);
-    if (prev == gridDim.x - 1) {
-      c.ct->blocks_done = 0;
-      c.ct->sround = r + 1;
-      __threadfence();
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -340,6 +433,7 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ct->lstart[nxt] = lend;
     ct->round_m = r;
+    c.ts[r & (TS_CAP - 1)] = wall_clock64();  // start of round r
   }
   const int lane = lane_id();
   const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
@@ -353,26 +447,39 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
     int id = -1, deg = 0, row = 0, q = 0;
     Label_t lab;
     bool carry = false;
-    float key = 0.f;
+    float key = 0.f, rd = 0.f;
     if (i < fn) {
-      id = (i < ncarry) ? c.carry[cur][i] : (int)(lstart + (i - ncarry));
+      // carried entries bring their query, key and pruning delay along, so a
+      // label that stays outside the window costs one 16-byte read per round
+      bool fresh = i >= ncarry;
+      if (!fresh) {
+        int4 cr = c.carry[cur][i];
+        id = cr.x;
+        q = cr.y;
+        key = __int_as_float(cr.z);
+        rd = __int_as_float(cr.w);
+      } else {
+        id = (int)(lstart + (i - ncarry));
+      }
       if (!c.dead[id]) {
-        int4 L = c.labs[id];
-        int2 meta = c.lmeta[id];
-        q = meta.x;
-        key = __int_as_float(meta.y);
-        lab = LT::unpack(L.x, make_float2(__int_as_float(L.y), __int_as_float(L.z)));
         bool live = true;
-        if (PRUNE != 0 && lab.vid == c.q_dst[q]) live = false;  // answers are not extended
-        if (live && PRUNE == 1) {
-          float2 b = c.q_best[q];
-          if (lex_ge(lab.d, lab.delay, b.x, b.y)) live = false;
+        if (fresh) {
+          int4 L = c.labs[id];
+          int2 meta = c.lmeta[id];
+          q = meta.x;
+          key = __int_as_float(meta.y);
+          lab = LT::unpack(L.x, make_float2(__int_as_float(L.y), __int_as_float(L.z)));
+          rd = lab.delay;
+          if (PRUNE != 0 && lab.vid == c.q_dst[q]) live = false;  // answers are not extended
+          if (live && PRUNE == 2) {
+            rd = lab.delay + c.lbd[(long long)q * c.n + lab.vid];
+            if (rd > c.q_bound[q]) live = false;
+          }
         }
-        if (live && PRUNE == 2) {
+        // (key, rd) is (d, delay) without bounds and (d + lbl, delay + lbd) with them
+        if (live && PRUNE != 0) {
           float2 b = c.q_best[q];
-          long long qv = (long long)q * c.n + lab.vid;
-          float rd = lab.delay + c.lbd[qv];
-          if (rd > c.q_bound[q] || lex_ge(key, rd, b.x, b.y)) live = false;
+          if (lex_ge(key, rd, b.x, b.y)) live = false;
         }
         if (live) {
           c.q_last[q] = r;
@@ -380,6 +487,10 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
           if (dop > 0) in_window = (i >= fn - dop);
           else in_window = !(key > __uint_as_float(fmin_cur[q]) + c.q_delta[q]);
           if (in_window) {
+            if (!fresh) {
+              int4 L = c.labs[id];
+              lab = LT::unpack(L.x, make_float2(__int_as_float(L.y), __int_as_float(L.z)));
+            }
             row = c.row_ptr[lab.vid];
             deg = c.row_ptr[lab.vid + 1] - row;
             n_exp++;
@@ -393,8 +504,8 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
     int cslot = wave_alloc32(&ct->carry_n[nxt], carry);
     if (carry) {
       if (cslot < c.carry_cap) {
-        c.carry[nxt][cslot] = id;
-        if (__float_as_uint(key) < fmin_nxt[q]) atomicMin(&fmin_nxt[q], __float_as_uint(key));
+        c.carry[nxt][cslot] = make_int4(id, q, __float_as_int(key), __float_as_int(rd));
+        if (__float_as_uint(key) < load_agent_u32(&fmin_nxt[q])) atomicMin(&fmin_nxt[q], __float_as_uint(key));
       } else {
         atomicOr(&ct->overflow, OVF_CARRY);
       }
@@ -431,14 +542,7 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
         if (ok) {
           // read-only dominance check against the Pareto set (no writer runs now)
           int2 ps = c.pset[qv];
-          int cnt = pset_cnt(ps);
-          const float2* ent = c.pool_cd + (uint32_t)ps.x;
-          for (int j = 0; j < cnt; j++) {
-            if (Label_t::dominate(LT::unpack(son.vid, ent[j]), son)) {
-              ok = false;
-              break;
-            }
-          }
+          if (set_dominates<Label_t>(c.pool_cd + (uint32_t)ps.x, pset_cnt(ps), son.vid, son)) ok = false;
         }
       }
       int cid = wave_alloc32(&ct->cand_ctr[cur], ok);
@@ -484,6 +588,7 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
     ct->touched_n[nxt] = 0;
     ct->cand_ctr[nxt] = 0;
     ct->carry_n[cur] = 0;
+    ct->round_e = r + 1;  // expand of this round has finished reading it
   }
   for (long long q = gtid; q < c.B; q += (long long)gridDim.x * blockDim.x)
     c.q_fmin[(size_t)cur * c.B + q] = __float_as_uint(FINF);
@@ -512,6 +617,7 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
       if (PRUNE == 2) lb = c.lbl[qv];
     }
     int steps = 0;
+    int scanned = 0;
     while (__ballot(h >= 0)) {
       if (h >= 0 && ++steps > max_steps) {
         atomicOr(&ct->overflow, OVF_CAND);
@@ -524,24 +630,9 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
         cv = c.cand[h];
         par = c.cand_par[h];
         Label_t cl = LT::unpack(v, cv);
-        bool dom = false;
-        for (int j = 0; j < cnt; j++) {
-          if (Label_t::dominate(LT::unpack(v, c.pool_cd[off + j]), cl)) {
-            dom = true;
-            break;
-          }
-        }
-        if (!dom) {
-          for (int j = 0; j < cnt;) {
-            if (Label_t::dominate(cl, LT::unpack(v, c.pool_cd[off + j]))) {
-              c.dead[c.pool_id[off + j]] = 1;
-              cnt--;
-              c.pool_cd[off + j] = c.pool_cd[off + cnt];
-              c.pool_id[off + j] = c.pool_id[off + cnt];
-            } else {
-              j++;
-            }
-          }
+        scanned += cnt;
+        if (!set_dominates<Label_t>(c.pool_cd + off, cnt, v, cl)) {
+          cnt = set_evict<Label_t>(c.pool_cd + off, c.pool_id + off, cnt, v, cl, c.dead);
           acc = true;
         }
       }
@@ -586,27 +677,21 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
               c.q_best_id[q] = (int)lid;
             }
           }
-          if (!(PRUNE != 0 && v == tq) && __float_as_uint(key) < fmin_nxt[q])
+          if (!(PRUNE != 0 && v == tq) && __float_as_uint(key) < load_agent_u32(&fmin_nxt[q]))
             atomicMin(&fmin_nxt[q], __float_as_uint(key));
         }
       }
       if (h >= 0) h = c.cand_next[h];
     }
     if (i < tn) c.pset[qv] = pset_make(off, cnt, cap);
+    int smax = steps;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) smax = max(smax, __shfl_xor(smax, o, WAVE));
+    if (lane == 0 && smax > 0) atomicMax(&ct->merge_chain_max, smax);
+    wave_add(&ct->merge_scanned, scanned);
+    wave_add(&ct->touched_total, i < tn ? 1 : 0);
   }
   wave_add(&ct->inserted, n_ins);
-  // last block stamps the end of the round and advances the round counter
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned prev = atomicAdd(&ct->blocks_done, 1u);
-    if (prev == gridDim.x - 1) {
-      ct->blocks_done = 0;
-      c.ts[r & (TS_CAP - 1)] = wall_clock64();
-      ct->round_e = r + 1;
-      __threadfence();
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -636,6 +721,7 @@ __global__ void k_paths(Ctx c, int nq, const long long* off, int32_t* path) {
 }
 
 __global__ void k_stamp_start(Ctx c) { c.ct->t_start = wall_clock64(); }
+__global__ void k_stamp_round(Ctx c, int r) { c.ts[r & (TS_CAP - 1)] = wall_clock64(); }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -45,7 +45,8 @@ class Options(ctypes.Structure):
         ("ar_max_labels", ctypes.c_double),
         ("record_trace", ctypes.c_int32),
         ("time_kernels", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("bound_delta", ctypes.c_float),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -68,7 +69,10 @@ class Stats(ctypes.Structure):
         ("expand_launches", ctypes.c_int64),
         ("merge_launches", ctypes.c_int64),
         ("sssp_launches", ctypes.c_int64),
-        ("reserved", ctypes.c_int64 * 6),
+        ("touched", ctypes.c_int64),
+        ("merge_scanned", ctypes.c_int64),
+        ("merge_chain_max", ctypes.c_int64),
+        ("reserved", ctypes.c_int64 * 3),
     ]
 
     def as_dict(self):
