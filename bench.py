@@ -73,7 +73,7 @@ def main():
     solver = S.Solver(dg, max_batch=args.batch, prune=S.PRUNE_BOUNDS, delta_policy=policy,
                       delta_fixed=args.delta, rounds_per_sync=16, time_kernels=1,
                       label_capacity=1_000_000_000, pool_capacity=2_000_000_000,
-                      cand_capacity=400_000_000, frontier_capacity=400_000_000)
+                      cand_capacity=100_000_000, frontier_capacity=400_000_000)
     log("solver ready (batch %d)" % args.batch)
     nq = len(qb)
     d_src = torch.from_numpy(qb.src).to(dev)

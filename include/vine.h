@@ -78,7 +78,7 @@ typedef struct {
   int32_t grid_blocks;      /* workgroups per round kernel (0 = auto) */
   int32_t hash_slots;       /* LDS hash slots per workgroup for candidate pre-merge (0 = off) */
   int64_t label_capacity;   /* label log entries (0 = auto from free memory) */
-  int64_t cand_capacity;    /* candidate entries per round (0 = auto) */
+  int64_t cand_capacity;    /* candidate slots per round (0 = auto); half as many list chunks are allocated */
   int64_t pool_capacity;    /* Pareto-set pool entries (0 = auto) */
   int64_t frontier_capacity;/* carried-frontier entries (0 = auto) */
   double ar_max_labels;     /* MAX for the DOP rule (0 = free device memory / 12 B, as the reference) */

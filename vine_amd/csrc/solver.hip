@@ -100,8 +100,8 @@ struct vine_solver {
   int4* d_labs = nullptr;
   int2* d_lmeta = nullptr;
   uint8_t* d_dead = nullptr;
-  float2* d_cand = nullptr;
-  int32_t *d_cand_par = nullptr, *d_cand_next = nullptr;
+  int2* d_chd = nullptr;
+  int4* d_crec = nullptr;
   long long* d_touched = nullptr;
   int4* d_carry[2] = {nullptr, nullptr};
   long long* d_sfr[2] = {nullptr, nullptr};
@@ -335,12 +335,12 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
     return VINE_E_OOM;
   }
   // auto capacities: share the remaining memory (labels 25 B, pool 12 B x 2,
-  // candidates 24 B, carry 32 B, bound frontier 16 B per entry)
+  // candidate chunks 144 B per two candidate slots, carry 32 B, bound frontier 16 B)
   long long lab_cap = o.label_capacity;
   if (!lab_cap) {
     // sized to the problem (64 labels per (query, vertex) pair plus slack),
     // capped by 60% of the free memory
-    double per = 25.0 + 2 * 12.0 + 0.5 * 24.0 + 0.5 * 32.0 + 0.25 * 16.0;
+    double per = 25.0 + 2 * 12.0 + 0.25 * 144.0 + 0.5 * 32.0 + 0.25 * 16.0;
     double want = 64.0 * (double)BN + (double)(1 << 22);
     lab_cap = (long long)std::min(std::min(remaining * 0.6 / per, want), (double)INT32_MAX - 1);
     lab_cap = std::max<long long>(lab_cap, 4 * B + 1024);
@@ -385,10 +385,12 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   ALLOC(s->d_labs, sizeof(int4) * lab_cap);
   ALLOC(s->d_lmeta, sizeof(int2) * lab_cap);
   ALLOC(s->d_dead, lab_cap);
-  ALLOC(s->d_cand, sizeof(float2) * cand_cap);
-  ALLOC(s->d_cand_par, sizeof(int32_t) * cand_cap);
-  ALLOC(s->d_cand_next, sizeof(int32_t) * cand_cap);
-  ALLOC(s->d_touched, sizeof(long long) * cand_cap);
+  // candidates travel in chunks of dev::CCHUNK; a round needs one chunk per
+  // touched (query, vertex) plus one per CCHUNK further candidates
+  long long chunk_cap = std::max<long long>(cand_cap / 2, 1024);
+  ALLOC(s->d_chd, sizeof(int2) * chunk_cap);
+  ALLOC(s->d_crec, sizeof(int4) * dev::CCHUNK * chunk_cap);
+  ALLOC(s->d_touched, sizeof(long long) * chunk_cap);
   ALLOC(s->d_carry[0], sizeof(int4) * carry_cap);
   ALLOC(s->d_carry[1], sizeof(int4) * carry_cap);
   ALLOC(s->d_ts, sizeof(unsigned long long) * dev::TS_CAP);
@@ -428,11 +430,10 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   c.lmeta = s->d_lmeta;
   c.dead = s->d_dead;
   c.label_cap = lab_cap;
-  c.cand = s->d_cand;
-  c.cand_par = s->d_cand_par;
-  c.cand_next = s->d_cand_next;
+  c.chd = s->d_chd;
+  c.crec = s->d_crec;
   c.touched = s->d_touched;
-  c.cand_cap = (int32_t)cand_cap;
+  c.chunk_cap = (int32_t)chunk_cap;
   c.carry[0] = s->d_carry[0];
   c.carry[1] = s->d_carry[1];
   c.carry_cap = (int32_t)carry_cap;
@@ -968,16 +969,26 @@ int64_t vine_solver_pareto(const vine_solver* s, int32_t q, int32_t v, float* ou
     set_err("vine_solver_pareto: %s", hipGetErrorString(e));
     return VINE_E_HIP;
   }
-  int64_t cnt = ps.y & ((1 << 26) - 1);
-  int64_t k = std::min(cnt, cap);
-  if (k > 0) {
-    e = hipMemcpy(out_pairs, s->d_pool_cd + (uint32_t)ps.x, sizeof(float2) * k, hipMemcpyDeviceToHost);
+  // the stored set may hold tombstones (+inf, +inf) left by evictions
+  int64_t slots = ps.y & ((1 << 26) - 1);
+  std::vector<float2> tmp(slots);
+  if (slots > 0) {
+    e = hipMemcpy(tmp.data(), s->d_pool_cd + (uint32_t)ps.x, sizeof(float2) * slots, hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
       set_err("vine_solver_pareto: %s", hipGetErrorString(e));
       return VINE_E_HIP;
     }
   }
-  return cnt;
+  int64_t live = 0;
+  for (const float2& p : tmp) {
+    if (std::isinf(p.x)) continue;
+    if (live < cap) {
+      out_pairs[2 * live] = p.x;
+      out_pairs[2 * live + 1] = p.y;
+    }
+    live++;
+  }
+  return live;
 }
 
 }  // extern "C"

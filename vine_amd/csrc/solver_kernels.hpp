@@ -36,13 +36,14 @@ constexpr float FINF = __builtin_huge_valf();
 constexpr float EXACT_LIMIT = 16777216.0f;  // 2^24
 constexpr int INIT_CAP = 4;                 // first Pareto-set allocation (entries)
 constexpr int TS_CAP = 1 << 20;             // per-round timestamps kept
+constexpr int CCHUNK = 8;                   // candidate records per list chunk
 
 enum : int { OVF_LABEL = 1, OVF_CAND = 2, OVF_TOUCH = 4, OVF_POOL = 8, OVF_CARRY = 16, OVF_SSSP = 32 };
 
 struct Counters {
   unsigned long long label_ctr;  // labels in the log
   unsigned long long pool_ctr;   // Pareto pool entries handed out
-  int cand_ctr[2];
+  int cand_ctr[2];   // candidate chunks handed out this round
   int touched_n[2];
   int carry_n[2];
   long long lstart[2];   // first label of the frontier's "new" range, per parity
@@ -96,11 +97,12 @@ struct Ctx {
   int2* lmeta;       // {query slot, key bits}
   uint8_t* dead;
   long long label_cap;
-  float2* cand;
-  int32_t* cand_par;
-  int32_t* cand_next;
+  // per-round candidates: per (query, vertex) a list of chunks of CCHUNK
+  // records {d bits, delay bits, parent label, 0}; chunk header {count, next}
+  int2* chd;
+  int4* crec;
   long long* touched;
-  int32_t cand_cap;
+  int32_t chunk_cap;
   int4* carry[2];    // carried frontier entries {label id, query slot, key bits, pruning delay bits}
   int32_t carry_cap;
   long long* sfr[2];
@@ -540,28 +542,51 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
           }
         }
         if (ok) {
-          // read-only dominance check against the Pareto set (no writer runs now)
+          // read-only dominance check against small Pareto sets (no writer runs
+          // now; tombstones dominate nothing).  Larger sets are left to the
+          // cooperative check in merge.
           int2 ps = c.pset[qv];
-          if (set_dominates<Label_t>(c.pool_cd + (uint32_t)ps.x, pset_cnt(ps), son.vid, son)) ok = false;
+          int cnt = pset_cnt(ps);
+          if (cnt <= SCAN_CHUNK && set_dominates<Label_t>(c.pool_cd + (uint32_t)ps.x, cnt, son.vid, son)) ok = false;
         }
       }
-      int cid = wave_alloc32(&ct->cand_ctr[cur], ok);
+      // lock-free append to the (query, vertex) chunk list: claim a slot in
+      // the head chunk, or publish a new head chunk holding the record
       bool first = false;
       if (ok) {
-        if (cid < c.cand_cap) {
-          n_cand++;
-          c.cand[cid] = LT::pack(son);
-          c.cand_par[cid] = oid;
-          int old = atomicExch(&c.cand_head[qv], cid);
-          c.cand_next[cid] = old;
-          first = (old < 0);
-        } else {
-          atomicOr(&ct->overflow, OVF_CAND);
+        n_cand++;
+        float2 pk = LT::pack(son);
+        int4 rec = make_int4(__float_as_int(pk.x), __float_as_int(pk.y), oid, 0);
+        int h = __hip_atomic_load(&c.cand_head[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool done = false;
+        if (h >= 0) {
+          int slot = atomicAdd(&c.chd[h].x, 1);
+          if (slot < CCHUNK) {
+            c.crec[(long long)h * CCHUNK + slot] = rec;
+            done = true;
+          }
+        }
+        if (!done) {
+          int nc = atomicAdd(&ct->cand_ctr[cur], 1);
+          if (nc >= c.chunk_cap) {
+            atomicOr(&ct->overflow, OVF_CAND);
+          } else {
+            c.crec[(long long)nc * CCHUNK] = rec;
+            int expected = h;
+            for (;;) {
+              c.chd[nc] = make_int2(1, expected);
+              __threadfence();
+              int prev = atomicCAS(&c.cand_head[qv], expected, nc);
+              if (prev == expected) break;
+              expected = prev;  // another lane published first: link behind it
+            }
+            first = (expected < 0);
+          }
         }
       }
       int tix = wave_alloc32(&ct->touched_n[cur], first);
       if (first) {
-        if (tix < c.cand_cap) c.touched[tix] = qv;
+        if (tix < c.chunk_cap) c.touched[tix] = qv;
         else atomicOr(&ct->overflow, OVF_TOUCH);
       }
     });
@@ -573,16 +598,29 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
 
 // ---------------------------------------------------------------------------
 // Merge: fold each touched (query, vertex)'s candidates into its Pareto set.
+// A group of MG lanes owns one (query, vertex): lane g holds set entries
+// g, g+MG, g+2MG, ...; for each candidate (broadcast from the lane that loaded
+// its chunk record) the group decides "dominated" by ballot, evicts the entries
+// the candidate dominates by overwriting them with a tombstone (+inf, +inf),
+// which nothing dominates and which later insertions reuse, and appends the
+// survivor.  Compaction happens only when the set outgrows its storage.
+// Loops are wave-uniform (ballot over the wave), predicates group-uniform, so
+// label and pool allocations aggregate over the whole wave.
+constexpr int MG = 16;     // lanes per (query, vertex) group
+constexpr int MCACHE = 4;  // set entries per lane kept in registers between scan and evict
+
+__device__ __forceinline__ bool is_tomb(float2 e) { return e.x == FINF; }
+
 template <typename Label_t, int PRUNE>
 __global__ void __launch_bounds__(256) k_merge(Ctx c) {
   using LT = label_traits<Label_t>;
   Counters* ct = c.ct;
   const int r = ct->round_m;
   const int cur = r & 1, nxt = cur ^ 1;
-  const int tn = min(ct->touched_n[cur], c.cand_cap);
-  // a candidate list never holds more than this round's candidates; the walk
-  // is cut there so that state left by an overflowed round cannot loop
-  const int max_steps = min(ct->cand_ctr[cur], c.cand_cap);
+  const int tn = min(ct->touched_n[cur], c.chunk_cap);
+  // a list never holds more chunks than this round allocated; the walk is cut
+  // there so that state left by an overflowed round cannot loop
+  const int max_steps = min(ct->cand_ctr[cur], c.chunk_cap);
   const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gtid == 0) {
     ct->touched_n[nxt] = 0;
@@ -594,21 +632,27 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
     c.q_fmin[(size_t)cur * c.B + q] = __float_as_uint(FINF);
   uint32_t* fmin_nxt = c.q_fmin + (size_t)nxt * c.B;
   const int lane = lane_id();
+  const int gl = lane & (MG - 1);
+  const int gbase = lane & ~(MG - 1);
+  const unsigned long long gmask = ((1ull << MG) - 1ull) << gbase;
+  constexpr int GPW = WAVE / MG;  // groups per wave
   const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
-  int n_ins = 0;
-  for (long long base = ((long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * WAVE;
-       base < tn; base += nwaves * WAVE) {
-    long long i = base + lane;
+  int n_ins = 0, scanned = 0, steps_max = 0;
+  const float2 TOMB = make_float2(FINF, FINF);
+
+  for (long long base = ((long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * GPW;
+       base < tn; base += nwaves * GPW) {
+    const long long ti = base + lane / MG;
+    const bool have = ti < tn;
     long long qv = 0;
     int q = 0, v = 0, h = -1, cnt = 0, cap = 0, tq = -1;
     uint32_t off = 0;
     float lb = 0.f;
-    if (i < tn) {
-      qv = c.touched[i];
+    if (have) {
+      qv = c.touched[ti];
       q = (int)(qv / c.n);
       v = (int)(qv - (long long)q * c.n);
       h = c.cand_head[qv];
-      c.cand_head[qv] = -1;
       int2 ps = c.pset[qv];
       off = (uint32_t)ps.x;
       cnt = pset_cnt(ps);
@@ -616,81 +660,162 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
       tq = c.q_dst[q];
       if (PRUNE == 2) lb = c.lbl[qv];
     }
+    if (have && gl == 0) c.cand_head[qv] = -1;
     int steps = 0;
-    int scanned = 0;
     while (__ballot(h >= 0)) {
       if (h >= 0 && ++steps > max_steps) {
         atomicOr(&ct->overflow, OVF_CAND);
         h = -1;
       }
-      bool acc = false;
-      float2 cv = make_float2(0.f, 0.f);
-      int par = -1;
+      int n8 = 0, nexth = -1;
+      int4 myrec = make_int4(0, 0, -1, 0);
       if (h >= 0) {
-        cv = c.cand[h];
-        par = c.cand_par[h];
-        Label_t cl = LT::unpack(v, cv);
-        scanned += cnt;
-        if (!set_dominates<Label_t>(c.pool_cd + off, cnt, v, cl)) {
-          cnt = set_evict<Label_t>(c.pool_cd + off, c.pool_id + off, cnt, v, cl, c.dead);
-          acc = true;
-        }
+        int2 hd = c.chd[h];
+        n8 = min(hd.x, CCHUNK);
+        nexth = hd.y;
+        if (gl < n8) myrec = c.crec[(long long)h * CCHUNK + gl];
       }
-      bool grow = acc && cnt == cap;
-      int newcap = grow ? (cap ? cap * 2 : INIT_CAP) : 0;
-      long long poff = wave_alloc64_n(&ct->pool_ctr, newcap);
-      long long lid = wave_alloc64(&ct->label_ctr, acc);
-      if (acc) {
-        bool fits = true;
+      int kmax = n8;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o, WAVE));
+      for (int k = 0; k < kmax; k++) {
+        const bool act = k < n8;  // group-uniform
+        const float cd = __int_as_float(__shfl(myrec.x, gbase + k, WAVE));
+        const float cdl = __int_as_float(__shfl(myrec.y, gbase + k, WAVE));
+        const int par = __shfl(myrec.z, gbase + k, WAVE);
+        const Label_t cl = LT::unpack(v, make_float2(cd, cdl));
+        // scan my slice: dominated?  first free slot?
+        float2 cache[MCACHE];
+        bool dom = false;
+        int free_slot = INT32_MAX;
+        if (act) {
+#pragma unroll
+          for (int t = 0; t < MCACHE; t++) {
+            int j = gl + t * MG;
+            cache[t] = (j < cnt) ? c.pool_cd[off + j] : TOMB;
+          }
+#pragma unroll
+          for (int t = 0; t < MCACHE; t++) {
+            int j = gl + t * MG;
+            if (j < cnt) {
+              if (is_tomb(cache[t])) free_slot = min(free_slot, j);
+              else dom |= Label_t::dominate(LT::unpack(v, cache[t]), cl);
+            }
+          }
+          for (int j = gl + MCACHE * MG; j < cnt; j += MG) {
+            float2 e = c.pool_cd[off + j];
+            if (is_tomb(e)) free_slot = min(free_slot, j);
+            else dom |= Label_t::dominate(LT::unpack(v, e), cl);
+          }
+          scanned += (cnt + MG - 1) / MG;
+        }
+        const bool dominated = (__ballot(dom) & gmask) != 0ull;
+        const bool acc = act && !dominated;
+        if (acc) {
+          // evict what the candidate dominates; evicted slots become free
+#pragma unroll
+          for (int t = 0; t < MCACHE; t++) {
+            int j = gl + t * MG;
+            if (j < cnt && !is_tomb(cache[t]) && Label_t::dominate(cl, LT::unpack(v, cache[t]))) {
+              c.dead[c.pool_id[off + j]] = 1;
+              c.pool_cd[off + j] = TOMB;
+              free_slot = min(free_slot, j);
+            }
+          }
+          for (int j = gl + MCACHE * MG; j < cnt; j += MG) {
+            float2 e = c.pool_cd[off + j];
+            if (!is_tomb(e) && Label_t::dominate(cl, LT::unpack(v, e))) {
+              c.dead[c.pool_id[off + j]] = 1;
+              c.pool_cd[off + j] = TOMB;
+              free_slot = min(free_slot, j);
+            }
+          }
+        }
+        // group minimum of the free slots (cnt means: append)
+        int slot = acc ? min(free_slot, cnt) : INT32_MAX;
+#pragma unroll
+        for (int o = MG / 2; o > 0; o >>= 1) slot = min(slot, __shfl_xor(slot, o, WAVE));
+        const bool grow = acc && slot == cnt && cnt == cap;
+        // growth: new storage of twice the live size, compacted (leader allocates)
+        int live = 0;
         if (grow) {
-          if ((unsigned long long)(poff + newcap) > c.pool_cap || poff + newcap > 0xFFFFFFFFll) {
-            atomicOr(&ct->overflow, OVF_POOL);
+          for (int j = gl; j < cnt; j += MG) live += is_tomb(c.pool_cd[off + j]) ? 0 : 1;
+        }
+        int live_incl = live;
+#pragma unroll
+        for (int o = 1; o < MG; o <<= 1) {
+          int tt = __shfl_up(live_incl, o, MG);
+          if (gl >= o) live_incl += tt;
+        }
+        const int live_tot = __shfl(live_incl, gbase + MG - 1, WAVE);
+        int newcap = 0;
+        if (grow) {
+          newcap = INIT_CAP;
+          while (newcap < 2 * (live_tot + 1)) newcap <<= 1;
+        }
+        long long poff = wave_alloc64_n(&ct->pool_ctr, (grow && gl == 0) ? newcap : 0);
+        poff = __shfl(poff, gbase, WAVE);
+        long long lid = wave_alloc64(&ct->label_ctr, acc && gl == 0);
+        lid = __shfl(lid, gbase, WAVE);
+        bool fits = acc;
+        if (grow) {
+          if (poff < 0 || (unsigned long long)(poff + newcap) > c.pool_cap || poff + newcap > 0xFFFFFFFFll) {
+            if (gl == 0) atomicOr(&ct->overflow, OVF_POOL);
             fits = false;
           } else {
-            for (int j = 0; j < cnt; j++) {
-              c.pool_cd[poff + j] = c.pool_cd[off + j];
-              c.pool_id[poff + j] = c.pool_id[off + j];
+            // copy live entries to the new storage in group order
+            int w = live_incl - live;
+            for (int j = gl; j < cnt; j += MG) {
+              float2 e = c.pool_cd[off + j];
+              if (!is_tomb(e)) {
+                c.pool_cd[poff + w] = e;
+                c.pool_id[poff + w] = c.pool_id[off + j];
+                w++;
+              }
             }
             off = (uint32_t)poff;
             cap = newcap;
+            cnt = live_tot;
+            slot = live_tot;
           }
         }
-        if (lid >= c.label_cap) {
-          atomicOr(&ct->overflow, OVF_LABEL);
+        if (fits && lid >= c.label_cap) {
+          if (gl == 0) atomicOr(&ct->overflow, OVF_LABEL);
           fits = false;
         }
         if (fits) {
-          n_ins++;
-          c.pool_cd[off + cnt] = cv;
-          c.pool_id[off + cnt] = (int)lid;
-          cnt++;
-          Label_t cl = LT::unpack(v, cv);
-          float key = LT::key(cl) + lb;
-          c.labs[lid] = make_int4(v, __float_as_int(cv.x), __float_as_int(cv.y), par);
-          c.lmeta[lid] = make_int2(q, __float_as_int(key));
-          c.dead[lid] = 0;
-          if (!(cv.x < EXACT_LIMIT) || !(cv.y < EXACT_LIMIT)) ct->range_flag = 1;
-          if (v == tq) {
-            float2 b = c.q_best[q];
-            if (cv.x < b.x || (cv.x == b.x && cv.y < b.y)) {
-              c.q_best[q] = cv;
-              c.q_best_id[q] = (int)lid;
+          if (gl == 0) {
+            n_ins++;
+            c.pool_cd[off + slot] = make_float2(cd, cdl);
+            c.pool_id[off + slot] = (int)lid;
+            float key = LT::key(cl) + lb;
+            c.labs[lid] = make_int4(v, __float_as_int(cd), __float_as_int(cdl), par);
+            c.lmeta[lid] = make_int2(q, __float_as_int(key));
+            c.dead[lid] = 0;
+            if (!(cd < EXACT_LIMIT) || !(cdl < EXACT_LIMIT)) ct->range_flag = 1;
+            if (v == tq) {
+              float2 b = c.q_best[q];
+              if (cd < b.x || (cd == b.x && cdl < b.y)) {
+                c.q_best[q] = make_float2(cd, cdl);
+                c.q_best_id[q] = (int)lid;
+              }
             }
+            if (!(PRUNE != 0 && v == tq) && __float_as_uint(key) < load_agent_u32(&fmin_nxt[q]))
+              atomicMin(&fmin_nxt[q], __float_as_uint(key));
           }
-          if (!(PRUNE != 0 && v == tq) && __float_as_uint(key) < load_agent_u32(&fmin_nxt[q]))
-            atomicMin(&fmin_nxt[q], __float_as_uint(key));
+          if (slot == cnt) cnt++;
         }
       }
-      if (h >= 0) h = c.cand_next[h];
+      h = nexth;
     }
-    if (i < tn) c.pset[qv] = pset_make(off, cnt, cap);
-    int smax = steps;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) smax = max(smax, __shfl_xor(smax, o, WAVE));
-    if (lane == 0 && smax > 0) atomicMax(&ct->merge_chain_max, smax);
-    wave_add(&ct->merge_scanned, scanned);
-    wave_add(&ct->touched_total, i < tn ? 1 : 0);
+    if (have && gl == 0) c.pset[qv] = pset_make(off, cnt, cap);
+    steps_max = max(steps_max, steps);
+    wave_add(&ct->touched_total, (have && gl == 0) ? 1 : 0);
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) steps_max = max(steps_max, __shfl_xor(steps_max, o, WAVE));
+  if (lane == 0 && steps_max > 0) atomicMax(&ct->merge_chain_max, steps_max);
+  wave_add(&ct->merge_scanned, scanned);
   wave_add(&ct->inserted, n_ins);
 }
 
