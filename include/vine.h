@@ -109,7 +109,12 @@ typedef struct {
   int64_t touched;          /* (query, vertex) Pareto sets merged into */
   int64_t merge_scanned;    /* Pareto entries compared by the merge kernel */
   int64_t merge_chain_max;  /* longest candidate list folded by one lane */
-  int64_t reserved[3];
+  int64_t far_pulled;       /* frontier entries moved from far tiers to the near tier */
+  double ms_pull;           /* HIP-event time of the far-tier pull kernels */
+  int64_t near_carried;     /* frontier entries re-queued in the near tier (summed over rounds) */
+  int64_t far_appended;     /* frontier entries parked in far tiers */
+  int64_t scanned;          /* frontier entries read by the expand kernels */
+  int64_t reserved[1];
 } vine_stats;
 
 void vine_options_default(vine_options* opt);

@@ -39,11 +39,31 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def _struct_fields(name):
+    text = open(os.path.join(ROOT, "include", "vine.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    body = re.search(r"typedef struct \{([^}]*)\}\s*%s;" % name, text).group(1)
+    out = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        m = re.match(r"(\w+)\s+(\w+)(\[(\d+)\])?$", decl)
+        assert m, decl
+        out.append((m.group(2), m.group(1), int(m.group(4) or 1)))
+    return out
+
+
 def test_options_layout_matches_header():
     o = S.default_options()
     assert o.max_batch == 1 and o.prune == S.PRUNE_BOUNDS and o.block_size == 256
-    assert ctypes.sizeof(S.Options) == 112
-    assert ctypes.sizeof(S.Stats) == 8 * 9 + 8 * 5 + 8 * 3 + 8 * 6
+    for cname, pystruct in (("vine_options", S.Options), ("vine_stats", S.Stats)):
+        hdr = _struct_fields(cname)
+        py = [(f[0], f[1]) for f in pystruct._fields_]
+        assert [h[0] for h in hdr] == [p[0] for p in py], cname
+        size = {"int32_t": 4, "int64_t": 8, "float": 4, "double": 8}
+        for (nm, ty, cnt), (_, cty) in zip(hdr, py):
+            assert ctypes.sizeof(cty) == size[ty] * cnt, (cname, nm)
 
 
 def test_errors_without_device():

@@ -91,7 +91,8 @@ struct vine_solver {
   // per (query, vertex)
   int2* d_pset = nullptr;
   int32_t* d_cand_head = nullptr;
-  float *d_lbl = nullptr, *d_lbd = nullptr;
+  float* d_lbl = nullptr;
+  unsigned long long* d_lbd = nullptr;
   uint32_t* d_inq = nullptr;
   uint32_t* d_qdmin = nullptr;
   // pools
@@ -104,6 +105,14 @@ struct vine_solver {
   int4* d_crec = nullptr;
   long long* d_touched = nullptr;
   int4* d_carry[2] = {nullptr, nullptr};
+  int32_t* d_far_head = nullptr;
+  int32_t* d_far_next = nullptr;
+  int32_t* d_far_cnt = nullptr;
+  float* d_far_base = nullptr;
+  float* d_far_bw = nullptr;
+  int2* d_fhd = nullptr;
+  int32_t* d_far_free = nullptr;
+  int4* d_frec = nullptr;
   long long* d_sfr[2] = {nullptr, nullptr};
   unsigned long long* d_ts = nullptr;
   Counters* d_ct = nullptr;
@@ -326,7 +335,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
     return VINE_E_HIP;
   }
   // per-(query, vertex) state
-  size_t per_qv = sizeof(int2) + sizeof(int32_t) + (o.prune == 2 ? 3 * sizeof(float) : 0);
+  size_t per_qv = sizeof(int2) + sizeof(int32_t) + (o.prune == 2 ? 4 * sizeof(float) : 0);
   double remaining = (double)freeb - (double)per_qv * BN - 64.0 * (1 << 20);
   if (remaining <= 0) {
     set_err("vine_solver_create: %lld query slots x %lld vertices need %.1f GB, %.1f GB free",
@@ -374,7 +383,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   ALLOC(s->d_cand_head, sizeof(int32_t) * BN);
   if (o.prune == 2) {
     ALLOC(s->d_lbl, sizeof(float) * BN);
-    ALLOC(s->d_lbd, sizeof(float) * BN);
+    ALLOC(s->d_lbd, sizeof(unsigned long long) * BN);
     ALLOC(s->d_inq, sizeof(uint32_t) * BN);
     ALLOC(s->d_qdmin, sizeof(uint32_t) * 3 * B);
     ALLOC(s->d_sfr[0], sizeof(long long) * sf_cap);
@@ -393,6 +402,18 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   ALLOC(s->d_touched, sizeof(long long) * chunk_cap);
   ALLOC(s->d_carry[0], sizeof(int4) * carry_cap);
   ALLOC(s->d_carry[1], sizeof(int4) * carry_cap);
+  // far tier: chunks of dev::FCH entries from a bump allocator (reset per
+  // batch); sized for four times the near tier's entries
+  long long far_cap = std::max<long long>(4 * carry_cap / dev::FCH, 2 * B + 64);
+  if (far_cap > INT32_MAX - 1) far_cap = INT32_MAX - 1;
+  ALLOC(s->d_far_head, sizeof(int32_t) * B * dev::NBK);
+  ALLOC(s->d_far_next, sizeof(int32_t) * B);
+  ALLOC(s->d_far_cnt, sizeof(int32_t) * B);
+  ALLOC(s->d_far_base, sizeof(float) * B);
+  ALLOC(s->d_far_bw, sizeof(float) * B);
+  ALLOC(s->d_fhd, sizeof(int2) * far_cap);
+  ALLOC(s->d_far_free, sizeof(int32_t) * far_cap);
+  ALLOC(s->d_frec, sizeof(int4) * dev::FCH * far_cap);
   ALLOC(s->d_ts, sizeof(unsigned long long) * dev::TS_CAP);
   ALLOC(s->d_ct, sizeof(Counters));
   if (hipHostMalloc((void**)&s->h_ct, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
@@ -437,6 +458,16 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   c.carry[0] = s->d_carry[0];
   c.carry[1] = s->d_carry[1];
   c.carry_cap = (int32_t)carry_cap;
+  c.far_head = s->d_far_head;
+  c.far_next = s->d_far_next;
+  c.far_cnt = s->d_far_cnt;
+  c.far_base = s->d_far_base;
+  c.far_bw = s->d_far_bw;
+  c.fhd = s->d_fhd;
+  c.far_free = s->d_far_free;
+  c.frec = s->d_frec;
+  c.far_cap = (int32_t)far_cap;
+  c.far_floor = 0.125f * g->mean_len;
   c.sfr[0] = s->d_sfr[0];
   c.sfr[1] = s->d_sfr[1];
   c.sf_cap = (int32_t)sf_cap;
@@ -492,7 +523,8 @@ vine_status resolve_events(vine_solver* s, vine_stats* st) {
     HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
     if (r.kind == 0) st->ms_expand += ms;
     else if (r.kind == 1) st->ms_merge += ms;
-    else st->ms_sssp += ms;
+    else if (r.kind == 2) st->ms_sssp += ms;
+    else st->ms_pull += ms;
   }
   s->evrec.clear();
   s->ev_used = 0;
@@ -524,6 +556,12 @@ vine_status launch_round(vine_solver* s, long long dop, int gsz, vine_stats* st)
   {
     Timer t(s, 1);
     hipLaunchKernelGGL((dev::k_merge<c_label, PRUNE>), dim3(gsz), dim3(s->block), 0, s->stream, c);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(t.stop());
+  }
+  if (s->opt.delta_policy != VINE_DELTA_INF) {
+    Timer t(s, 3);
+    hipLaunchKernelGGL((dev::k_pull<PRUNE>), dim3((s->B + 3) / 4), dim3(256), 0, s->stream, c);
     HIP_TRY(hipGetLastError());
     HIP_TRY(t.stop());
   }
@@ -605,6 +643,7 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
   // reset per-(query, vertex) headers and counters
   HIP_TRY(hipMemsetAsync(s->d_pset, 0, sizeof(int2) * BN, s->stream));
   HIP_TRY(hipMemsetAsync(s->d_cand_head, 0xFF, sizeof(int32_t) * BN, s->stream));
+  HIP_TRY(hipMemsetAsync(s->d_far_head, 0xFF, sizeof(int32_t) * s->B * dev::NBK, s->stream));
   std::memset(s->h_ct, 0, sizeof(Counters));
   s->h_ct->label_ctr = (unsigned long long)s->B;
   s->h_ct->pool_ctr = (unsigned long long)s->B * dev::INIT_CAP;
@@ -617,7 +656,7 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
     if (fs != VINE_OK) return fs;
   }
   if (s->opt.prune == 2) {
-    vine_status fs = fill_f32(s, s->d_lbd, dev::FINF, BN);
+    vine_status fs = fill_f32(s, (float*)s->d_lbd, dev::FINF, 2 * BN);
     if (fs == VINE_OK) fs = fill_f32(s, s->d_lbl, dev::FINF, BN);
     if (fs != VINE_OK) return fs;
     HIP_TRY(hipMemsetAsync(s->d_inq, 0, sizeof(uint32_t) * BN, s->stream));
@@ -668,7 +707,7 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
     if (ps != VINE_OK) return ps;
     const Counters& h = *s->h_ct;
     int par = h.round_e & 1;
-    long long fn = (long long)h.carry_n[par] + ((long long)h.label_ctr - h.lstart[par]);
+    long long fn = (long long)h.carry_n[par] + ((long long)h.label_ctr - h.lstart[par]) + h.far_live;
     st->max_frontier = std::max<int64_t>(st->max_frontier, fn);
     fn_last = std::max<long long>(fn, 1);
     if (s->opt.record_trace) {
@@ -703,6 +742,10 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
   st->touched += (int64_t)s->h_ct->touched_total;
   st->merge_scanned += (int64_t)s->h_ct->merge_scanned;
   st->merge_chain_max = std::max<int64_t>(st->merge_chain_max, s->h_ct->merge_chain_max);
+  st->far_pulled += (int64_t)s->h_ct->far_pulled;
+  st->near_carried += (int64_t)s->h_ct->near_carried;
+  st->far_appended += (int64_t)s->h_ct->far_appended;
+  st->scanned += (int64_t)s->h_ct->scanned;
   st->batches++;
   return VINE_OK;
 }
@@ -830,7 +873,7 @@ vine_status vine_solve_device(vine_solver* s, int32_t nq, const int32_t* d_src, 
   }
   vine_status r = solve_impl(s, nq, d_src, d_dst, d_bound, d_cost, d_delay, d_feasible, nullptr, &st);
   st.ms_total = now_ms() - t0;
-  st.ms_kernels = st.ms_expand + st.ms_merge + st.ms_sssp;
+  st.ms_kernels = st.ms_expand + st.ms_merge + st.ms_sssp + st.ms_pull;
   if (out_stats) *out_stats = st;
   return r;
 }
@@ -913,7 +956,7 @@ vine_status vine_solve(vine_solver* s, int32_t nq, const int32_t* src, const int
   HIP_TRY(hipMemcpy(out_delay, s->d_out_delay, sizeof(float) * nq, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(out_feasible, s->d_out_feas, sizeof(int32_t) * nq, hipMemcpyDeviceToHost));
   st.ms_total = now_ms() - t0;
-  st.ms_kernels = st.ms_expand + st.ms_merge + st.ms_sssp;
+  st.ms_kernels = st.ms_expand + st.ms_merge + st.ms_sssp + st.ms_pull;
   if (out_stats) *out_stats = st;
   if (path_overflow) {
     set_err("vine_solve: path_cap %lld too small (need %lld)", (long long)path_cap,

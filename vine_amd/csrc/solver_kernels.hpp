@@ -37,8 +37,11 @@ constexpr float EXACT_LIMIT = 16777216.0f;  // 2^24
 constexpr int INIT_CAP = 4;                 // first Pareto-set allocation (entries)
 constexpr int TS_CAP = 1 << 20;             // per-round timestamps kept
 constexpr int CCHUNK = 8;                   // candidate records per list chunk
+constexpr int FCH = 32;                     // far-tier records per list chunk
+constexpr float FAR_MULT = 1.0f;            // near tier covers FAR_MULT windows past the front
+constexpr int NBK = 4096;                   // far-tier key buckets per query
 
-enum : int { OVF_LABEL = 1, OVF_CAND = 2, OVF_TOUCH = 4, OVF_POOL = 8, OVF_CARRY = 16, OVF_SSSP = 32 };
+enum : int { OVF_LABEL = 1, OVF_CAND = 2, OVF_TOUCH = 4, OVF_POOL = 8, OVF_CARRY = 16, OVF_SSSP = 32, OVF_FAR = 64 };
 
 struct Counters {
   unsigned long long label_ctr;  // labels in the log
@@ -60,7 +63,11 @@ struct Counters {
   unsigned long long touched_total;  // (query, vertex) merges
   unsigned long long merge_scanned;  // Pareto entries compared in merge
   int merge_chain_max;               // longest candidate list merged by one lane
-  int pad2;
+  int far_chunks;                    // far-tier chunks handed out (bump)
+  int far_free_top;                  // far-tier chunks on the free stack
+  long long far_live;                // records currently in far tiers
+  unsigned long long far_pulled;     // records moved far -> near
+  unsigned long long near_carried, far_appended, scanned;
 };
 
 struct Ctx {
@@ -86,7 +93,9 @@ struct Ctx {
   int2* pset;        // {pool offset, cnt | capcode << 26}
   int32_t* cand_head;
   float* lbl;
-  float* lbd;
+  unsigned long long* lbd;  // packed (delay bits << 32 | len bits): lexicographic
+                            // (delay, len) distance to t; the len part is the
+                            // length of a minimum-delay path
   uint32_t* inq;     // (query, vertex) in the bound-search frontier
   uint32_t* q_dmin;  // [3][B] smallest frontier distance per query (bound search)
   // pools
@@ -103,8 +112,22 @@ struct Ctx {
   int4* crec;
   long long* touched;
   int32_t chunk_cap;
-  int4* carry[2];    // carried frontier entries {label id, query slot, key bits, pruning delay bits}
+  int4* carry[2];    // near tier: carried entries {label id, query slot, key bits, pruning delay bits}
   int32_t carry_cap;
+  // far tier: per query NBK key buckets, bucket b holding the carried entries
+  // with key in [base + b*bw, base + (b+1)*bw) (the last one also everything
+  // beyond) as a list of chunks of FCH entries.  Buckets below far_next are
+  // "pulled": their keys belong to the near tier.
+  int32_t* far_head;  // [B * NBK]
+  int32_t* far_next;  // [B] first bucket not yet pulled
+  int32_t* far_cnt;   // [B] entries in the query's far tier
+  float* far_base;    // [B] smallest possible key of the query
+  float* far_bw;      // [B] bucket width
+  int2* fhd;          // chunk header {count, next}
+  int4* frec;         // chunk records
+  int32_t far_cap;    // chunks
+  int32_t* far_free;  // free stack of chunks (pushed by pull, popped by expand)
+  float far_floor;    // smallest near-tier reach (key units)
   long long* sfr[2];
   int32_t sf_cap;
   unsigned long long* ts;  // [TS_CAP] per-round merge-end wall clock
@@ -204,6 +227,19 @@ __device__ __forceinline__ uint32_t load_agent_u32(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ unsigned long long pack2(float hi, float lo) {
+  return ((unsigned long long)__float_as_uint(hi) << 32) | (unsigned long long)__float_as_uint(lo);
+}
+__device__ __forceinline__ float hi_of(unsigned long long p) { return __uint_as_float((uint32_t)(p >> 32)); }
+__device__ __forceinline__ float lo_of(unsigned long long p) { return __uint_as_float((uint32_t)p); }
+__device__ __forceinline__ float lbd_of(const Ctx& c, long long qv) { return hi_of(c.lbd[qv]); }
+
+__device__ __forceinline__ int far_bucket(const Ctx& c, int q, float key) {
+  float t = (key - c.far_base[q]) / c.far_bw[q];
+  if (!(t < (float)(NBK - 1))) return NBK - 1;
+  return t < 0.f ? 0 : (int)t;
+}
+
 __device__ __forceinline__ int pset_cnt(int2 p) { return p.y & ((1 << 26) - 1); }
 __device__ __forceinline__ int pset_cap(int2 p) {
   int code = (int)((uint32_t)p.y >> 26);
@@ -293,9 +329,27 @@ __global__ void k_init_batch(Ctx c, int nq) {
     c.pool_id[(long long)q * INIT_CAP] = q;
     c.pset[qs] = pset_make((uint32_t)(q * INIT_CAP), 1, INIT_CAP);
     bool st = (s == c.q_dst[q]);
-    c.q_best[q] = st ? make_float2(0.f, 0.f) : make_float2(FINF, FINF);
+    float2 best0 = make_float2(FINF, FINF);
+    if (PRUNE == 2) {
+      // a minimum-delay s->t path is feasible, so its length UB bounds the
+      // answer: start t's best at (UB + 1, 0), which prunes keys > UB and no
+      // label that could still reach an answer (sums are integers)
+      unsigned long long pk = c.lbd[qs];
+      if (hi_of(pk) <= c.q_bound[q]) best0 = make_float2(lo_of(pk) + 1.0f, 0.f);
+    }
+    c.q_best[q] = st ? make_float2(0.f, 0.f) : best0;
     c.q_best_id[q] = st ? q : -1;
     c.q_fmin[q] = __float_as_uint(key);
+    // far-tier buckets: from the source key (the smallest any label can have)
+    // up to the upper bound when there is one
+    float reach = fmaxf(FAR_MULT * c.q_delta[q], c.far_floor);
+    float bw = reach;
+    if (PRUNE == 2 && best0.x < FINF) bw = fmaxf(bw, (best0.x - key) / (float)(NBK - 2));
+    c.far_base[q] = key;
+    c.far_bw[q] = bw;
+    c.far_cnt[q] = 0;
+    // no far tier without a finite window or for an unreachable source
+    c.far_next[q] = (reach < FINF && key < FINF) ? 1 + (int)(reach / bw) : NBK;
   } else {
     // unused slot: a dead label so that the log stays dense
     c.labs[q] = make_int4(0, 0, 0, -1);
@@ -304,6 +358,10 @@ __global__ void k_init_batch(Ctx c, int nq) {
     c.q_best[q] = make_float2(FINF, FINF);
     c.q_best_id[q] = -1;
     c.q_fmin[q] = __float_as_uint(FINF);
+    c.far_base[q] = 0.f;
+    c.far_bw[q] = 1.f;
+    c.far_next[q] = NBK;
+    c.far_cnt[q] = 0;
   }
   c.q_fmin[c.B + q] = __float_as_uint(FINF);
   c.q_last[q] = -1;
@@ -329,8 +387,8 @@ __global__ void k_sssp_init(Ctx c, int nq, int which, int r) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
   long long qt = (long long)q * c.n + c.q_dst[q];
-  float* dist = which ? c.lbl : c.lbd;
-  dist[qt] = 0.f;
+  if (which) c.lbl[qt] = 0.f;
+  else c.lbd[qt] = pack2(0.f, 0.f);
   c.inq[qt] = 1;
   c.q_dmin[(size_t)(r % 3) * c.B + q] = __float_as_uint(0.f);
   int slot = atomicAdd(&c.ct->sf_n[r % 3], 1);
@@ -349,7 +407,6 @@ __global__ void __launch_bounds__(256) k_sssp(Ctx c, int r, float delta) {
   uint32_t* dmin_old = c.q_dmin + (size_t)((r + 2) % 3) * c.B;
   for (long long q = gtid; q < c.B; q += (long long)gridDim.x * blockDim.x)
     dmin_old[q] = __float_as_uint(FINF);
-  float* dist = WHICH ? c.lbl : c.lbd;
   const int lane = lane_id();
   const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
   int relaxed = 0;
@@ -357,14 +414,19 @@ __global__ void __launch_bounds__(256) k_sssp(Ctx c, int r, float delta) {
        base < n_in; base += nwaves * WAVE) {
     long long i = base + lane;
     int deg = 0, row = 0, q = 0;
-    float du = 0.f;
+    float du = 0.f, dl2 = 0.f;  // distance (delay for WHICH 0, len for 1); len tie key
     bool carry = false;
     long long qv = 0;
     if (i < n_in) {
       qv = c.sfr[cur][i];
       q = (int)(qv / c.n);
       int u = (int)(qv - (long long)q * c.n);
-      du = load_agent(&dist[qv]);
+      if (WHICH) {
+        du = load_agent(&c.lbl[qv]);
+      } else {
+        unsigned long long pk = __hip_atomic_load(&c.lbd[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        du = hi_of(pk);
+      }
       if (du > __uint_as_float(dmin_cur[q]) + delta) {
         carry = true;
       } else {
@@ -372,7 +434,13 @@ __global__ void __launch_bounds__(256) k_sssp(Ctx c, int r, float delta) {
         // decrease by another lane sees inq == 0 and re-queues the vertex
         c.inq[qv] = 0;
         __threadfence();
-        du = load_agent(&dist[qv]);
+        if (WHICH) {
+          du = load_agent(&c.lbl[qv]);
+        } else {
+          unsigned long long pk = __hip_atomic_load(&c.lbd[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          du = hi_of(pk);
+          dl2 = lo_of(pk);
+        }
         row = c.rrow_ptr[u];
         deg = c.rrow_ptr[u + 1] - row;
       }
@@ -388,6 +456,7 @@ __global__ void __launch_bounds__(256) k_sssp(Ctx c, int r, float delta) {
     }
     wave_flat(deg, row, [&](int owner, int pos, bool act) {
       float odu = __shfl(du, owner, WAVE);
+      float odl = __shfl(dl2, owner, WAVE);
       int oq = __shfl(q, owner, WAVE);
       bool push = false;
       long long wv = 0;
@@ -395,17 +464,26 @@ __global__ void __launch_bounds__(256) k_sssp(Ctx c, int r, float delta) {
       if (act) {
         relaxed++;
         int4 e = c.redges[pos];
-        float wt = WHICH ? __int_as_float(e.y) : __int_as_float(e.z);
-        nd = odu + wt;
         float bnd = c.q_bound[oq];
         wv = (long long)oq * c.n + e.x;
-        bool ok = WHICH ? (c.lbd[wv] <= bnd) : (nd <= bnd);
-        if (ok && nd < dist[wv]) {
-          unsigned old = atomicMin((unsigned*)&dist[wv], __float_as_uint(nd));
-          if (__float_as_uint(nd) < old) {
-            if (__float_as_uint(nd) < load_agent_u32(&dmin_nxt[oq])) atomicMin(&dmin_nxt[oq], __float_as_uint(nd));
-            push = (atomicExch(&c.inq[wv], 1u) == 0u);
+        bool lowered = false;
+        if (WHICH) {
+          nd = odu + __int_as_float(e.y);
+          if (lbd_of(c, wv) <= bnd && nd < c.lbl[wv]) {
+            unsigned old = atomicMin((unsigned*)&c.lbl[wv], __float_as_uint(nd));
+            lowered = __float_as_uint(nd) < old;
           }
+        } else {
+          nd = odu + __int_as_float(e.z);
+          const unsigned long long pk = pack2(nd, odl + __int_as_float(e.y));
+          if (nd <= bnd && pk < c.lbd[wv]) {
+            unsigned long long old = atomicMin(&c.lbd[wv], pk);
+            lowered = pk < old;
+          }
+        }
+        if (lowered) {
+          if (__float_as_uint(nd) < load_agent_u32(&dmin_nxt[oq])) atomicMin(&dmin_nxt[oq], __float_as_uint(nd));
+          push = (atomicExch(&c.inq[wv], 1u) == 0u);
         }
       }
       int slot = wave_alloc32(&c.ct->sf_n[(r + 1) % 3], push);
@@ -474,7 +552,7 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
           rd = lab.delay;
           if (PRUNE != 0 && lab.vid == c.q_dst[q]) live = false;  // answers are not extended
           if (live && PRUNE == 2) {
-            rd = lab.delay + c.lbd[(long long)q * c.n + lab.vid];
+            rd = lab.delay + lbd_of(c, (long long)q * c.n + lab.vid);
             if (rd > c.q_bound[q]) live = false;
           }
         }
@@ -502,15 +580,66 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
         }
       }
     }
-    // carried labels keep their place in the next frontier
-    int cslot = wave_alloc32(&ct->carry_n[nxt], carry);
-    if (carry) {
+    // carried labels: near tier if within FAR_MULT windows of the front,
+    // else the query's far tier (not scanned again until pulled)
+    bool far = false;
+    int fb = 0;
+    if (carry && dop <= 0) {
+      fb = far_bucket(c, q, key);
+      far = fb >= c.far_next[q];
+    }
+    const bool near = carry && !far;
+    int cslot = wave_alloc32(&ct->carry_n[nxt], near);
+    const int4 crec4 = make_int4(id, q, __float_as_int(key), __float_as_int(rd));
+    if (near) {
       if (cslot < c.carry_cap) {
-        c.carry[nxt][cslot] = make_int4(id, q, __float_as_int(key), __float_as_int(rd));
+        c.carry[nxt][cslot] = crec4;
         if (__float_as_uint(key) < load_agent_u32(&fmin_nxt[q])) atomicMin(&fmin_nxt[q], __float_as_uint(key));
       } else {
         atomicOr(&ct->overflow, OVF_CARRY);
       }
+    }
+    if (far) {
+      atomicAdd(&c.far_cnt[q], 1);
+      // lock-free append to the query's far chunk list (as the candidate lists)
+      int* headp = &c.far_head[(long long)q * NBK + fb];
+      int h = __hip_atomic_load(headp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      bool done = false;
+      if (h >= 0) {
+        int slot = atomicAdd(&c.fhd[h].x, 1);
+        if (slot < FCH) {
+          c.frec[(long long)h * FCH + slot] = crec4;
+          done = true;
+        }
+      }
+      if (!done) {
+        // chunks come from a bump allocator reset with each batch
+        const int nc = atomicAdd(&ct->far_chunks, 1);
+        if (nc >= c.far_cap) {
+          atomicOr(&ct->overflow, OVF_FAR);
+        } else {
+          c.frec[(long long)nc * FCH] = crec4;
+          int expected = h;
+          for (;;) {
+            c.fhd[nc] = make_int2(1, expected);
+            __threadfence();
+            int prev = atomicCAS(headp, expected, nc);
+            if (prev == expected) break;
+            expected = prev;
+          }
+        }
+      }
+    }
+    {
+      int nf = far ? 1 : 0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) nf += __shfl_xor(nf, o, WAVE);
+      if (lane == 0 && nf) {
+        atomicAdd((unsigned long long*)&ct->far_live, (unsigned long long)nf);
+        atomicAdd(&ct->far_appended, (unsigned long long)nf);
+      }
+      wave_add(&ct->near_carried, near ? 1 : 0);
+      wave_add(&ct->scanned, i < fn ? 1 : 0);
     }
     float ld = lab.d, ldl = lab.delay;
     wave_flat(deg, row, [&](int owner, int pos, bool act) {
@@ -533,7 +662,7 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
           if (lex_ge(son.d, son.delay, b.x, b.y)) ok = false;
         }
         if (ok && PRUNE == 2) {
-          float rd = son.delay + c.lbd[qv];
+          float rd = son.delay + lbd_of(c, qv);
           if (rd > C.delay_L) {
             ok = false;
           } else {
@@ -817,6 +946,116 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
   if (lane == 0 && steps_max > 0) atomicMax(&ct->merge_chain_max, steps_max);
   wave_add(&ct->merge_scanned, scanned);
   wave_add(&ct->inserted, n_ins);
+}
+
+// ---------------------------------------------------------------------------
+// Pull: between merge of round r and expand of round r+1.  For each query the
+// buckets whose key range comes within reach of the front (front + reach) are
+// moved whole into the near tier; if the near tier is empty, the next
+// non-empty bucket is.  A far entry is therefore moved at most once, and
+// entries whose keys the upper bound prunes are never read again.  Dead and
+// pruned entries are dropped on the way, chunks go to the free stack.  One
+// wave per query.
+template <int PRUNE>
+__global__ void __launch_bounds__(256) k_pull(Ctx c) {
+  Counters* ct = c.ct;
+  const int rn = ct->round_e;  // the round that follows
+  const int nc = rn & 1;
+  uint32_t* fmin = c.q_fmin + (size_t)nc * c.B;
+  const int lane = lane_id();
+  const long long q = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  if (q >= c.B) return;  // whole waves exit together
+  const int nb = c.far_next[q];
+  if (nb >= NBK) return;
+  const float front = __uint_as_float(load_agent_u32(&fmin[q]));
+  const float reach = fmaxf(FAR_MULT * c.q_delta[q], c.far_floor);
+  const int* heads = c.far_head + q * NBK;
+  if (c.far_cnt[q] <= 0) {
+    // nothing parked: only keep the near/far boundary up with the front
+    if (lane == 0 && front < FINF) c.far_next[q] = max(nb, far_bucket(c, (int)q, front + reach) + 1);
+    return;
+  }
+  const float2 best = c.q_best[q];
+  if (PRUNE != 0 && c.far_base[q] + (float)nb * c.far_bw[q] > best.x) {
+    // every parked key exceeds t's best answer: the whole far tier is
+    // pruned without reading it (its chunks are reclaimed at the next batch)
+    if (lane == 0) {
+      const int left = c.far_cnt[q];
+      c.far_cnt[q] = 0;
+      c.far_next[q] = NBK;
+      atomicAdd((unsigned long long*)&ct->far_live, (unsigned long long)(-(long long)left));
+    }
+    return;
+  }
+  int target;
+  if (front < FINF) {
+    target = far_bucket(c, (int)q, front + reach);
+  } else {
+    // near tier empty: the first non-empty bucket, if any
+    target = -1;
+    for (int b0 = nb; b0 < NBK && target < 0; b0 += WAVE) {
+      int b = b0 + lane;
+      unsigned long long m = __ballot(b < NBK && heads[b] >= 0);
+      if (m) target = b0 + __ffsll((long long)m) - 1;
+    }
+    if (target < 0) return;
+  }
+  if (target < nb) return;
+  long long moved = 0, dropped = 0;
+  for (int b0 = nb; b0 <= target; b0 += WAVE) {
+   // find the non-empty buckets of this stretch together, then drain them
+   const int bl = b0 + lane;
+   unsigned long long ne = __ballot(bl <= target && heads[bl] >= 0);
+   while (ne) {
+    const int b = b0 + __ffsll((long long)ne) - 1;
+    ne &= ne - 1;
+    int h = c.far_head[q * NBK + b];
+    int guard = 0;
+    int2 hd = (h >= 0) ? c.fhd[h] : make_int2(0, -1);
+    while (h >= 0 && guard++ < c.far_cap) {
+      // the next header is requested before this chunk is processed
+      const int2 hd_next = (hd.y >= 0) ? c.fhd[hd.y] : make_int2(0, -1);
+      const int cnt = min(hd.x, FCH);
+      const int4 rec = (lane < cnt) ? c.frec[(long long)h * FCH + lane] : make_int4(-1, 0, 0, 0);
+      bool live = lane < cnt && !c.dead[rec.x];
+      const float key = __int_as_float(rec.z), rd = __int_as_float(rec.w);
+      if (live && PRUNE != 0 && lex_ge(key, rd, best.x, best.y)) live = false;
+      dropped += (lane < cnt && !live) ? 1 : 0;
+      int slot = wave_alloc32(&ct->carry_n[nc], live);
+      if (live) {
+        moved++;
+        if (slot < c.carry_cap) {
+          c.carry[nc][slot] = rec;
+          if (__float_as_uint(key) < load_agent_u32(&fmin[q])) atomicMin(&fmin[q], __float_as_uint(key));
+        } else {
+          atomicOr(&ct->overflow, OVF_CARRY);
+        }
+      }
+      h = hd.y;
+      hd = hd_next;
+    }
+    if (lane == 0) c.far_head[q * NBK + b] = -1;
+   }
+  }
+  {
+    long long mq = moved + dropped;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mq += __shfl_xor(mq, o, WAVE);
+    if (lane == 0) {
+      c.far_next[q] = target + 1;
+      c.far_cnt[q] -= (int)mq;
+      c.q_last[q] = rn;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    moved += __shfl_xor(moved, o, WAVE);
+    dropped += __shfl_xor(dropped, o, WAVE);
+  }
+  if (lane == 0 && (moved + dropped)) {
+    atomicAdd((unsigned long long*)&ct->far_live, (unsigned long long)(-(moved + dropped)));
+    atomicAdd(&ct->far_pulled, (unsigned long long)moved);
+  }
 }
 
 // ---------------------------------------------------------------------------

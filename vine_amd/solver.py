@@ -72,7 +72,12 @@ class Stats(ctypes.Structure):
         ("touched", ctypes.c_int64),
         ("merge_scanned", ctypes.c_int64),
         ("merge_chain_max", ctypes.c_int64),
-        ("reserved", ctypes.c_int64 * 3),
+        ("far_pulled", ctypes.c_int64),
+        ("ms_pull", ctypes.c_double),
+        ("near_carried", ctypes.c_int64),
+        ("far_appended", ctypes.c_int64),
+        ("scanned", ctypes.c_int64),
+        ("reserved", ctypes.c_int64 * 1),
     ]
 
     def as_dict(self):
