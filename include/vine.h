@@ -147,10 +147,12 @@ vine_status vine_solve_device(vine_solver* s, int32_t nq, const int32_t* d_src, 
  * solve), measured with the device wall clock. */
 vine_status vine_solver_latency(const vine_solver* s, int32_t nq, double* out_us);
 
-/* Per-round trace of the last solve (record_trace = 1): for each round r,
- * rows[r*8 .. r*8+8) = {query slot, round, frontier size, expanded, min key,
- * max key, delta used, candidates}.  Returns the number of rows written. */
-int64_t vine_solver_trace(const vine_solver* s, double* rows, int64_t max_rows);
+/* Per-(query, round) scheduler features of the last solve (record_trace = k
+ * records every k-th round of every live query, up to 8M rows): row i is
+ * rows[i*10 .. i*10+10) = {query, round, x0..x7} with the MLP inputs x0..x7
+ * (see vine_amd/csrc/solver_kernels.hpp, k_policy).  With rows = NULL returns
+ * the row count; else copies up to max_rows and returns the number copied. */
+int64_t vine_solver_trace(const vine_solver* s, float* rows, int64_t max_rows);
 
 /* MLP Delta policy weights (8 -> 32 -> 32 -> 2, fp32, row-major W[out][in]):
  * w1[32*8], b1[32], w2[32*32], b2[32], w3[2*32], b3[2], plus input mean/std
@@ -161,6 +163,30 @@ vine_status vine_solver_set_mlp(vine_solver* s, const float* params, int64_t npa
  * batch of the last solve (VINE_PRUNE_NONE gives the reference's full sets).
  * Writes up to cap (d, delay) pairs, returns the set size or a negative status. */
 int64_t vine_solver_pareto(const vine_solver* s, int32_t q, int32_t v, float* out_pairs, int64_t cap);
+
+/* Bucket-width MLP (8 -> 32 -> 32 -> 2, fp32, ReLU), trained and evaluated on
+ * the GPU.  No counterpart in the reference: it replaces the AR(3) model of
+ * include/AR.cpp as the window scheduler.  Parameter layout: W1[32*8] b1[32]
+ * W2[32*32] b2[32] W3[2*32] b3[2] xmu[8] xsd[8] (row-major W[out][in]); the
+ * last 16 entries normalise the inputs and are not trained.  Loss per sample:
+ * 0.5 * sum_k mask_k (y_k - target_k)^2.  Arrays are host buffers: x[n*8],
+ * target[n*2], mask[n*2], y[n*2], grad[1410]. */
+typedef struct vine_mlp vine_mlp;
+int32_t vine_mlp_param_count(void);
+const char* vine_mlp_last_error(void);
+vine_status vine_mlp_create(vine_mlp** out);
+vine_status vine_mlp_destroy(vine_mlp* m);
+vine_status vine_mlp_set_params(vine_mlp* m, const float* params);
+vine_status vine_mlp_get_params(const vine_mlp* m, float* params);
+vine_status vine_mlp_forward(const vine_mlp* m, const float* x, int64_t n, float* y);
+vine_status vine_mlp_gradient(const vine_mlp* m, const float* x, const float* target, const float* mask,
+                              int64_t n, float* grad, double* loss);
+vine_status vine_mlp_train(vine_mlp* m, const float* x, const float* target, const float* mask, int64_t n,
+                           int32_t epochs, int32_t batch, float lr, uint64_t seed, double* loss_curve);
+
+/* Per-query work of the last solve: labels expanded and rounds in which the
+ * query had live frontier labels. */
+vine_status vine_solver_query_stats(const vine_solver* s, int32_t nq, int64_t* expanded, int32_t* rounds);
 
 /* Host scheduling helpers (also used by the solver's VINE_DELTA_AR policy).
  * vine_ar_burg: Burg AR fit, the reference's AR::AutoRegression with the

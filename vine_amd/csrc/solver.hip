@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "mlp.hpp"
 #include "sched.hpp"
 #include "solver_kernels.hpp"
 #include "vine.h"
@@ -88,6 +89,11 @@ struct vine_solver {
   int32_t* d_qlast = nullptr;
   float* d_qfmax = nullptr;
   int32_t* d_qnfront = nullptr;
+  int32_t *d_qexp = nullptr, *d_qins = nullptr, *d_qnear = nullptr;
+  long long* d_qexptot = nullptr;
+  float* d_qfcast = nullptr;
+  float* d_trace = nullptr;
+  long long trace_cap = 0;
   // per (query, vertex)
   int2* d_pset = nullptr;
   int32_t* d_cand_head = nullptr;
@@ -140,8 +146,11 @@ struct vine_solver {
 
   // last-solve records
   std::vector<double> latency_us;
-  std::vector<double> trace;
+  std::vector<float> trace;  // rows of 10 floats
+  std::vector<int64_t> q_expanded;
+  std::vector<int32_t> q_rounds;
   int last_batch_q = 0;
+  int nb_active = 0;  // queries in the current batch
 };
 
 static vine_status dev_alloc(vine_solver* s, void** p, size_t bytes) {
@@ -379,6 +388,15 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   ALLOC(s->d_qlast, sizeof(int32_t) * B);
   ALLOC(s->d_qfmax, sizeof(float) * B);
   ALLOC(s->d_qnfront, sizeof(int32_t) * B);
+  ALLOC(s->d_qexp, sizeof(int32_t) * B);
+  ALLOC(s->d_qins, sizeof(int32_t) * B);
+  ALLOC(s->d_qnear, sizeof(int32_t) * B);
+  ALLOC(s->d_qexptot, sizeof(long long) * B);
+  ALLOC(s->d_qfcast, sizeof(float) * B);
+  if (o.record_trace > 0) {
+    s->trace_cap = 8LL << 20;
+    ALLOC(s->d_trace, sizeof(float) * 10 * s->trace_cap);
+  }
   ALLOC(s->d_pset, sizeof(int2) * BN);
   ALLOC(s->d_cand_head, sizeof(int32_t) * BN);
   if (o.prune == 2) {
@@ -438,6 +456,17 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   c.q_last = s->d_qlast;
   c.q_fmax = s->d_qfmax;
   c.q_nfront = s->d_qnfront;
+  c.q_exp = s->d_qexp;
+  c.q_ins = s->d_qins;
+  c.q_near = s->d_qnear;
+  c.q_exptot = s->d_qexptot;
+  c.q_fcast = s->d_qfcast;
+  c.policy = o.delta_policy;
+  c.stats_on = (o.delta_policy == VINE_DELTA_MLP || o.record_trace > 0) ? 1 : 0;
+  c.mlp = nullptr;
+  c.trace = s->d_trace;
+  c.trace_cap = s->trace_cap;
+  c.trace_every = o.record_trace;
   c.pset = s->d_pset;
   c.cand_head = s->d_cand_head;
   c.lbl = s->d_lbl;
@@ -565,6 +594,12 @@ vine_status launch_round(vine_solver* s, long long dop, int gsz, vine_stats* st)
     HIP_TRY(hipGetLastError());
     HIP_TRY(t.stop());
   }
+  if (c.stats_on) {
+    Timer t(s, 3);
+    hipLaunchKernelGGL(dev::k_policy, dim3((s->B + 255) / 256), dim3(256), 0, s->stream, c, s->nb_active);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(t.stop());
+  }
   st->expand_launches++;
   st->merge_launches++;
   return VINE_OK;
@@ -649,11 +684,21 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
   s->h_ct->pool_ctr = (unsigned long long)s->B * dev::INIT_CAP;
   HIP_TRY(hipMemcpyAsync(s->d_ct, s->h_ct, sizeof(Counters), hipMemcpyHostToDevice, s->stream));
   // window per query
+  // initial window: fixed policy uses delta_fixed, the learned one starts
+  // from delta_fixed (or a quarter arc) and the AR rule from one arc length
+  float delta_ar = s->g->mean_len;
   {
     float dv = dev::FINF;
     if (s->opt.delta_policy == VINE_DELTA_FIXED) dv = s->opt.delta_fixed;
+    if (s->opt.delta_policy == VINE_DELTA_MLP) dv = s->opt.delta_fixed > 0 ? s->opt.delta_fixed : 0.25f * s->g->mean_len;
+    if (s->opt.delta_policy == VINE_DELTA_AR) dv = delta_ar;
     vine_status fs = fill_f32(s, s->d_qdelta, dv, s->B);
     if (fs != VINE_OK) return fs;
+  }
+  s->nb_active = nb;
+  if (s->opt.delta_policy == VINE_DELTA_MLP && !c.mlp) {
+    set_err("VINE_DELTA_MLP needs parameters: call vine_solver_set_mlp first");
+    return VINE_E_ARG;
   }
   if (s->opt.prune == 2) {
     vine_status fs = fill_f32(s, (float*)s->d_lbd, dev::FINF, 2 * BN);
@@ -694,12 +739,12 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
   const int K = ar ? 1 : std::max(1, s->opt.rounds_per_sync);
   int64_t rounds = 0;
   long long fn_last = nb;
+  unsigned long long exp_prev = 0;
   for (;;) {
     for (int k = 0; k < K; k++) {
-      long long dop = ar ? (long long)rule.dop() : 0;
       // the frontier may grow between polls: size the grid for 4x growth
       // (grid-stride loops keep any size correct)
-      vine_status ls = launch_round_any(s, dop, grid_for(s, fn_last * 4), st);
+      vine_status ls = launch_round_any(s, 0, grid_for(s, fn_last * 4), st);
       if (ls != VINE_OK) return ls;
       rounds++;
     }
@@ -710,11 +755,6 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
     long long fn = (long long)h.carry_n[par] + ((long long)h.label_ctr - h.lstart[par]) + h.far_live;
     st->max_frontier = std::max<int64_t>(st->max_frontier, fn);
     fn_last = std::max<long long>(fn, 1);
-    if (s->opt.record_trace) {
-      double row[8] = {-1, (double)rounds, (double)fn, (double)h.expanded, 0, 0,
-                       ar ? (double)rule.dop() : (double)s->opt.delta_fixed, (double)h.cands};
-      s->trace.insert(s->trace.end(), row, row + 8);
-    }
     if (fn == 0) break;
     // every round expands at least the minimum-key label of each live query
     // (or re-keys carried labels once), so rounds are bounded by twice the
@@ -723,7 +763,24 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
       set_err("solver made no progress after %lld rounds", (long long)rounds);
       return VINE_E_OVERFLOW;
     }
-    if (ar) rule.update((int64_t)fn);
+    if (ar) {
+      // the reference's DOP rule picks how many labels the next round should
+      // expand; a multiplicative controller turns that count into the key
+      // window (DOP >= frontier means the whole frontier, as in the reference)
+      rule.update((int64_t)fn);
+      const double dop = (double)rule.dop();
+      const unsigned long long x = h.expanded - exp_prev;
+      exp_prev = h.expanded;
+      if (dop >= (double)fn) {
+        delta_ar = dev::FINF;
+      } else {
+        if (!(delta_ar < dev::FINF)) delta_ar = s->g->mean_len;
+        double ratio = x > 0 ? dop / (double)x : 4.0;
+        delta_ar = (float)std::min(1e7, std::max(0.25, delta_ar * std::min(4.0, std::max(0.25, std::sqrt(ratio)))));
+      }
+      vine_status fs = fill_f32(s, s->d_qdelta, delta_ar, s->B);
+      if (fs != VINE_OK) return fs;
+    }
   }
   hipLaunchKernelGGL(dev::k_stamp_round, dim3(1), dim3(1), 0, s->stream, c, (int)rounds);
   HIP_TRY(hipGetLastError());
@@ -733,6 +790,13 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
   }
   vine_status es = resolve_events(s, st);
   if (es != VINE_OK) return es;
+  if (s->d_trace) {
+    long long nrow = std::min<long long>(s->h_ct->trace_n, s->trace_cap);
+    size_t at = s->trace.size();
+    s->trace.resize(at + 10 * (size_t)nrow);
+    if (nrow > 0)
+      HIP_TRY(hipMemcpy(s->trace.data() + at, s->d_trace, sizeof(float) * 10 * nrow, hipMemcpyDeviceToHost));
+  }
   st->rounds += rounds;
   st->expanded += (int64_t)s->h_ct->expanded;
   st->relaxed += (int64_t)s->h_ct->relaxed;
@@ -780,6 +844,8 @@ vine_status solve_impl(vine_solver* s, int32_t nq, const int32_t* d_src, const i
                        int32_t* d_plen, vine_stats* st) {
   Ctx& c = s->ctx;
   s->latency_us.assign(nq, 0.0);
+  s->q_expanded.assign(nq, 0);
+  s->q_rounds.assign(nq, 0);
   s->trace.clear();
   for (int32_t q0 = 0; q0 < nq; q0 += s->B) {
     int nb = std::min<int32_t>(s->B, nq - q0);
@@ -807,6 +873,16 @@ vine_status solve_impl(vine_solver* s, int32_t nq, const int32_t* d_src, const i
     if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, s->g->device) != hipSuccess ||
         rate_khz <= 0)
       rate_khz = 100000;
+    {
+      std::vector<long long> ex(nb);
+      HIP_TRY(hipMemcpy(ex.data(), s->d_qexptot, sizeof(long long) * nb, hipMemcpyDeviceToHost));
+      for (int i = 0; i < nb; i++) {
+        if (q0 + i < (int32_t)s->q_expanded.size()) {
+          s->q_expanded[q0 + i] = ex[i];
+          s->q_rounds[q0 + i] = last[i] + 1;
+        }
+      }
+    }
     for (int i = 0; i < nb; i++) {
       int r = last[i] + 1;  // the query's last active round ends when the next one starts
       double us = 0.0;
@@ -972,17 +1048,27 @@ vine_status vine_solver_latency(const vine_solver* s, int32_t nq, double* out_us
   return VINE_OK;
 }
 
-int64_t vine_solver_trace(const vine_solver* s, double* rows, int64_t max_rows) {
+int64_t vine_solver_trace(const vine_solver* s, float* rows, int64_t max_rows) {
   if (!s) return VINE_E_ARG;
-  int64_t n = (int64_t)s->trace.size() / 8;
+  int64_t n = (int64_t)s->trace.size() / 10;
   if (!rows) return n;
   int64_t k = std::min(n, max_rows);
-  std::memcpy(rows, s->trace.data(), sizeof(double) * 8 * k);
+  std::memcpy(rows, s->trace.data(), sizeof(float) * 10 * k);
   return k;
 }
 
+vine_status vine_solver_query_stats(const vine_solver* s, int32_t nq, int64_t* expanded, int32_t* rounds) {
+  if (!s || nq < 0 || (nq > 0 && (!expanded || !rounds))) return VINE_E_ARG;
+  for (int32_t i = 0; i < nq; i++) {
+    bool have = i < (int32_t)s->q_expanded.size();
+    expanded[i] = have ? s->q_expanded[i] : 0;
+    rounds[i] = have ? s->q_rounds[i] : 0;
+  }
+  return VINE_OK;
+}
+
 vine_status vine_solver_set_mlp(vine_solver* s, const float* params, int64_t nparams) {
-  if (!s || !params || nparams <= 0) {
+  if (!s || !params || nparams != vine::mlp::NPARAM) {
     set_err("vine_solver_set_mlp: invalid arguments");
     return VINE_E_ARG;
   }
@@ -998,6 +1084,7 @@ vine_status vine_solver_set_mlp(vine_solver* s, const float* params, int64_t npa
     s->mlp_nparams = nparams;
   }
   HIP_TRY(hipMemcpy(s->d_mlp, params, sizeof(float) * nparams, hipMemcpyHostToDevice));
+  s->ctx.mlp = s->d_mlp;
   return VINE_OK;
 }
 

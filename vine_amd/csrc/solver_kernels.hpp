@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mlp.hpp"
 #include "vine/label.hpp"
 
 namespace vine {
@@ -68,6 +69,7 @@ struct Counters {
   long long far_live;                // records currently in far tiers
   unsigned long long far_pulled;     // records moved far -> near
   unsigned long long near_carried, far_appended, scanned;
+  long long trace_n;  // scheduler feature rows recorded
 };
 
 struct Ctx {
@@ -82,7 +84,7 @@ struct Ctx {
   const int32_t* q_src;
   const int32_t* q_dst;
   const float* q_bound;
-  const float* q_delta;
+  float* q_delta;
   float2* q_best;
   int32_t* q_best_id;
   uint32_t* q_fmin;  // [2][B] float bits
@@ -133,6 +135,18 @@ struct Ctx {
   unsigned long long* ts;  // [TS_CAP] per-round merge-end wall clock
   Counters* ct;
   int32_t hash_slots;
+  // scheduler: per-query round statistics, MLP, feature trace
+  int32_t stats_on;
+  int32_t policy;     // VINE_DELTA_*
+  int32_t* q_exp;     // [B] labels expanded this round
+  int32_t* q_ins;     // [B] labels inserted this round
+  int32_t* q_near;    // [B] near-tier entries queued this round
+  long long* q_exptot;  // [B] labels expanded in the batch
+  float* q_fcast;     // [B] MLP forecast of log2(1 + next round's expansions)
+  const float* mlp;   // MLP parameters (VINE_DELTA_MLP)
+  float* trace;       // [trace_cap][10] rows {q, round, x0..x7}
+  long long trace_cap;
+  int32_t trace_every;
 };
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
@@ -367,6 +381,10 @@ __global__ void k_init_batch(Ctx c, int nq) {
   c.q_last[q] = -1;
   c.q_fmax[q] = 0.f;
   c.q_nfront[q] = 0;
+  c.q_exp[q] = 0;
+  c.q_ins[q] = 0;
+  c.q_near[q] = 0;
+  c.q_exptot[q] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -567,6 +585,7 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
           if (dop > 0) in_window = (i >= fn - dop);
           else in_window = !(key > __uint_as_float(fmin_cur[q]) + c.q_delta[q]);
           if (in_window) {
+            if (c.stats_on) atomicAdd(&c.q_exp[q], 1);
             if (!fresh) {
               int4 L = c.labs[id];
               lab = LT::unpack(L.x, make_float2(__int_as_float(L.y), __int_as_float(L.z)));
@@ -592,6 +611,7 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
     int cslot = wave_alloc32(&ct->carry_n[nxt], near);
     const int4 crec4 = make_int4(id, q, __float_as_int(key), __float_as_int(rd));
     if (near) {
+      if (c.stats_on) atomicAdd(&c.q_near[q], 1);
       if (cslot < c.carry_cap) {
         c.carry[nxt][cslot] = crec4;
         if (__float_as_uint(key) < load_agent_u32(&fmin_nxt[q])) atomicMin(&fmin_nxt[q], __float_as_uint(key));
@@ -915,6 +935,10 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
         if (fits) {
           if (gl == 0) {
             n_ins++;
+            if (c.stats_on) {
+              atomicAdd(&c.q_ins[q], 1);
+              atomicAdd(&c.q_near[q], 1);
+            }
             c.pool_cd[off + slot] = make_float2(cd, cdl);
             c.pool_id[off + slot] = (int)lid;
             float key = LT::key(cl) + lb;
@@ -1041,10 +1065,14 @@ __global__ void __launch_bounds__(256) k_pull(Ctx c) {
     long long mq = moved + dropped;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mq += __shfl_xor(mq, o, WAVE);
+    long long mv = moved;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mv += __shfl_xor(mv, o, WAVE);
     if (lane == 0) {
       c.far_next[q] = target + 1;
       c.far_cnt[q] -= (int)mq;
       c.q_last[q] = rn;
+      if (c.stats_on) c.q_near[q] += (int)mv;
     }
   }
 #pragma unroll
@@ -1055,6 +1083,70 @@ __global__ void __launch_bounds__(256) k_pull(Ctx c) {
   if (lane == 0 && (moved + dropped)) {
     atomicAdd((unsigned long long*)&ct->far_live, (unsigned long long)(-(moved + dropped)));
     atomicAdd(&ct->far_pulled, (unsigned long long)moved);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scheduler step, after pull: per query, the 8 features of the state the next
+// round starts from, optionally recorded as a trace row, and (VINE_DELTA_MLP)
+// the next window width from the MLP:
+//   x0 log2(1 + near-tier entries queued this round)
+//   x1 log2(1 + far-tier entries)
+//   x2 log2(1 + labels expanded this round)
+//   x3 log2(1 + labels inserted this round)
+//   x4 progress of the front from the smallest key to t's best bound, in [0, 1.5]
+//   x5 log2(1 + round)
+//   x6 log2 of the window used this round (20 for an unbounded window)
+//   x7 1 if an answer has been found
+// Output 0 gives the next window as 2^y0 (clamped to [2^-2, 2^16]); output 1
+// is a forecast of log2(1 + next round's expansions).
+__global__ void k_policy(Ctx c, int nq) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= c.B) return;
+  Counters* ct = c.ct;
+  const int rn = ct->round_e;
+  const int nc = rn & 1;
+  const int exp_ = c.q_exp[q], ins = c.q_ins[q], near = c.q_near[q];
+  c.q_exp[q] = 0;
+  c.q_ins[q] = 0;
+  c.q_near[q] = 0;
+  c.q_exptot[q] += exp_;
+  const int far = c.far_cnt[q];
+  const bool active = q < nq && (near + far + exp_) > 0;
+  if (!active) return;
+  const float fmin = __uint_as_float(c.q_fmin[(size_t)nc * c.B + q]);
+  const float base = c.far_base[q];
+  const float2 best = c.q_best[q];
+  float prog = 1.5f;
+  if (fmin < FINF && best.x < FINF && best.x > base) prog = fminf(fmaxf((fmin - base) / (best.x - base), 0.f), 1.5f);
+  const float d = c.q_delta[q];
+  float x[mlp::NIN];
+  x[0] = log2f(1.f + (float)near);
+  x[1] = log2f(1.f + (float)far);
+  x[2] = log2f(1.f + (float)exp_);
+  x[3] = log2f(1.f + (float)ins);
+  x[4] = prog;
+  x[5] = log2f(1.f + (float)rn);
+  x[6] = (d < FINF) ? log2f(fmaxf(d, 0.25f)) : 20.f;
+  x[7] = c.q_best_id[q] >= 0 ? 1.f : 0.f;
+  if (c.trace && c.trace_every > 0 && rn % c.trace_every == 0) {
+    long long row = atomicAdd((unsigned long long*)&ct->trace_n, 1ull);
+    if (row < c.trace_cap) {
+      float* out = c.trace + row * 10;
+      out[0] = (float)q;
+      out[1] = (float)rn;
+#pragma unroll
+      for (int j = 0; j < mlp::NIN; j++) out[2 + j] = x[j];
+    }
+  }
+  if (c.policy == 3 && c.mlp) {
+    float y[mlp::NOUT];
+    mlp::apply(c.mlp, x, y);
+    float l = y[0];
+    if (!(l == l)) l = log2f(fmaxf(c.far_floor, 1.f));  // NaN output: fall back to a small window
+    l = fminf(fmaxf(l, -2.f), 16.f);
+    c.q_delta[q] = exp2f(l);
+    c.q_fcast[q] = y[1];
   }
 }
 

@@ -88,7 +88,7 @@ EXPORTS = (
     "vine_options_default", "vine_last_error", "vine_device_count", "vine_graph_create",
     "vine_graph_destroy", "vine_solver_create", "vine_solver_destroy", "vine_solve",
     "vine_solve_device", "vine_solver_latency", "vine_solver_trace", "vine_solver_set_mlp",
-    "vine_solver_pareto",
+    "vine_solver_pareto", "vine_solver_query_stats",
 )
 
 _declared = False
@@ -112,6 +112,7 @@ def _L():
             "vine_solve_device": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, P(Stats)]),
             "vine_solver_latency": (i32, [vp, i32, vp]),
             "vine_solver_trace": (i64, [vp, vp, i64]),
+            "vine_solver_query_stats": (i32, [vp, i32, vp, vp]),
             "vine_solver_set_mlp": (i32, [vp, vp, i64]),
             "vine_solver_pareto": (i64, [vp, i32, i32, vp, i64]),
         }
@@ -245,10 +246,18 @@ class Solver:
         return out
 
     def trace(self):
+        """Scheduler feature rows of the last solve: columns (query, round, x0..x7)."""
         n = _L().vine_solver_trace(self.h, None, 0)
-        rows = np.zeros((max(n, 1), 8), np.float64)
+        rows = np.zeros((max(n, 1), 10), np.float32)
         k = _L().vine_solver_trace(self.h, rows.ctypes.data, n)
         return rows[:k]
+
+    def query_stats(self, nq):
+        """(labels expanded, active rounds) per query of the last solve."""
+        ex = np.zeros(nq, np.int64)
+        rd = np.zeros(nq, np.int32)
+        _check(_L().vine_solver_query_stats(self.h, nq, ex.ctypes.data, rd.ctypes.data))
+        return ex, rd
 
     def pareto(self, q, v, cap=1 << 16):
         out = np.zeros(2 * cap, np.float32)
