@@ -1,0 +1,154 @@
+// vine/cspp.hpp -- C++ host facade over the C ABI (include/vine.h).
+//
+// Mirrors the reference's solver facade cspp<Label_t, Edge_t>
+// (algorithm/cspp.hpp:11-69) and its graph builder Graph<Edge_t>
+// (include/graph.hpp:90-195): build a graph from arcs, construct the solver
+// with the number of tasks, the source labels, the sinks and one constraint
+// per task, call solve().  Results land on the object (cost_l, delay_l,
+// feasible_l, path_l, size_of_result_l); the reference allocates path_l but
+// never fills it.  Differences: solve() returns the vine_status instead of
+// always 0, and every task uses its own constraint (the reference passes
+// C_l[0] for all tasks, algorithm/cspp.cuh:196,234).
+//
+// The C ABI carries the c_label / c_constraint / c_edge instantiation; other
+// label types need the kernel templates in vine_amd/csrc/solver_kernels.hpp
+// compiled for them.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "vine.h"
+#include "vine/label.hpp"
+
+namespace vine {
+
+// Host graph in adjacency insertion order (the CSR order of
+// Graph::adj_to_csr, include/graph.hpp:136-167).
+template <typename Edge_t>
+class Graph {
+ public:
+  explicit Graph(int n) : N(n), M(0), adj_(n) {}
+  // directed arc u -> v
+  void add_edge(int u, int v, const Edge_t& e) {
+    if (u < 0 || u >= N || v < 0 || v >= N) throw std::out_of_range("vertex id");
+    adj_[u].push_back({v, e});
+    M++;
+  }
+  // both arcs with the same weights (the reference's undirected loader,
+  // include/io.hpp:50-60)
+  void add_undirected(int u, int v, const Edge_t& e) {
+    add_edge(u, v, e);
+    add_edge(v, u, e);
+  }
+  void adj_to_csr(std::vector<int32_t>& row, std::vector<int32_t>& dst, std::vector<float>& len,
+                  std::vector<float>& delay) const {
+    row.assign(N + 1, 0);
+    dst.clear();
+    len.clear();
+    delay.clear();
+    for (int u = 0; u < N; u++) {
+      for (const auto& a : adj_[u]) {
+        dst.push_back(a.first);
+        len.push_back(a.second.len);
+        delay.push_back(a.second.delay);
+      }
+      row[u + 1] = (int32_t)dst.size();
+    }
+  }
+  int N;
+  int64_t M;
+
+ private:
+  std::vector<std::vector<std::pair<int, Edge_t>>> adj_;
+};
+
+template <typename Label_t, typename Edge_t>
+class cspp {
+  static_assert(std::is_same<Label_t, c_label>::value && std::is_same<Edge_t, c_edge>::value,
+                "the C ABI carries the c_label / c_edge instantiation");
+  using Constraint_t = c_constraint<Label_t>;
+
+ public:
+  cspp(Graph<Edge_t>* g, int ntask, std::vector<Label_t>& source_state, std::vector<int>& sink_l,
+       std::vector<Constraint_t>& C_l)
+      : g_(g), ntask(ntask), init_state(source_state), sink_l(sink_l), C_l(C_l) {
+    vine_options_default(&options);
+    if ((int)init_state.size() != ntask || (int)sink_l.size() != ntask || (int)C_l.size() != ntask)
+      throw std::invalid_argument("ntask must match the sources, sinks and constraints");
+  }
+
+  // Solve every task on the GPU.  Returns VINE_OK or the failing vine_status
+  // (message in last_error).
+  int solve() {
+    std::vector<int32_t> row, dst;
+    std::vector<float> len, delay;
+    g_->adj_to_csr(row, dst, len, delay);
+    vine_graph* dg = nullptr;
+    vine_status st = vine_graph_create(g_->N, (int64_t)dst.size(), row.data(), dst.data(), len.data(),
+                                       delay.data(), &dg);
+    if (st != VINE_OK) return fail(st);
+    vine_solver* sv = nullptr;
+    st = vine_solver_create(dg, &options, &sv);
+    if (st != VINE_OK) {
+      vine_graph_destroy(dg);
+      return fail(st);
+    }
+    std::vector<int32_t> src(ntask), snk(sink_l.begin(), sink_l.end());
+    std::vector<float> bound(ntask);
+    for (int i = 0; i < ntask; i++) {
+      src[i] = init_state[i].vid;
+      bound[i] = C_l[i].delay_L;
+    }
+    cost_l.assign(ntask, 0.f);
+    delay_l.assign(ntask, 0.f);
+    std::vector<int32_t> feas(ntask, 0);
+    std::vector<int64_t> off(ntask + 1, 0);
+    std::vector<int32_t> path((size_t)ntask * 64 + 1);
+    st = vine_solve(sv, ntask, src.data(), snk.data(), bound.data(), cost_l.data(), delay_l.data(), feas.data(),
+                    off.data(), path.data(), (int64_t)path.size(), &stats);
+    if (st == VINE_E_OVERFLOW && off[ntask] > (int64_t)path.size()) {
+      path.resize(off[ntask]);
+      st = vine_solve(sv, ntask, src.data(), snk.data(), bound.data(), cost_l.data(), delay_l.data(),
+                      feas.data(), off.data(), path.data(), (int64_t)path.size(), &stats);
+    }
+    if (st == VINE_OK) {
+      feasible_l.assign(feas.begin(), feas.end());
+      path_l.assign(ntask, {});
+      size_of_result_l.assign(ntask, 0);
+      for (int i = 0; i < ntask; i++) {
+        path_l[i].assign(path.begin() + off[i], path.begin() + off[i + 1]);
+        size_of_result_l[i] = (int)path_l[i].size();
+      }
+    } else {
+      last_error = vine_last_error();
+    }
+    vine_solver_destroy(sv);
+    vine_graph_destroy(dg);
+    return st;
+  }
+
+  vine_options options;
+  vine_stats stats{};
+  int ntask;
+  std::vector<Label_t> init_state;
+  std::vector<int> sink_l;
+  std::vector<Constraint_t> C_l;
+  std::vector<float> cost_l, delay_l;
+  std::vector<int> feasible_l;
+  std::vector<std::vector<int>> path_l;
+  std::vector<int> size_of_result_l;
+  std::string last_error;
+
+ private:
+  int fail(vine_status st) {
+    last_error = vine_last_error();
+    return st;
+  }
+  Graph<Edge_t>* g_;
+};
+
+}  // namespace vine

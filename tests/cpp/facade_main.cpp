@@ -1,0 +1,45 @@
+// Builds a 4x4 grid through the C++ facade (include/vine/cspp.hpp), solves a
+// few tasks and prints one line per task: feasible cost delay path...
+// With argument "args" it only checks the argument errors (no GPU needed).
+#include <cstdio>
+#include <cstring>
+
+#include "vine/cspp.hpp"
+
+int main(int argc, char** argv) {
+  using namespace vine;
+  Graph<c_edge> g(16);
+  for (int r = 0; r < 4; r++)
+    for (int c = 0; c < 4; c++) {
+      int id = r * 4 + c;
+      if (c + 1 < 4) g.add_undirected(id, id + 1, c_edge(1.f + (float)((id * 7) % 5), 1.f + (float)((id * 3) % 4)));
+      if (r + 1 < 4) g.add_undirected(id, id + 4, c_edge(2.f + (float)((id * 5) % 3), 1.f + (float)((id * 11) % 3)));
+    }
+  std::vector<c_label> src = {c_label(0), c_label(0), c_label(5), c_label(3)};
+  std::vector<int> snk = {15, 15, 10, 3};
+  std::vector<c_constraint<c_label>> C = {c_constraint<c_label>(100.f), c_constraint<c_label>(7.f),
+                                          c_constraint<c_label>(2.f), c_constraint<c_label>(0.f)};
+  if (argc > 1 && std::strcmp(argv[1], "args") == 0) {
+    std::vector<int> bad = {15, 15, 10};
+    try {
+      cspp<c_label, c_edge> s(&g, 4, src, bad, C);
+      std::printf("no exception\n");
+      return 1;
+    } catch (const std::invalid_argument&) {
+      std::printf("args ok\n");
+      return 0;
+    }
+  }
+  cspp<c_label, c_edge> solver(&g, 4, src, snk, C);
+  int st = solver.solve();
+  if (st != VINE_OK) {
+    std::printf("error %d %s\n", st, solver.last_error.c_str());
+    return 2;
+  }
+  for (int i = 0; i < 4; i++) {
+    std::printf("%d %.0f %.0f", solver.feasible_l[i], solver.cost_l[i], solver.delay_l[i]);
+    for (int v : solver.path_l[i]) std::printf(" %d", v);
+    std::printf("\n");
+  }
+  return 0;
+}

@@ -164,3 +164,24 @@ def test_product_dop_rule_matches_reference_golden():
         out = np.zeros(sizes.shape[0], np.int64)
         assert fn(sizes.ctypes.data, sizes.shape[0], case["max_labels"], out.ctypes.data) == sizes.shape[0]
         assert out.tolist() == case["dop"]
+
+
+def test_cpp_facade_argument_check():
+    """The C++ facade (include/vine/cspp.hpp) builds with a plain host compiler
+    and rejects mismatched task arrays before touching the device."""
+    import subprocess
+    exe = os.path.join(ROOT, "vine_amd", "vine_facade_check")
+    assert os.path.exists(exe), "built by vine_amd.build"
+    r = subprocess.run([exe, "args"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "args ok" in r.stdout
+
+
+def test_window_labels_are_batch_aware():
+    """Window labelling for the learned scheduler: a query that sets the
+    batch's round count gets the wide window, the rest the least-work one."""
+    from vine_amd.learn import label_windows
+    E = np.array([[100, 400], [100, 400], [1000, 4000]], float)  # labels expanded per window
+    R = np.array([[10, 5], [10, 5], [500, 100]], float)          # rounds per window
+    pick, budget, cost = label_windows(E, R, round_cost=50.0)
+    assert list(pick) == [0, 0, 1] and budget == 100
+    assert cost == 100 + 100 + 4000 + 50.0 * 100

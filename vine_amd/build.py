@@ -65,6 +65,14 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     shutil.move(tmp, OUT)
+    # the C++ facade (include/vine/cspp.hpp) compiled by a plain host compiler
+    # against the library, as a user program would be
+    facade = os.path.join(HERE, "vine_facade_check")
+    cmd = [CXX, "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"), "-o", facade,
+           os.path.join(ROOT, "tests", "cpp", "facade_main.cpp"), "-L" + HERE, "-lvine", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("facade build failed: %s\n%s" % (" ".join(cmd), r.stderr))
     if verbose:
         print("built", OUT)
     return OUT

@@ -672,7 +672,7 @@ vine_status run_sssp(vine_solver* s, int nb, int which, int& sr, vine_stats* st)
 }
 
 // One batch of nb <= B queries whose arrays are already in the slot buffers.
-vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
+vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
   Ctx& c = s->ctx;
   const long long BN = (long long)s->B * c.n;
   // reset per-(query, vertex) headers and counters
@@ -796,6 +796,8 @@ vine_status solve_batch(vine_solver* s, int nb, vine_stats* st) {
     s->trace.resize(at + 10 * (size_t)nrow);
     if (nrow > 0)
       HIP_TRY(hipMemcpy(s->trace.data() + at, s->d_trace, sizeof(float) * 10 * nrow, hipMemcpyDeviceToHost));
+    // slot -> query index of the whole solve
+    for (long long i = 0; i < nrow; i++) s->trace[at + 10 * i] += (float)q0;
   }
   st->rounds += rounds;
   st->expanded += (int64_t)s->h_ct->expanded;
@@ -858,7 +860,7 @@ vine_status solve_impl(vine_solver* s, int32_t nq, const int32_t* d_src, const i
       HIP_TRY(hipMemsetAsync(s->d_qdst + nb, 0, sizeof(int32_t) * (s->B - nb), s->stream));
       HIP_TRY(hipMemsetAsync(s->d_qbound + nb, 0, sizeof(float) * (s->B - nb), s->stream));
     }
-    vine_status bs = solve_batch(s, nb, st);
+    vine_status bs = solve_batch(s, nb, q0, st);
     if (bs != VINE_OK) return bs;
     // per-query finish times: merge-end clock of the last round with live labels
     std::vector<int32_t> last(nb);

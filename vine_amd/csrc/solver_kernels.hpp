@@ -41,6 +41,8 @@ constexpr int CCHUNK = 8;                   // candidate records per list chunk
 constexpr int FCH = 32;                     // far-tier records per list chunk
 constexpr float FAR_MULT = 1.0f;            // near tier covers FAR_MULT windows past the front
 constexpr int NBK = 4096;                   // far-tier key buckets per query
+constexpr float MLP_LOG2_MIN = 0.f;         // learned window range: 1 ..
+constexpr float MLP_LOG2_MAX = 7.f;         //   128 key units
 
 enum : int { OVF_LABEL = 1, OVF_CAND = 2, OVF_TOUCH = 4, OVF_POOL = 8, OVF_CARRY = 16, OVF_SSSP = 32, OVF_FAR = 64 };
 
@@ -199,6 +201,19 @@ __device__ __forceinline__ void wave_add(unsigned long long* ctr, int v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
   if (lane == 0 && v) atomicAdd(ctr, (unsigned long long)v);
+}
+
+// Per-key counter increments aggregated over the wave: lanes holding the same
+// key add together (one atomic per distinct key).  Wave-uniform call.
+__device__ __forceinline__ void wave_add_keyed(int* arr, int key, bool pred) {
+  unsigned long long todo = __ballot(pred);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int k = __shfl(key, leader, WAVE);
+    const unsigned long long same = __ballot(pred && key == k) & todo;
+    if (lane_id() == leader) atomicAdd(&arr[k], __popcll(same));
+    todo &= ~same;
+  }
 }
 
 // Visit every (item, sub-item) pair of the wave, 64 pairs per step: lane i
@@ -544,7 +559,7 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
     long long i = base + lane;
     int id = -1, deg = 0, row = 0, q = 0;
     Label_t lab;
-    bool carry = false;
+    bool carry = false, expd = false;
     float key = 0.f, rd = 0.f;
     if (i < fn) {
       // carried entries bring their query, key and pruning delay along, so a
@@ -585,7 +600,6 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
           if (dop > 0) in_window = (i >= fn - dop);
           else in_window = !(key > __uint_as_float(fmin_cur[q]) + c.q_delta[q]);
           if (in_window) {
-            if (c.stats_on) atomicAdd(&c.q_exp[q], 1);
             if (!fresh) {
               int4 L = c.labs[id];
               lab = LT::unpack(L.x, make_float2(__int_as_float(L.y), __int_as_float(L.z)));
@@ -593,11 +607,15 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
             row = c.row_ptr[lab.vid];
             deg = c.row_ptr[lab.vid + 1] - row;
             n_exp++;
+            expd = true;
           } else {
             carry = true;
           }
         }
       }
+    }
+    if (c.stats_on) {
+      wave_add_keyed(c.q_exp, q, expd);
     }
     // carried labels: near tier if within FAR_MULT windows of the front,
     // else the query's far tier (not scanned again until pulled)
@@ -610,8 +628,8 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
     const bool near = carry && !far;
     int cslot = wave_alloc32(&ct->carry_n[nxt], near);
     const int4 crec4 = make_int4(id, q, __float_as_int(key), __float_as_int(rd));
+    if (c.stats_on) wave_add_keyed(c.q_near, q, near);
     if (near) {
-      if (c.stats_on) atomicAdd(&c.q_near[q], 1);
       if (cslot < c.carry_cap) {
         c.carry[nxt][cslot] = crec4;
         if (__float_as_uint(key) < load_agent_u32(&fmin_nxt[q])) atomicMin(&fmin_nxt[q], __float_as_uint(key));
@@ -935,10 +953,6 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
         if (fits) {
           if (gl == 0) {
             n_ins++;
-            if (c.stats_on) {
-              atomicAdd(&c.q_ins[q], 1);
-              atomicAdd(&c.q_near[q], 1);
-            }
             c.pool_cd[off + slot] = make_float2(cd, cdl);
             c.pool_id[off + slot] = (int)lid;
             float key = LT::key(cl) + lb;
@@ -957,6 +971,10 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
               atomicMin(&fmin_nxt[q], __float_as_uint(key));
           }
           if (slot == cnt) cnt++;
+        }
+        if (c.stats_on) {
+          wave_add_keyed(c.q_ins, q, fits && gl == 0);
+          wave_add_keyed(c.q_near, q, fits && gl == 0);
         }
       }
       h = nexth;
@@ -1091,14 +1109,15 @@ __global__ void __launch_bounds__(256) k_pull(Ctx c) {
 // round starts from, optionally recorded as a trace row, and (VINE_DELTA_MLP)
 // the next window width from the MLP:
 //   x0 log2(1 + near-tier entries queued this round)
-//   x1 log2(1 + far-tier entries)
+//   x1 log2(1 + key range left between the front and t's best bound)
 //   x2 log2(1 + labels expanded this round)
 //   x3 log2(1 + labels inserted this round)
 //   x4 progress of the front from the smallest key to t's best bound, in [0, 1.5]
 //   x5 log2(1 + round)
-//   x6 log2 of the window used this round (20 for an unbounded window)
+//   x6 log2(1 + labels the query has expanded so far)
 //   x7 1 if an answer has been found
-// Output 0 gives the next window as 2^y0 (clamped to [2^-2, 2^16]); output 1
+// Output 0 gives the next window as 2^y0, clamped to the range the network is
+// trained on, [2^MLP_LOG2_MIN, 2^MLP_LOG2_MAX]; output 1
 // is a forecast of log2(1 + next round's expansions).
 __global__ void k_policy(Ctx c, int nq) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1110,7 +1129,8 @@ __global__ void k_policy(Ctx c, int nq) {
   c.q_exp[q] = 0;
   c.q_ins[q] = 0;
   c.q_near[q] = 0;
-  c.q_exptot[q] += exp_;
+  const long long tot = c.q_exptot[q] + exp_;
+  c.q_exptot[q] = tot;
   const int far = c.far_cnt[q];
   const bool active = q < nq && (near + far + exp_) > 0;
   if (!active) return;
@@ -1119,15 +1139,14 @@ __global__ void k_policy(Ctx c, int nq) {
   const float2 best = c.q_best[q];
   float prog = 1.5f;
   if (fmin < FINF && best.x < FINF && best.x > base) prog = fminf(fmaxf((fmin - base) / (best.x - base), 0.f), 1.5f);
-  const float d = c.q_delta[q];
   float x[mlp::NIN];
   x[0] = log2f(1.f + (float)near);
-  x[1] = log2f(1.f + (float)far);
+  x[1] = (fmin < FINF && best.x < FINF) ? log2f(1.f + fmaxf(best.x - fmin, 0.f)) : 0.f;
   x[2] = log2f(1.f + (float)exp_);
   x[3] = log2f(1.f + (float)ins);
   x[4] = prog;
   x[5] = log2f(1.f + (float)rn);
-  x[6] = (d < FINF) ? log2f(fmaxf(d, 0.25f)) : 20.f;
+  x[6] = log2f(1.f + (float)tot);
   x[7] = c.q_best_id[q] >= 0 ? 1.f : 0.f;
   if (c.trace && c.trace_every > 0 && rn % c.trace_every == 0) {
     long long row = atomicAdd((unsigned long long*)&ct->trace_n, 1ull);
@@ -1144,7 +1163,7 @@ __global__ void k_policy(Ctx c, int nq) {
     mlp::apply(c.mlp, x, y);
     float l = y[0];
     if (!(l == l)) l = log2f(fmaxf(c.far_floor, 1.f));  // NaN output: fall back to a small window
-    l = fminf(fmaxf(l, -2.f), 16.f);
+    l = fminf(fmaxf(l, MLP_LOG2_MIN), MLP_LOG2_MAX);
     c.q_delta[q] = exp2f(l);
     c.q_fcast[q] = y[1];
   }
