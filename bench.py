@@ -36,7 +36,7 @@ def main():
     ap.add_argument("--queries", type=int, default=4096)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--policy", default="fixed", choices=["fixed", "ar", "mlp", "inf"])
-    ap.add_argument("--delta", type=float, default=2000.0)
+    ap.add_argument("--delta", type=float, default=12.0)
     ap.add_argument("--plain-queries", type=int, default=100, help="per bound factor")
     ap.add_argument("--cpu-queries", type=int, default=32)
     ap.add_argument("--cpu-limit", type=float, default=12.0)
