@@ -85,7 +85,10 @@ typedef struct {
   int32_t record_trace;     /* keep per-round frontier statistics (for vine_solver_trace) */
   int32_t time_kernels;     /* bracket every round kernel with HIP events (vine_stats.ms_*) */
   float bound_delta;        /* distance window of the bound search (0 = 2 x mean arc weight) */
-  int32_t reserved[5];
+  int32_t bound_ball;       /* 0: a forward min-delay search from s first, then the reverse bound
+                               searches only inside fwd + lbd <= bound; 1: reverse searches over
+                               the whole ball lbd <= bound (no forward search) */
+  int32_t reserved[4];
 } vine_options;
 
 typedef struct {

@@ -121,14 +121,17 @@ def test_rmat_hubs(gpu, rmat_case, prune):
 
 
 @pytest.mark.gpu
-def test_knn_sample_bounds(gpu):
+@pytest.mark.parametrize("bound_ball", [0, 1])
+def test_knn_sample_bounds(gpu, bound_ball):
     """Geometric k-NN graph (the road-like generator) at 20k vertices with the
-    lower-bound pruning, whole-frontier and windowed schedules."""
+    lower-bound pruning, whole-frontier and windowed schedules; bounds from the
+    forward-restricted searches (0) and from whole balls around t (1)."""
     g = W.knn(20000, seed=5)
     q = W.hop_queries(g, 30, hop_lo=5, hop_hi=80, f_permille=(1050, 1500), seed=5)
     ref = [O.cspp(g, q.src[i], q.dst[i], q.bound[i], prune_target=2) for i in range(len(q))]
+    print("knn20k bounds bound_ball=%d" % bound_ball, flush=True)
     for pol, dlt in ((S.DELTA_INF, 0.0), (S.DELTA_FIXED, 500.0)):
-        with S.Solver(g, max_batch=16, prune=2, delta_policy=pol, delta_fixed=dlt) as s:
+        with S.Solver(g, max_batch=16, prune=2, delta_policy=pol, delta_fixed=dlt, bound_ball=bound_ball) as s:
             ans = s.solve(q.src, q.dst, q.bound)
         _compare(g, q, ref, ans)
 

@@ -112,6 +112,7 @@ struct vine_solver {
   long long* d_touched = nullptr;
   int4* d_carry[2] = {nullptr, nullptr};
   int32_t* d_far_head = nullptr;
+  unsigned long long* d_far_occ = nullptr;
   int32_t* d_far_next = nullptr;
   int32_t* d_far_cnt = nullptr;
   float* d_far_base = nullptr;
@@ -314,7 +315,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   else vine_options_default(&o);
   if (o.max_batch < 1 || o.prune < 0 || o.prune > 2 || o.delta_policy < 0 || o.delta_policy > 3 ||
       o.block_size < 64 || o.block_size > 1024 || o.block_size % 64 || o.rounds_per_sync < 1 ||
-      o.grid_blocks < 0 || o.hash_slots < 0 || !(o.delta_fixed >= 0.f)) {
+      o.grid_blocks < 0 || o.hash_slots < 0 || !(o.delta_fixed >= 0.f) || o.bound_ball < 0 || o.bound_ball > 1) {
     set_err("vine_solver_create: invalid options");
     return VINE_E_ARG;
   }
@@ -425,6 +426,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   long long far_cap = std::max<long long>(4 * carry_cap / dev::FCH, 2 * B + 64);
   if (far_cap > INT32_MAX - 1) far_cap = INT32_MAX - 1;
   ALLOC(s->d_far_head, sizeof(int32_t) * B * dev::NBK);
+  ALLOC(s->d_far_occ, sizeof(unsigned long long) * B * dev::NBW);
   ALLOC(s->d_far_next, sizeof(int32_t) * B);
   ALLOC(s->d_far_cnt, sizeof(int32_t) * B);
   ALLOC(s->d_far_base, sizeof(float) * B);
@@ -488,6 +490,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   c.carry[1] = s->d_carry[1];
   c.carry_cap = (int32_t)carry_cap;
   c.far_head = s->d_far_head;
+  c.far_occ = s->d_far_occ;
   c.far_next = s->d_far_next;
   c.far_cnt = s->d_far_cnt;
   c.far_base = s->d_far_base;
@@ -496,13 +499,14 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   c.far_free = s->d_far_free;
   c.frec = s->d_frec;
   c.far_cap = (int32_t)far_cap;
-  c.far_floor = 0.125f * g->mean_len;
+  c.far_floor = 0.0625f * g->mean_len;
   c.sfr[0] = s->d_sfr[0];
   c.sfr[1] = s->d_sfr[1];
   c.sf_cap = (int32_t)sf_cap;
   c.ts = s->d_ts;
   c.ct = s->d_ct;
   c.hash_slots = o.hash_slots;
+  c.ellipse = o.bound_ball ? 0 : 1;
   *out = s;
   return VINE_OK;
 }
@@ -640,7 +644,7 @@ vine_status run_sssp(vine_solver* s, int nb, int which, int& sr, vine_stats* st)
   hipLaunchKernelGGL(dev::k_sssp_init, dim3((nb + 255) / 256), dim3(256), 0, s->stream, c, nb, which, sr);
   HIP_TRY(hipGetLastError());
   float delta = s->opt.bound_delta > 0 ? s->opt.bound_delta
-                                       : 2.0f * (which ? s->g->mean_len : s->g->mean_delay);
+                                       : 2.0f * (which == 1 ? s->g->mean_len : s->g->mean_delay);
   const int K = std::max(1, std::min(4, s->opt.rounds_per_sync));
   long long last = nb;
   const int r0 = sr;
@@ -650,6 +654,8 @@ vine_status run_sssp(vine_solver* s, int nb, int which, int& sr, vine_stats* st)
       Timer t(s, 2);
       if (which == 0)
         hipLaunchKernelGGL(dev::k_sssp<0>, dim3(gsz), dim3(s->block), 0, s->stream, c, sr, delta);
+      else if (which == 2)
+        hipLaunchKernelGGL(dev::k_sssp<2>, dim3(gsz), dim3(s->block), 0, s->stream, c, sr, delta);
       else
         hipLaunchKernelGGL(dev::k_sssp<1>, dim3(gsz), dim3(s->block), 0, s->stream, c, sr, delta);
       HIP_TRY(hipGetLastError());
@@ -679,6 +685,7 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
   HIP_TRY(hipMemsetAsync(s->d_pset, 0, sizeof(int2) * BN, s->stream));
   HIP_TRY(hipMemsetAsync(s->d_cand_head, 0xFF, sizeof(int32_t) * BN, s->stream));
   HIP_TRY(hipMemsetAsync(s->d_far_head, 0xFF, sizeof(int32_t) * s->B * dev::NBK, s->stream));
+  HIP_TRY(hipMemsetAsync(s->d_far_occ, 0, sizeof(unsigned long long) * s->B * dev::NBW, s->stream));
   std::memset(s->h_ct, 0, sizeof(Counters));
   s->h_ct->label_ctr = (unsigned long long)s->B;
   s->h_ct->pool_ctr = (unsigned long long)s->B * dev::INIT_CAP;
@@ -706,7 +713,16 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
     if (fs != VINE_OK) return fs;
     HIP_TRY(hipMemsetAsync(s->d_inq, 0, sizeof(uint32_t) * BN, s->stream));
     int sr = 0;
-    vine_status ss = run_sssp(s, nb, 0, sr, st);
+    vine_status ss = VINE_OK;
+    if (c.ellipse) {
+      // forward min delay into lbl, the delay bound search inside
+      // fwd + lbd <= bound, then lbl back to +inf for the len search
+      ss = run_sssp(s, nb, 2, sr, st);
+      if (ss == VINE_OK) ss = run_sssp(s, nb, 0, sr, st);
+      if (ss == VINE_OK) ss = fill_f32(s, s->d_lbl, dev::FINF, BN);
+    } else {
+      ss = run_sssp(s, nb, 0, sr, st);
+    }
     if (ss == VINE_OK) ss = run_sssp(s, nb, 1, sr, st);
     if (ss != VINE_OK) return ss;
   }

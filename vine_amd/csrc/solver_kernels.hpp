@@ -38,9 +38,10 @@ constexpr float EXACT_LIMIT = 16777216.0f;  // 2^24
 constexpr int INIT_CAP = 4;                 // first Pareto-set allocation (entries)
 constexpr int TS_CAP = 1 << 20;             // per-round timestamps kept
 constexpr int CCHUNK = 8;                   // candidate records per list chunk
-constexpr int FCH = 32;                     // far-tier records per list chunk
+constexpr int FCH = 64;                     // far-tier records per list chunk (one per lane)
 constexpr float FAR_MULT = 1.0f;            // near tier covers FAR_MULT windows past the front
-constexpr int NBK = 4096;                   // far-tier key buckets per query
+constexpr int NBK = 32768;                  // far-tier key buckets per query
+constexpr int NBW = NBK / 64;               // occupancy words per query
 constexpr float MLP_LOG2_MIN = 0.f;         // learned window range: 1 ..
 constexpr float MLP_LOG2_MAX = 7.f;         //   128 key units
 
@@ -123,6 +124,7 @@ struct Ctx {
   // beyond) as a list of chunks of FCH entries.  Buckets below far_next are
   // "pulled": their keys belong to the near tier.
   int32_t* far_head;  // [B * NBK]
+  unsigned long long* far_occ;  // [B * NBW] bit b: bucket b holds a chunk
   int32_t* far_next;  // [B] first bucket not yet pulled
   int32_t* far_cnt;   // [B] entries in the query's far tier
   float* far_base;    // [B] smallest possible key of the query
@@ -137,6 +139,7 @@ struct Ctx {
   unsigned long long* ts;  // [TS_CAP] per-round merge-end wall clock
   Counters* ct;
   int32_t hash_slots;
+  int32_t ellipse;  // reverse bound searches restricted to fwd + dist <= bound (fwd in lbl)
   // scheduler: per-query round statistics, MLP, feature trace
   int32_t stats_on;
   int32_t policy;     // VINE_DELTA_*
@@ -409,7 +412,11 @@ __global__ void k_init_batch(Ctx c, int nq) {
 // and carries the rest, which keeps re-relaxation close to Dijkstra's while a
 // round still covers every query of the batch.  inq marks (query, vertex)
 // pairs present in the frontier (exactly one copy each).
-//   WHICH = 0: delay, limited to radius `bound`.
+//   WHICH = 2: forward min delay fwd from s (on the graph itself, into lbl),
+//   limited to radius `bound`.  Run first when c.ellipse is set.
+//   WHICH = 0: delay to t, limited to radius `bound`, or with c.ellipse to
+//   fwd + lbd <= bound.  A label at u has delay >= fwd(u), so outside that
+//   region every label fails the delay test and lbd = +inf prunes it the same.
 //   WHICH = 1: len, restricted to vertices with lbd <= bound: only those can
 //   lie on a feasible completion, so the restricted distance is a valid lower
 //   bound for every label that passes the delay test.
@@ -419,7 +426,8 @@ __global__ void k_init_batch(Ctx c, int nq) {
 __global__ void k_sssp_init(Ctx c, int nq, int which, int r) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
-  long long qt = (long long)q * c.n + c.q_dst[q];
+  // which 2: forward from s (into lbl); 0 and 1: reverse from t
+  long long qt = (long long)q * c.n + (which == 2 ? c.q_src[q] : c.q_dst[q]);
   if (which) c.lbl[qt] = 0.f;
   else c.lbd[qt] = pack2(0.f, 0.f);
   c.inq[qt] = 1;
@@ -474,8 +482,9 @@ __global__ void __launch_bounds__(256) k_sssp(Ctx c, int r, float delta) {
           du = hi_of(pk);
           dl2 = lo_of(pk);
         }
-        row = c.rrow_ptr[u];
-        deg = c.rrow_ptr[u + 1] - row;
+        const int32_t* rp = (WHICH == 2) ? c.row_ptr : c.rrow_ptr;
+        row = rp[u];
+        deg = rp[u + 1] - row;
       }
     }
     int cslot = wave_alloc32(&c.ct->sf_n[(r + 1) % 3], carry);
@@ -496,11 +505,17 @@ __global__ void __launch_bounds__(256) k_sssp(Ctx c, int r, float delta) {
       float nd = 0.f;
       if (act) {
         relaxed++;
-        int4 e = c.redges[pos];
+        int4 e = (WHICH == 2) ? c.edges[pos] : c.redges[pos];
         float bnd = c.q_bound[oq];
         wv = (long long)oq * c.n + e.x;
         bool lowered = false;
-        if (WHICH) {
+        if (WHICH == 2) {
+          nd = odu + __int_as_float(e.z);
+          if (nd <= bnd && nd < c.lbl[wv]) {
+            unsigned old = atomicMin((unsigned*)&c.lbl[wv], __float_as_uint(nd));
+            lowered = __float_as_uint(nd) < old;
+          }
+        } else if (WHICH == 1) {
           nd = odu + __int_as_float(e.y);
           if (lbd_of(c, wv) <= bnd && nd < c.lbl[wv]) {
             unsigned old = atomicMin((unsigned*)&c.lbl[wv], __float_as_uint(nd));
@@ -509,7 +524,11 @@ __global__ void __launch_bounds__(256) k_sssp(Ctx c, int r, float delta) {
         } else {
           nd = odu + __int_as_float(e.z);
           const unsigned long long pk = pack2(nd, odl + __int_as_float(e.y));
-          if (nd <= bnd && pk < c.lbd[wv]) {
+          // with the forward search done, only vertices with fwd + lbd <= bound:
+          // every vertex of a feasible s-t path lies there, and so does every
+          // vertex of such a vertex's shortest path to t
+          const float lim = c.ellipse ? bnd - c.lbl[wv] : bnd;
+          if (nd <= lim && pk < c.lbd[wv]) {
             unsigned long long old = atomicMin(&c.lbd[wv], pk);
             lowered = pk < old;
           }
@@ -665,6 +684,8 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
             if (prev == expected) break;
             expected = prev;
           }
+          // a bucket becomes non-empty only through a CAS from -1
+          if (expected < 0) atomicOr(&c.far_occ[(long long)q * NBW + (fb >> 6)], 1ull << (fb & 63));
         }
       }
     }
@@ -996,8 +1017,9 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
 // moved whole into the near tier; if the near tier is empty, the next
 // non-empty bucket is.  A far entry is therefore moved at most once, and
 // entries whose keys the upper bound prunes are never read again.  Dead and
-// pruned entries are dropped on the way, chunks go to the free stack.  One
-// wave per query.
+// pruned entries are dropped on the way; chunks are reclaimed with the batch.
+// An occupancy bitmap (one bit per bucket) lets a wave find the non-empty
+// buckets of a key range 4,096 buckets per load step.  One wave per query.
 template <int PRUNE>
 __global__ void __launch_bounds__(256) k_pull(Ctx c) {
   Counters* ct = c.ct;
@@ -1011,7 +1033,6 @@ __global__ void __launch_bounds__(256) k_pull(Ctx c) {
   if (nb >= NBK) return;
   const float front = __uint_as_float(load_agent_u32(&fmin[q]));
   const float reach = fmaxf(FAR_MULT * c.q_delta[q], c.far_floor);
-  const int* heads = c.far_head + q * NBK;
   if (c.far_cnt[q] <= 0) {
     // nothing parked: only keep the near/far boundary up with the front
     if (lane == 0 && front < FINF) c.far_next[q] = max(nb, far_bucket(c, (int)q, front + reach) + 1);
@@ -1029,56 +1050,86 @@ __global__ void __launch_bounds__(256) k_pull(Ctx c) {
     }
     return;
   }
+  unsigned long long* occ = c.far_occ + q * NBW;
   int target;
   if (front < FINF) {
     target = far_bucket(c, (int)q, front + reach);
   } else {
-    // near tier empty: the first non-empty bucket, if any
+    // near tier empty: the first non-empty bucket, if any (64 buckets a word)
     target = -1;
-    for (int b0 = nb; b0 < NBK && target < 0; b0 += WAVE) {
-      int b = b0 + lane;
-      unsigned long long m = __ballot(b < NBK && heads[b] >= 0);
-      if (m) target = b0 + __ffsll((long long)m) - 1;
+    for (int w0 = nb >> 6; w0 < NBW && target < 0; w0 += WAVE) {
+      const int w = w0 + lane;
+      unsigned long long bits = 0;
+      if (w < NBW) {
+        bits = occ[w];
+        if (w == (nb >> 6)) bits &= ~0ull << (nb & 63);
+      }
+      const unsigned long long m = __ballot(bits != 0);
+      if (m) {
+        const int l = __ffsll((long long)m) - 1;
+        const unsigned long long wb = __shfl(bits, l, WAVE);
+        target = ((w0 + l) << 6) + __ffsll((long long)wb) - 1;
+      }
     }
     if (target < 0) return;
   }
   if (target < nb) return;
   long long moved = 0, dropped = 0;
-  for (int b0 = nb; b0 <= target; b0 += WAVE) {
-   // find the non-empty buckets of this stretch together, then drain them
-   const int bl = b0 + lane;
-   unsigned long long ne = __ballot(bl <= target && heads[bl] >= 0);
-   while (ne) {
-    const int b = b0 + __ffsll((long long)ne) - 1;
-    ne &= ne - 1;
-    int h = c.far_head[q * NBK + b];
-    int guard = 0;
-    int2 hd = (h >= 0) ? c.fhd[h] : make_int2(0, -1);
-    while (h >= 0 && guard++ < c.far_cap) {
-      // the next header is requested before this chunk is processed
-      const int2 hd_next = (hd.y >= 0) ? c.fhd[hd.y] : make_int2(0, -1);
-      const int cnt = min(hd.x, FCH);
-      const int4 rec = (lane < cnt) ? c.frec[(long long)h * FCH + lane] : make_int4(-1, 0, 0, 0);
-      bool live = lane < cnt && !c.dead[rec.x];
-      const float key = __int_as_float(rec.z), rd = __int_as_float(rec.w);
-      if (live && PRUNE != 0 && lex_ge(key, rd, best.x, best.y)) live = false;
-      dropped += (lane < cnt && !live) ? 1 : 0;
-      int slot = wave_alloc32(&ct->carry_n[nc], live);
-      if (live) {
-        moved++;
-        if (slot < c.carry_cap) {
-          c.carry[nc][slot] = rec;
-          if (__float_as_uint(key) < load_agent_u32(&fmin[q])) atomicMin(&fmin[q], __float_as_uint(key));
-        } else {
-          atomicOr(&ct->overflow, OVF_CARRY);
+  float kmin = FINF;
+  const int wlast = target >> 6;
+  for (int w0 = nb >> 6; w0 <= wlast; w0 += WAVE) {
+    // the occupied buckets of [nb, target], 64 words at a time; their bits
+    // are cleared here (expand appends only to buckets >= the new far_next)
+    const int w = w0 + lane;
+    unsigned long long bits = 0;
+    if (w <= wlast) {
+      const unsigned long long word = occ[w];
+      bits = word;
+      if (w == (nb >> 6)) bits &= ~0ull << (nb & 63);
+      if (w == wlast && (target & 63) != 63) bits &= (1ull << ((target & 63) + 1)) - 1ull;
+      if (bits) occ[w] = word & ~bits;
+    }
+    unsigned long long lanes = __ballot(bits != 0);
+    while (lanes) {
+      const int l = __ffsll((long long)lanes) - 1;
+      lanes &= lanes - 1;
+      unsigned long long wb = __shfl(bits, l, WAVE);
+      // the word's 64 bucket heads in one load; drained heads reset at once
+      int* hrow = c.far_head + q * NBK + ((long long)(w0 + l) << 6);
+      const int myhead = hrow[lane];
+      if ((wb >> lane) & 1ull) hrow[lane] = -1;
+      while (wb) {
+        const int bit = __ffsll((long long)wb) - 1;
+        wb &= wb - 1;
+        int h = __shfl(myhead, bit, WAVE);
+        int guard = 0;
+        int2 hd = (h >= 0) ? c.fhd[h] : make_int2(0, -1);
+        while (h >= 0 && guard++ < c.far_cap) {
+          // the next header is requested before this chunk is processed
+          const int2 hd_next = (hd.y >= 0) ? c.fhd[hd.y] : make_int2(0, -1);
+          const int cnt = min(hd.x, FCH);
+          const int4 rec = (lane < cnt) ? c.frec[(long long)h * FCH + lane] : make_int4(-1, 0, 0, 0);
+          bool live = lane < cnt && !c.dead[rec.x];
+          const float key = __int_as_float(rec.z), rd = __int_as_float(rec.w);
+          if (live && PRUNE != 0 && lex_ge(key, rd, best.x, best.y)) live = false;
+          dropped += (lane < cnt && !live) ? 1 : 0;
+          int slot = wave_alloc32(&ct->carry_n[nc], live);
+          if (live) {
+            moved++;
+            kmin = fminf(kmin, key);
+            if (slot < c.carry_cap) c.carry[nc][slot] = rec;
+            else atomicOr(&ct->overflow, OVF_CARRY);
+          }
+          h = hd.y;
+          hd = hd_next;
         }
       }
-      h = hd.y;
-      hd = hd_next;
     }
-    if (lane == 0) c.far_head[q * NBK + b] = -1;
-   }
   }
+  // this wave is the only writer of the query's front during pull
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) kmin = fminf(kmin, __shfl_xor(kmin, o, WAVE));
+  if (lane == 0 && kmin < FINF && __float_as_uint(kmin) < fmin[q]) atomicMin(&fmin[q], __float_as_uint(kmin));
   {
     long long mq = moved + dropped;
 #pragma unroll

@@ -46,7 +46,8 @@ class Options(ctypes.Structure):
         ("record_trace", ctypes.c_int32),
         ("time_kernels", ctypes.c_int32),
         ("bound_delta", ctypes.c_float),
-        ("reserved", ctypes.c_int32 * 5),
+        ("bound_ball", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
