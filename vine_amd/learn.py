@@ -30,8 +30,11 @@ from . import mlp as M
 from . import solver as S
 from . import workload as W
 
-CANDIDATES = (3.0, 6.0, 12.0, 24.0, 48.0)
-ROUND_COST = 2000.0  # labels a round's fixed kernel cost is worth (batched solver, measured)
+CANDIDATES = (4.0, 8.0, 12.0, 16.0, 24.0, 32.0, 48.0)
+# labels a round's fixed kernel cost is worth: batched solver on MI355X,
+# about 77 us per round and 68 ns per expanded label (least squares over
+# fixed-window runs of the 4,096-query NY workload)
+ROUND_COST = 1150.0
 TRACE_EVERY = 4  # feature rows every 4th round
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(HERE, "models", "delta_mlp.npy")
