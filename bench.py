@@ -3,11 +3,12 @@
 Workload (one "step"): 4,096 queries on the 264,346-vertex 3-NN geometric
 graph, hop distances log-uniform in [10, 2000] (truncated at the graph's hop
 diameter), bounds floor(U[1.05, 2.0] x min delay), solved by the batched
-solver in one pass (all queries share every round launch).  Inputs are in HBM
-before the timed region; answers stay in HBM.  Beside it: the plain
-single-query solver on the random-pair sweeps (bound 1.1x and 1.5x), the
-roofline of the dominant kernel from HIP-event durations, and the CPU oracle
-timed on a bounded sample.
+solver in one pass (all queries share every round launch) with the learned
+per-query window (vine_amd/models/delta_mlp.npy).  Inputs are in HBM before
+the timed region; answers stay in HBM.  Beside it: the plain single-query
+solver (one query per solve, fixed window) on the random-pair sweeps (bound
+1.1x and 1.5x), the roofline of the dominant kernel from HIP-event durations,
+and the CPU oracle timed on a bounded sample.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 Prints one JSON line.
@@ -35,8 +36,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--queries", type=int, default=4096)
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--policy", default="fixed", choices=["fixed", "ar", "mlp", "inf"])
-    ap.add_argument("--delta", type=float, default=12.0)
+    ap.add_argument("--policy", default="mlp", choices=["fixed", "ar", "mlp", "inf"])
+    ap.add_argument("--delta", type=float, default=None,
+                    help="window (fixed) or first window (mlp); default 12 and 4 key units")
+    ap.add_argument("--model", default=os.path.join(ROOT, "vine_amd", "models", "delta_mlp.npy"))
+    ap.add_argument("--plain-delta", type=float, default=12.0, help="window of the plain single-query solver")
     ap.add_argument("--plain-queries", type=int, default=100, help="per bound factor")
     ap.add_argument("--cpu-queries", type=int, default=32)
     ap.add_argument("--cpu-limit", type=float, default=12.0)
@@ -69,12 +73,16 @@ def main():
         % (len(qb), np.percentile(qb.hops, 50), np.percentile(qb.hops, 99), time.time() - t0))
 
     policy = {"fixed": S.DELTA_FIXED, "ar": S.DELTA_AR, "mlp": S.DELTA_MLP, "inf": S.DELTA_INF}[args.policy]
+    if args.delta is None:
+        args.delta = 4.0 if args.policy == "mlp" else 12.0
     dg = S.DeviceGraph(g)
     solver = S.Solver(dg, max_batch=args.batch, prune=S.PRUNE_BOUNDS, delta_policy=policy,
                       delta_fixed=args.delta, rounds_per_sync=16, time_kernels=1,
                       label_capacity=1_000_000_000, pool_capacity=2_000_000_000,
                       cand_capacity=100_000_000, frontier_capacity=400_000_000)
-    log("solver ready (batch %d)" % args.batch)
+    if policy == S.DELTA_MLP:
+        solver.set_mlp(np.load(args.model))
+    log("solver ready (batch %d, policy %s, window %g)" % (args.batch, args.policy, args.delta))
     nq = len(qb)
     d_src = torch.from_numpy(qb.src).to(dev)
     d_dst = torch.from_numpy(qb.dst).to(dev)
@@ -113,6 +121,7 @@ def main():
         "expand": (agg["ms_expand"], agg["expand_launches"], 20.0 * agg["expanded"] + 12.0 * agg["relaxed"]),
         "merge": (agg["ms_merge"], agg["merge_launches"], 24.0 * agg["inserted"]),
         "bound_sssp": (agg["ms_sssp"], agg["sssp_launches"], 20.0 * agg["sssp_relaxed"]),
+        "pull_policy": (agg["ms_pull"], agg["expand_launches"], 16.0 * agg["far_pulled"]),
     }
     dom = max(kern, key=lambda k: kern[k][0])
     ms_k, launches, nbytes = kern[dom]
@@ -124,14 +133,16 @@ def main():
                 "launches_per_step": launches / args.steps,
                 "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()}}
 
-    # plain single-query solver (reference-style: one query at a time)
+    # plain single-query solver: the drop-in, one query per solve, with the
+    # fixed window (the reference's own AR rule is far slower still: it never
+    # narrows the window at its default MAX)
     plain = None
     if not args.no_plain and args.plain_queries > 0:
         q11 = W.random_queries(g, 1000, (11, 10), seed=0)
         q15 = W.random_queries(g, 1000, (15, 10), seed=0)
         sub = np.arange(args.plain_queries)
-        with S.Solver(dg, max_batch=1, prune=S.PRUNE_BOUNDS, delta_policy=policy, delta_fixed=args.delta,
-                      rounds_per_sync=1, label_capacity=200_000_000) as ps:
+        with S.Solver(dg, max_batch=1, prune=S.PRUNE_BOUNDS, delta_policy=S.DELTA_FIXED,
+                      delta_fixed=args.plain_delta, rounds_per_sync=16, label_capacity=200_000_000) as ps:
             res = {}
             for name, qq in (("1.1x", q11.subset(sub)), ("1.5x", q15.subset(sub))):
                 t1 = time.perf_counter()
@@ -202,6 +213,8 @@ def main():
         "config": {"workload": "ny_batch: 264,346-vertex 3-NN graph (%d arcs), %d queries, hop-distance "
                                "log-uniform [10,2000], bound U[1.05,2.0] x min delay" % (g.m, nq),
                    "batch": args.batch, "policy": args.policy, "delta": args.delta, "prune": "bounds",
+                   "plain": "1 query per solve, fixed window %g, random pairs bound 1.1x (%d queries)"
+                            % (args.plain_delta, args.plain_queries),
                    "parallelism": "single GPU"},
         "latency_us": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "work_per_step": {k: agg[k] / args.steps for k in ("rounds", "sssp_rounds", "expanded", "relaxed",

@@ -1,0 +1,53 @@
+"""What the trained window network gains over the method without it.
+
+The method without it is the reference's bucket-width rule: the DOP count of
+include/AR.hpp driven by the AR(3) forecast, with the reference's MAX (free
+device memory / sizeof(label), algorithm/cspp.cuh:43-47).  Both runs use the
+same kernels, bounds and batch; only the window policy differs, so answers
+must be identical and the gain shows in labels expanded, rounds and time.
+The network is the committed one (vine_amd/models/delta_mlp.npy), trained on
+NY-scale queries with seeds disjoint from these.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from vine_amd import learn as L
+from vine_amd import solver as S
+from vine_amd import workload as W
+
+
+def _run(dg, q, policy, delta, params=None):
+    with S.Solver(dg, max_batch=len(q), prune=S.PRUNE_BOUNDS, delta_policy=policy, delta_fixed=delta,
+                  rounds_per_sync=16) as s:
+        if params is not None:
+            s.set_mlp(params)
+        s.solve(q.src[:8], q.dst[:8], q.bound[:8], paths=False)  # warm-up (code objects, allocations)
+        t0 = time.perf_counter()
+        a = s.solve(q.src, q.dst, q.bound, paths=False)
+        return a, time.perf_counter() - t0
+
+
+@pytest.mark.gpu
+def test_learned_window_beats_ar_rule(gpu):
+    params = np.load(L.DEFAULT_MODEL)
+    g = W.knn(50000, seed=7)
+    q = W.hop_queries(g, 256, hop_lo=20, hop_hi=300, seed=7)
+    dg = S.DeviceGraph(g)
+    ar, t_ar = _run(dg, q, S.DELTA_AR, 0.0)
+    ml, t_ml = _run(dg, q, S.DELTA_MLP, 4.0, params)
+    cost = {}
+    for name, a, t in (("ar", ar, t_ar), ("mlp", ml, t_ml)):
+        st = a.stats
+        cost[name] = st["expanded"] + L.ROUND_COST * st["rounds"]
+        print("%s: %.3f s, %d rounds, %d labels expanded, modelled cost %.3g"
+              % (name, t, st["rounds"], st["expanded"], cost[name]), flush=True)
+    # scheduling only: the same exact answers
+    assert np.array_equal(ar.feasible, ml.feasible)
+    assert np.array_equal(ar.cost, ml.cost) and np.array_equal(ar.delay, ml.delay)
+    assert ml.feasible.sum() > 0.9 * len(q)
+    # the gain: less work in the measured cost model, and less time
+    assert ml.stats["expanded"] < 0.5 * ar.stats["expanded"]
+    assert cost["mlp"] < 0.5 * cost["ar"]
+    assert t_ml < t_ar

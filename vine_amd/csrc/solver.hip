@@ -118,7 +118,7 @@ struct vine_solver {
   float* d_far_base = nullptr;
   float* d_far_bw = nullptr;
   int2* d_fhd = nullptr;
-  int32_t* d_far_free = nullptr;
+  int2* d_dl = nullptr;
   int4* d_frec = nullptr;
   long long* d_sfr[2] = {nullptr, nullptr};
   unsigned long long* d_ts = nullptr;
@@ -421,10 +421,11 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   ALLOC(s->d_touched, sizeof(long long) * chunk_cap);
   ALLOC(s->d_carry[0], sizeof(int4) * carry_cap);
   ALLOC(s->d_carry[1], sizeof(int4) * carry_cap);
-  // far tier: chunks of dev::FCH entries from a bump allocator (reset per
-  // batch); sized for four times the near tier's entries
-  long long far_cap = std::max<long long>(4 * carry_cap / dev::FCH, 2 * B + 64);
-  if (far_cap > INT32_MAX - 1) far_cap = INT32_MAX - 1;
+  // far tier: blocks of dev::FCH records from a bump allocator (reset per
+  // batch), sized for four times the near tier's entries; one drain-list
+  // segment at most per block
+  long long far_cap = std::max<long long>(4 * carry_cap / dev::FCH, 4LL * B + 4096);
+  if (far_cap > (1LL << 27) - 1) far_cap = (1LL << 27) - 1;  // chunk pointers keep 27 bits of block
   ALLOC(s->d_far_head, sizeof(int32_t) * B * dev::NBK);
   ALLOC(s->d_far_occ, sizeof(unsigned long long) * B * dev::NBW);
   ALLOC(s->d_far_next, sizeof(int32_t) * B);
@@ -432,7 +433,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   ALLOC(s->d_far_base, sizeof(float) * B);
   ALLOC(s->d_far_bw, sizeof(float) * B);
   ALLOC(s->d_fhd, sizeof(int2) * far_cap);
-  ALLOC(s->d_far_free, sizeof(int32_t) * far_cap);
+  ALLOC(s->d_dl, sizeof(int2) * far_cap);
   ALLOC(s->d_frec, sizeof(int4) * dev::FCH * far_cap);
   ALLOC(s->d_ts, sizeof(unsigned long long) * dev::TS_CAP);
   ALLOC(s->d_ct, sizeof(Counters));
@@ -496,7 +497,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   c.far_base = s->d_far_base;
   c.far_bw = s->d_far_bw;
   c.fhd = s->d_fhd;
-  c.far_free = s->d_far_free;
+  c.dl = s->d_dl;
   c.frec = s->d_frec;
   c.far_cap = (int32_t)far_cap;
   c.far_floor = 0.0625f * g->mean_len;
@@ -768,7 +769,9 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
     if (ps != VINE_OK) return ps;
     const Counters& h = *s->h_ct;
     int par = h.round_e & 1;
-    long long fn = (long long)h.carry_n[par] + ((long long)h.label_ctr - h.lstart[par]) + h.far_live;
+    // near tier, new labels, far segments queued for the next expand, far tier
+    long long fn = (long long)h.carry_n[par] + ((long long)h.label_ctr - h.lstart[par]) +
+                   (long long)h.drain_n * dev::FCH + h.far_live;
     st->max_frontier = std::max<int64_t>(st->max_frontier, fn);
     fn_last = std::max<long long>(fn, 1);
     if (fn == 0) break;

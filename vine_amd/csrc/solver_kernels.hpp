@@ -38,7 +38,8 @@ constexpr float EXACT_LIMIT = 16777216.0f;  // 2^24
 constexpr int INIT_CAP = 4;                 // first Pareto-set allocation (entries)
 constexpr int TS_CAP = 1 << 20;             // per-round timestamps kept
 constexpr int CCHUNK = 8;                   // candidate records per list chunk
-constexpr int FCH = 64;                     // far-tier records per list chunk (one per lane)
+constexpr int FCH = 64;                     // far-tier block: records per segment (one per lane)
+constexpr int FCLS_MAX = 10;                // far chunks grow 1, 2, 4 .. 1024 blocks along a bucket
 constexpr float FAR_MULT = 1.0f;            // near tier covers FAR_MULT windows past the front
 constexpr int NBK = 32768;                  // far-tier key buckets per query
 constexpr int NBW = NBK / 64;               // occupancy words per query
@@ -67,8 +68,8 @@ struct Counters {
   unsigned long long touched_total;  // (query, vertex) merges
   unsigned long long merge_scanned;  // Pareto entries compared in merge
   int merge_chain_max;               // longest candidate list merged by one lane
-  int far_chunks;                    // far-tier chunks handed out (bump)
-  int far_free_top;                  // far-tier chunks on the free stack
+  int far_chunks;                    // far-tier blocks handed out (bump)
+  int drain_n;                       // drain-list segments queued for the next expand
   long long far_live;                // records currently in far tiers
   unsigned long long far_pulled;     // records moved far -> near
   unsigned long long near_carried, far_appended, scanned;
@@ -121,18 +122,21 @@ struct Ctx {
   int32_t carry_cap;
   // far tier: per query NBK key buckets, bucket b holding the carried entries
   // with key in [base + b*bw, base + (b+1)*bw) (the last one also everything
-  // beyond) as a list of chunks of FCH entries.  Buckets below far_next are
-  // "pulled": their keys belong to the near tier.
+  // beyond) as a list of chunks.  A chunk is 2^cls blocks of FCH records
+  // (cls grows by one along a bucket's list up to FCLS_MAX); a head or next
+  // pointer is (first block << 4) | cls, and the chunk header lives at
+  // fhd[first block].  Buckets below far_next are "pulled": their keys belong
+  // to the near tier.
   int32_t* far_head;  // [B * NBK]
   unsigned long long* far_occ;  // [B * NBW] bit b: bucket b holds a chunk
   int32_t* far_next;  // [B] first bucket not yet pulled
   int32_t* far_cnt;   // [B] entries in the query's far tier
   float* far_base;    // [B] smallest possible key of the query
   float* far_bw;      // [B] bucket width
-  int2* fhd;          // chunk header {count, next}
-  int4* frec;         // chunk records
-  int32_t far_cap;    // chunks
-  int32_t* far_free;  // free stack of chunks (pushed by pull, popped by expand)
+  int2* fhd;          // [far_cap] chunk header {count, next pointer}, at the chunk's first block
+  int4* frec;         // [far_cap * FCH] records
+  int32_t far_cap;    // blocks
+  int2* dl;           // [far_cap] drain list: segments {block, records} for the next expand
   float far_floor;    // smallest near-tier reach (key units)
   long long* sfr[2];
   int32_t sf_cap;
@@ -559,9 +563,13 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
   const int r = ct->round_e;
   const int cur = r & 1, nxt = cur ^ 1;
   const int ncarry = min(ct->carry_n[cur], c.carry_cap);
+  // input: near-tier carry records, then the far segments pull queued (64
+  // lanes each, some empty), then the labels merge inserted last round
+  const long long nd64 = (long long)min(ct->drain_n, c.far_cap) * FCH;
+  const long long nold = (long long)ncarry + nd64;
   const long long lstart = ct->lstart[cur];
   const long long lend = min((long long)ct->label_ctr, c.label_cap);
-  const long long fn = (long long)ncarry + (lend - lstart);
+  const long long fn = nold + (lend - lstart);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ct->lstart[nxt] = lend;
     ct->round_m = r;
@@ -583,17 +591,25 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
     if (i < fn) {
       // carried entries bring their query, key and pruning delay along, so a
       // label that stays outside the window costs one 16-byte read per round
-      bool fresh = i >= ncarry;
+      bool fresh = i >= nold;
       if (!fresh) {
-        int4 cr = c.carry[cur][i];
+        int4 cr = make_int4(-1, 0, 0, 0);
+        if (i < ncarry) {
+          cr = c.carry[cur][i];
+        } else {
+          const long long j = i - ncarry;
+          const int2 sg = c.dl[j / FCH];  // {block, records}
+          const int l = (int)(j % FCH);
+          if (l < sg.y) cr = c.frec[(long long)sg.x * FCH + l];
+        }
         id = cr.x;
         q = cr.y;
         key = __int_as_float(cr.z);
         rd = __int_as_float(cr.w);
       } else {
-        id = (int)(lstart + (i - ncarry));
+        id = (int)(lstart + (i - nold));
       }
-      if (!c.dead[id]) {
+      if (id >= 0 && !c.dead[id]) {
         bool live = true;
         if (fresh) {
           int4 L = c.labs[id];
@@ -658,29 +674,35 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
     }
     if (far) {
       atomicAdd(&c.far_cnt[q], 1);
-      // lock-free append to the query's far chunk list (as the candidate lists)
+      // lock-free append to the bucket's chunk list (as the candidate lists);
+      // a full head chunk is followed by one twice its size
       int* headp = &c.far_head[(long long)q * NBK + fb];
-      int h = __hip_atomic_load(headp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int hv = __hip_atomic_load(headp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       bool done = false;
-      if (h >= 0) {
-        int slot = atomicAdd(&c.fhd[h].x, 1);
-        if (slot < FCH) {
-          c.frec[(long long)h * FCH + slot] = crec4;
+      int cls = 0;
+      if (hv >= 0) {
+        const int blk = hv >> 4;
+        cls = (hv & 15) + 1;
+        const int slot = atomicAdd(&c.fhd[blk].x, 1);
+        if (slot < (FCH << (hv & 15))) {
+          c.frec[(long long)blk * FCH + slot] = crec4;
           done = true;
         }
       }
       if (!done) {
-        // chunks come from a bump allocator reset with each batch
-        const int nc = atomicAdd(&ct->far_chunks, 1);
-        if (nc >= c.far_cap) {
+        cls = min(cls, FCLS_MAX);
+        // blocks come from a bump allocator reset with each batch
+        const int blk = atomicAdd(&ct->far_chunks, 1 << cls);
+        if (blk < 0 || blk > c.far_cap - (1 << cls)) {
           atomicOr(&ct->overflow, OVF_FAR);
         } else {
-          c.frec[(long long)nc * FCH] = crec4;
-          int expected = h;
+          c.frec[(long long)blk * FCH] = crec4;
+          const int mine = (blk << 4) | cls;
+          int expected = hv;
           for (;;) {
-            c.fhd[nc] = make_int2(1, expected);
+            c.fhd[blk] = make_int2(1, expected);
             __threadfence();
-            int prev = atomicCAS(headp, expected, nc);
+            int prev = atomicCAS(headp, expected, mine);
             if (prev == expected) break;
             expected = prev;
           }
@@ -814,6 +836,7 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
     ct->touched_n[nxt] = 0;
     ct->cand_ctr[nxt] = 0;
     ct->carry_n[cur] = 0;
+    ct->drain_n = 0;  // expand of this round has scanned the segments
     ct->round_e = r + 1;  // expand of this round has finished reading it
   }
   for (long long q = gtid; q < c.B; q += (long long)gridDim.x * blockDim.x)
@@ -1014,12 +1037,15 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
 // ---------------------------------------------------------------------------
 // Pull: between merge of round r and expand of round r+1.  For each query the
 // buckets whose key range comes within reach of the front (front + reach) are
-// moved whole into the near tier; if the near tier is empty, the next
+// handed whole to the next expand; if the near tier is empty, the next
 // non-empty bucket is.  A far entry is therefore moved at most once, and
-// entries whose keys the upper bound prunes are never read again.  Dead and
-// pruned entries are dropped on the way; chunks are reclaimed with the batch.
-// An occupancy bitmap (one bit per bucket) lets a wave find the non-empty
-// buckets of a key range 4,096 buckets per load step.  One wave per query.
+// buckets whose keys the upper bound prunes are dropped without being read.
+// The pull itself only chases chunk headers (chunks double in size along a
+// bucket, so even a dense bucket is a short list) and queues each 64-record
+// block as a drain segment; the records are read by the next expand across
+// the whole grid, where dead and pruned ones are dropped.  An occupancy
+// bitmap (one bit per bucket) finds the non-empty buckets of a key range
+// 4,096 buckets per load step.  One wave per query.
 template <int PRUNE>
 __global__ void __launch_bounds__(256) k_pull(Ctx c) {
   Counters* ct = c.ct;
@@ -1074,12 +1100,14 @@ __global__ void __launch_bounds__(256) k_pull(Ctx c) {
     if (target < 0) return;
   }
   if (target < nb) return;
-  long long moved = 0, dropped = 0;
-  float kmin = FINF;
+  // walk the chunk headers of the occupied buckets of [nb, target] and queue
+  // their blocks as drain segments: the next expand scans them like carried
+  // entries, spread over the whole grid.  Only headers are chased here.
+  long long queued = 0;
   const int wlast = target >> 6;
   for (int w0 = nb >> 6; w0 <= wlast; w0 += WAVE) {
-    // the occupied buckets of [nb, target], 64 words at a time; their bits
-    // are cleared here (expand appends only to buckets >= the new far_next)
+    // 64 occupancy words at a time; their bits are cleared here (expand
+    // appends only to buckets >= the new far_next)
     const int w = w0 + lane;
     unsigned long long bits = 0;
     if (w <= wlast) {
@@ -1101,57 +1129,35 @@ __global__ void __launch_bounds__(256) k_pull(Ctx c) {
       while (wb) {
         const int bit = __ffsll((long long)wb) - 1;
         wb &= wb - 1;
-        int h = __shfl(myhead, bit, WAVE);
+        int hv = __shfl(myhead, bit, WAVE);
         int guard = 0;
-        int2 hd = (h >= 0) ? c.fhd[h] : make_int2(0, -1);
-        while (h >= 0 && guard++ < c.far_cap) {
-          // the next header is requested before this chunk is processed
-          const int2 hd_next = (hd.y >= 0) ? c.fhd[hd.y] : make_int2(0, -1);
-          const int cnt = min(hd.x, FCH);
-          const int4 rec = (lane < cnt) ? c.frec[(long long)h * FCH + lane] : make_int4(-1, 0, 0, 0);
-          bool live = lane < cnt && !c.dead[rec.x];
-          const float key = __int_as_float(rec.z), rd = __int_as_float(rec.w);
-          if (live && PRUNE != 0 && lex_ge(key, rd, best.x, best.y)) live = false;
-          dropped += (lane < cnt && !live) ? 1 : 0;
-          int slot = wave_alloc32(&ct->carry_n[nc], live);
-          if (live) {
-            moved++;
-            kmin = fminf(kmin, key);
-            if (slot < c.carry_cap) c.carry[nc][slot] = rec;
-            else atomicOr(&ct->overflow, OVF_CARRY);
+        while (hv >= 0 && guard++ < c.far_cap) {
+          const int blk = hv >> 4;
+          const int2 hd = c.fhd[blk];
+          const int cnt = min(hd.x, FCH << (hv & 15));
+          const int nseg = (cnt + FCH - 1) / FCH;
+          int base = 0;
+          if (lane == 0) base = atomicAdd(&ct->drain_n, nseg);
+          base = __shfl(base, 0, WAVE);
+          for (int k = lane; k < nseg; k += WAVE) {
+            if (base + k < c.far_cap) c.dl[base + k] = make_int2(blk + k, min(FCH, cnt - k * FCH));
+            else atomicOr(&ct->overflow, OVF_FAR);
           }
-          h = hd.y;
-          hd = hd_next;
+          queued += cnt;
+          hv = hd.y;
         }
       }
     }
   }
-  // this wave is the only writer of the query's front during pull
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) kmin = fminf(kmin, __shfl_xor(kmin, o, WAVE));
-  if (lane == 0 && kmin < FINF && __float_as_uint(kmin) < fmin[q]) atomicMin(&fmin[q], __float_as_uint(kmin));
-  {
-    long long mq = moved + dropped;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mq += __shfl_xor(mq, o, WAVE);
-    long long mv = moved;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mv += __shfl_xor(mv, o, WAVE);
-    if (lane == 0) {
-      c.far_next[q] = target + 1;
-      c.far_cnt[q] -= (int)mq;
+  if (lane == 0) {
+    c.far_next[q] = target + 1;
+    if (queued) {
+      c.far_cnt[q] -= (int)queued;
       c.q_last[q] = rn;
-      if (c.stats_on) c.q_near[q] += (int)mv;
+      if (c.stats_on) c.q_near[q] += (int)queued;
+      atomicAdd((unsigned long long*)&ct->far_live, (unsigned long long)(-queued));
+      atomicAdd(&ct->far_pulled, (unsigned long long)queued);
     }
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    moved += __shfl_xor(moved, o, WAVE);
-    dropped += __shfl_xor(dropped, o, WAVE);
-  }
-  if (lane == 0 && (moved + dropped)) {
-    atomicAdd((unsigned long long*)&ct->far_live, (unsigned long long)(-(moved + dropped)));
-    atomicAdd(&ct->far_pulled, (unsigned long long)moved);
   }
 }
 
