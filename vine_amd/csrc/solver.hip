@@ -1120,7 +1120,7 @@ int64_t vine_solver_pareto(const vine_solver* s, int32_t q, int32_t v, float* ou
     set_err("vine_solver_pareto: %s", hipGetErrorString(e));
     return VINE_E_HIP;
   }
-  // the stored set may hold tombstones (+inf, +inf) left by evictions
+  // the stored front holds exactly its live entries (sorted by d)
   int64_t slots = ps.y & ((1 << 26) - 1);
   std::vector<float2> tmp(slots);
   if (slots > 0) {

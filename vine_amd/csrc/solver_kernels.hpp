@@ -1,22 +1,23 @@
 // Device side of the exact CSP solver for gfx950 (CDNA4, wave64).
 //
-// A round is two kernels:
+// A round is two main kernels (plus the far-tier pull and the window policy):
 //   expand  (k_expand)  -- counterpart of ck_expand_bucket_with_filter
 //                          (algorithm/bucket.cuh:15-195) and ck_recheck_bucket
 //                          (:217-245): every live frontier label inside the
 //                          window is extended along its out-arcs; a candidate is
 //                          filtered by the constraint, by the pruning bounds and
 //                          by a read-only dominance check against the target
-//                          vertex's Pareto set, pre-merged in an LDS hash, and
-//                          pushed onto a per-(query, vertex) candidate list.
-//                          Labels outside the window are carried.
+//                          vertex's Pareto front, and pushed onto a
+//                          per-(query, vertex) candidate list.  Labels outside
+//                          the window are carried.
 //   merge   (k_merge)   -- counterpart of second_level_pruning
-//                          (algorithm/dominance.cuh:15-153): one lane owns each
-//                          touched (query, vertex) and folds its candidates into
-//                          the Pareto set (drop dominated, evict dominated
-//                          entries, append survivors to the label log, which is
-//                          the next frontier).  No locks: the two kernels never
-//                          run concurrently and each set has a single writer.
+//                          (algorithm/dominance.cuh:15-153): a group of lanes
+//                          owns each touched (query, vertex) and folds its
+//                          candidates into the sorted Pareto front (drop
+//                          dominated, evict dominated entries, append survivors
+//                          to the label log, which is the next frontier).  No
+//                          locks: the two kernels never run concurrently and
+//                          each front has a single writer.
 // The thrust scan + D2H size + ck_fill_buckets sequence of the reference
 // (algorithm/cspp.cuh:245-292, bucket.cuh:197-215) is replaced by wave-level
 // ballot/mbcnt compaction into the label log; the frontier size never leaves
@@ -291,63 +292,30 @@ __device__ __forceinline__ bool lex_ge(float a, float b, float c, float d) {
   // (a, b) >= (c, d) lexicographically
   return a > c || (a == c && b >= d);
 }
-
-// Is `cand` dominated by one of the cnt entries at ent?  Entries are read
-// SCAN_CHUNK at a time with all loads issued before any compare, so a set of S
-// entries costs about S / SCAN_CHUNK memory round trips per lane.
-constexpr int SCAN_CHUNK = 8;
-
+// Pareto fronts in the pool are sorted by d ascending (so delay strictly
+// descends; label_traits::pack gives the two criteria, and the store needs
+// dominate(a, b) to be their componentwise <=).  Is `cand` dominated by one of
+// the cnt entries at ent?  Only the last entry with d <= cand.d can dominate
+// it: usually the set's last entry, else found by binary search.
 template <typename Label_t>
-__device__ __forceinline__ bool set_dominates(const float2* ent, int cnt, int vid, const Label_t& cand) {
+__device__ __forceinline__ bool front_dominates(const float2* ent, int cnt, int vid, const Label_t& cand) {
   using LT = label_traits<Label_t>;
-  for (int j = 0; j < cnt; j += SCAN_CHUNK) {
-    float2 e[SCAN_CHUNK];
-#pragma unroll
-    for (int k = 0; k < SCAN_CHUNK; k++) e[k] = (j + k < cnt) ? ent[j + k] : make_float2(FINF, FINF);
-    bool dom = false;
-#pragma unroll
-    for (int k = 0; k < SCAN_CHUNK; k++)
-      dom |= (j + k < cnt) && Label_t::dominate(LT::unpack(vid, e[k]), cand);
-    if (dom) return true;
+  if (cnt <= 0) return false;
+  const float cx = LT::pack(cand).x;
+  float2 e = ent[cnt - 1];
+  if (!(e.x <= cx)) {
+    int lo = 0, hi = cnt - 1;  // entries >= hi have d > cand.d
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (ent[mid].x <= cx) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo == 0) return false;
+    e = ent[lo - 1];
   }
-  return false;
+  return Label_t::dominate(LT::unpack(vid, e), cand);
 }
 
-// Remove (and mark dead) every entry that `cand` dominates, compacting the
-// survivors in place; returns the new count.
-template <typename Label_t>
-__device__ __forceinline__ int set_evict(float2* ent, int32_t* ids, int cnt, int vid, const Label_t& cand,
-                                         uint8_t* dead) {
-  using LT = label_traits<Label_t>;
-  int w = 0;
-  for (int j = 0; j < cnt; j += SCAN_CHUNK) {
-    float2 e[SCAN_CHUNK];
-    int32_t id[SCAN_CHUNK];
-#pragma unroll
-    for (int k = 0; k < SCAN_CHUNK; k++) {
-      e[k] = (j + k < cnt) ? ent[j + k] : make_float2(0.f, 0.f);
-      id[k] = (j + k < cnt) ? ids[j + k] : -1;
-    }
-#pragma unroll
-    for (int k = 0; k < SCAN_CHUNK; k++) {
-      if (j + k >= cnt) break;
-      if (Label_t::dominate(cand, LT::unpack(vid, e[k]))) {
-        dead[id[k]] = 1;
-      } else {
-        if (w != j + k) {
-          ent[w] = e[k];
-          ids[w] = id[k];
-        }
-        w++;
-      }
-    }
-  }
-  return w;
-}
-
-// ---------------------------------------------------------------------------
-// Batch initialisation: one source label per query slot (the ck_init of
-// algorithm/cspp.cuh:63-67), answers reset.
 template <int PRUNE>
 __global__ void k_init_batch(Ctx c, int nq) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -752,12 +720,11 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
           }
         }
         if (ok) {
-          // read-only dominance check against small Pareto sets (no writer runs
-          // now; tombstones dominate nothing).  Larger sets are left to the
-          // cooperative check in merge.
+          // read-only dominance check against the vertex's front (no writer
+          // runs now); candidates of this round are checked against each
+          // other in merge
           int2 ps = c.pset[qv];
-          int cnt = pset_cnt(ps);
-          if (cnt <= SCAN_CHUNK && set_dominates<Label_t>(c.pool_cd + (uint32_t)ps.x, cnt, son.vid, son)) ok = false;
+          if (front_dominates<Label_t>(c.pool_cd + (uint32_t)ps.x, pset_cnt(ps), son.vid, son)) ok = false;
         }
       }
       // lock-free append to the (query, vertex) chunk list: claim a slot in
@@ -807,19 +774,18 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
 }
 
 // ---------------------------------------------------------------------------
-// Merge: fold each touched (query, vertex)'s candidates into its Pareto set.
-// A group of MG lanes owns one (query, vertex): lane g holds set entries
-// g, g+MG, g+2MG, ...; for each candidate (broadcast from the lane that loaded
-// its chunk record) the group decides "dominated" by ballot, evicts the entries
-// the candidate dominates by overwriting them with a tombstone (+inf, +inf),
-// which nothing dominates and which later insertions reuse, and appends the
-// survivor.  Compaction happens only when the set outgrows its storage.
-// Loops are wave-uniform (ballot over the wave), predicates group-uniform, so
-// label and pool allocations aggregate over the whole wave.
-constexpr int MG = 16;     // lanes per (query, vertex) group
-constexpr int MCACHE = 4;  // set entries per lane kept in registers between scan and evict
+// Merge: fold each touched (query, vertex)'s candidates into its Pareto front,
+// kept sorted by d ascending (delay strictly descending).  A group of MG lanes
+// owns one (query, vertex); for each candidate (broadcast from the lane that
+// loaded its chunk record) the group locates its position (the last entry
+// first, then MG entries per step backwards), decides "dominated" from the
+// entry before it, counts the entries it dominates (a contiguous run after
+// it) and replaces that run by the candidate, shifting the tail MG entries a
+// step.  The front grows into storage of twice its size when full.
+// Candidate loops are wave-uniform, predicates group-uniform, so label and
+// pool allocations aggregate over the whole wave.
+constexpr int MG = 16;  // lanes per (query, vertex) group
 
-__device__ __forceinline__ bool is_tomb(float2 e) { return e.x == FINF; }
 
 template <typename Label_t, int PRUNE>
 __global__ void __launch_bounds__(256) k_merge(Ctx c) {
@@ -849,7 +815,6 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
   constexpr int GPW = WAVE / MG;  // groups per wave
   const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
   int n_ins = 0, scanned = 0, steps_max = 0;
-  const float2 TOMB = make_float2(FINF, FINF);
 
   for (long long base = ((long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * GPW;
        base < tn; base += nwaves * GPW) {
@@ -895,110 +860,118 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
         const float cdl = __int_as_float(__shfl(myrec.y, gbase + k, WAVE));
         const int par = __shfl(myrec.z, gbase + k, WAVE);
         const Label_t cl = LT::unpack(v, make_float2(cd, cdl));
-        // scan my slice: dominated?  first free slot?
-        float2 cache[MCACHE];
-        bool dom = false;
-        int free_slot = INT32_MAX;
-        if (act) {
-#pragma unroll
-          for (int t = 0; t < MCACHE; t++) {
-            int j = gl + t * MG;
-            cache[t] = (j < cnt) ? c.pool_cd[off + j] : TOMB;
-          }
-#pragma unroll
-          for (int t = 0; t < MCACHE; t++) {
-            int j = gl + t * MG;
-            if (j < cnt) {
-              if (is_tomb(cache[t])) free_slot = min(free_slot, j);
-              else dom |= Label_t::dominate(LT::unpack(v, cache[t]), cl);
+        // The set is a Pareto front sorted by d ascending, so delay strictly
+        // descends.  p = entries with d <= cd; the candidate is dominated iff
+        // entry p-1 has delay <= cdl.  Children reach a vertex in nearly
+        // ascending d (the window follows d + lbl, and lbl is fixed per
+        // vertex), so the last entry usually settles it.
+        int p = 0;
+        float2 prev = make_float2(-1.f, FINF);
+        if (act && cnt > 0) {
+          const float2 last = c.pool_cd[off + cnt - 1];
+          scanned++;
+          if (last.x <= cd) {
+            p = cnt;
+            prev = last;
+          } else {
+            // group scan backwards, MG entries per step (sorted: the entries
+            // with d <= cd are a prefix of each block)
+            int hi = cnt - 1;
+            for (;;) {
+              const int lo = max(0, hi - MG);
+              const int j = lo + gl;
+              const bool in = j < hi;
+              const float2 e = in ? c.pool_cd[off + j] : make_float2(FINF, FINF);
+              const int np = __popcll(__ballot(in && e.x <= cd) & gmask);
+              scanned++;
+              if (np > 0 || lo == 0) {
+                p = lo + np;
+                break;
+              }
+              hi = lo;
             }
+            if (p > 0) prev = c.pool_cd[off + p - 1];
           }
-          for (int j = gl + MCACHE * MG; j < cnt; j += MG) {
-            float2 e = c.pool_cd[off + j];
-            if (is_tomb(e)) free_slot = min(free_slot, j);
-            else dom |= Label_t::dominate(LT::unpack(v, e), cl);
-          }
-          scanned += (cnt + MG - 1) / MG;
         }
-        const bool dominated = (__ballot(dom) & gmask) != 0ull;
-        const bool acc = act && !dominated;
+        const bool acc = act && !(p > 0 && Label_t::dominate(LT::unpack(v, prev), cl));
+        // entries the candidate dominates: from `start` (entry p-1 when it has
+        // the same d and a larger delay) those with delay >= cdl, a prefix
+        int start = p, kdom = 0;
         if (acc) {
-          // evict what the candidate dominates; evicted slots become free
-#pragma unroll
-          for (int t = 0; t < MCACHE; t++) {
-            int j = gl + t * MG;
-            if (j < cnt && !is_tomb(cache[t]) && Label_t::dominate(cl, LT::unpack(v, cache[t]))) {
-              c.dead[c.pool_id[off + j]] = 1;
-              c.pool_cd[off + j] = TOMB;
-              free_slot = min(free_slot, j);
-            }
-          }
-          for (int j = gl + MCACHE * MG; j < cnt; j += MG) {
-            float2 e = c.pool_cd[off + j];
-            if (!is_tomb(e) && Label_t::dominate(cl, LT::unpack(v, e))) {
-              c.dead[c.pool_id[off + j]] = 1;
-              c.pool_cd[off + j] = TOMB;
-              free_slot = min(free_slot, j);
-            }
+          if (p > 0 && prev.x == cd) start = p - 1;
+          for (int lo = start; lo < cnt; lo += MG) {
+            const int j = lo + gl;
+            const bool in = j < cnt;
+            const float2 e = in ? c.pool_cd[off + j] : make_float2(FINF, -FINF);
+            const bool dm = in && Label_t::dominate(cl, LT::unpack(v, e));
+            if (dm) c.dead[c.pool_id[off + j]] = 1;
+            const int nd = __popcll(__ballot(dm) & gmask);
+            kdom += nd;
+            scanned++;
+            if (nd < MG) break;
           }
         }
-        // group minimum of the free slots (cnt means: append)
-        int slot = acc ? min(free_slot, cnt) : INT32_MAX;
-#pragma unroll
-        for (int o = MG / 2; o > 0; o >>= 1) slot = min(slot, __shfl_xor(slot, o, WAVE));
-        const bool grow = acc && slot == cnt && cnt == cap;
-        // growth: new storage of twice the live size, compacted (leader allocates)
-        int live = 0;
-        if (grow) {
-          for (int j = gl; j < cnt; j += MG) live += is_tomb(c.pool_cd[off + j]) ? 0 : 1;
-        }
-        int live_incl = live;
-#pragma unroll
-        for (int o = 1; o < MG; o <<= 1) {
-          int tt = __shfl_up(live_incl, o, MG);
-          if (gl >= o) live_incl += tt;
-        }
-        const int live_tot = __shfl(live_incl, gbase + MG - 1, WAVE);
+        const int newcnt = cnt - kdom + 1;
+        const bool grow = acc && newcnt > cap;
         int newcap = 0;
         if (grow) {
           newcap = INIT_CAP;
-          while (newcap < 2 * (live_tot + 1)) newcap <<= 1;
+          while (newcap < 2 * newcnt) newcap <<= 1;
         }
         long long poff = wave_alloc64_n(&ct->pool_ctr, (grow && gl == 0) ? newcap : 0);
         poff = __shfl(poff, gbase, WAVE);
         long long lid = wave_alloc64(&ct->label_ctr, acc && gl == 0);
         lid = __shfl(lid, gbase, WAVE);
         bool fits = acc;
-        if (grow) {
-          if (poff < 0 || (unsigned long long)(poff + newcap) > c.pool_cap || poff + newcap > 0xFFFFFFFFll) {
-            if (gl == 0) atomicOr(&ct->overflow, OVF_POOL);
-            fits = false;
-          } else {
-            // copy live entries to the new storage in group order
-            int w = live_incl - live;
-            for (int j = gl; j < cnt; j += MG) {
-              float2 e = c.pool_cd[off + j];
-              if (!is_tomb(e)) {
-                c.pool_cd[poff + w] = e;
-                c.pool_id[poff + w] = c.pool_id[off + j];
-                w++;
-              }
-            }
-            off = (uint32_t)poff;
-            cap = newcap;
-            cnt = live_tot;
-            slot = live_tot;
-          }
+        if (grow && (poff < 0 || (unsigned long long)(poff + newcap) > c.pool_cap || poff + newcap > 0xFFFFFFFFll)) {
+          if (gl == 0) atomicOr(&ct->overflow, OVF_POOL);
+          fits = false;
         }
         if (fits && lid >= c.label_cap) {
           if (gl == 0) atomicOr(&ct->overflow, OVF_LABEL);
           fits = false;
         }
         if (fits) {
+          if (grow) {
+            // new storage: [0, start), the candidate, [start + kdom, cnt)
+            for (int j = gl; j < cnt; j += MG) {
+              if (j < start || j >= start + kdom) {
+                const int dj = j < start ? j : j - kdom + 1;
+                c.pool_cd[poff + dj] = c.pool_cd[off + j];
+                c.pool_id[poff + dj] = c.pool_id[off + j];
+              }
+            }
+            off = (uint32_t)poff;
+            cap = newcap;
+          } else if (kdom == 0 && start < cnt) {
+            // open slot `start`: shift [start, cnt) up by one, last block
+            // first (each block is read whole before it is written)
+            for (int hi = cnt; hi > start; hi -= MG) {
+              const int lo = max(start, hi - MG);
+              const int j = lo + gl;
+              if (j < hi) {
+                const float2 e = c.pool_cd[off + j];
+                const int32_t id = c.pool_id[off + j];
+                c.pool_cd[off + j + 1] = e;
+                c.pool_id[off + j + 1] = id;
+              }
+            }
+          } else if (kdom > 1) {
+            // close the gap: shift [start + kdom, cnt) down by kdom - 1
+            for (int lo = start + kdom; lo < cnt; lo += MG) {
+              const int j = lo + gl;
+              if (j < cnt) {
+                const float2 e = c.pool_cd[off + j];
+                const int32_t id = c.pool_id[off + j];
+                c.pool_cd[off + j - kdom + 1] = e;
+                c.pool_id[off + j - kdom + 1] = id;
+              }
+            }
+          }
           if (gl == 0) {
             n_ins++;
-            c.pool_cd[off + slot] = make_float2(cd, cdl);
-            c.pool_id[off + slot] = (int)lid;
+            c.pool_cd[off + start] = make_float2(cd, cdl);
+            c.pool_id[off + start] = (int)lid;
             float key = LT::key(cl) + lb;
             c.labs[lid] = make_int4(v, __float_as_int(cd), __float_as_int(cdl), par);
             c.lmeta[lid] = make_int2(q, __float_as_int(key));
@@ -1014,7 +987,7 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
             if (!(PRUNE != 0 && v == tq) && __float_as_uint(key) < load_agent_u32(&fmin_nxt[q]))
               atomicMin(&fmin_nxt[q], __float_as_uint(key));
           }
-          if (slot == cnt) cnt++;
+          cnt = newcnt;
         }
         if (c.stats_on) {
           wave_add_keyed(c.q_ins, q, fits && gl == 0);
