@@ -93,7 +93,7 @@ typedef struct {
 
 typedef struct {
   int64_t rounds;           /* CSP rounds, summed over batches */
-  int64_t sssp_rounds;      /* bound-computation rounds (VINE_PRUNE_BOUNDS) */
+  int64_t sssp_rounds;      /* bound-search rounds of the longest query, summed over batches (VINE_PRUNE_BOUNDS) */
   int64_t expanded;         /* labels expanded (X) */
   int64_t relaxed;          /* arcs relaxed (E) */
   int64_t candidates;       /* candidates that survived the read-only dominance check */

@@ -100,7 +100,8 @@ struct vine_solver {
   float* d_lbl = nullptr;
   unsigned long long* d_lbd = nullptr;
   uint32_t* d_inq = nullptr;
-  uint32_t* d_qdmin = nullptr;
+  int32_t* d_wq = nullptr;
+  int32_t* d_qorder = nullptr;
   // pools
   float2* d_pool_cd = nullptr;
   int32_t* d_pool_id = nullptr;
@@ -120,7 +121,6 @@ struct vine_solver {
   int2* d_fhd = nullptr;
   int2* d_dl = nullptr;
   int4* d_frec = nullptr;
-  long long* d_sfr[2] = {nullptr, nullptr};
   unsigned long long* d_ts = nullptr;
   Counters* d_ct = nullptr;
   Counters* h_ct = nullptr;  // pinned
@@ -372,7 +372,6 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   long long pool_cap = o.pool_capacity ? o.pool_capacity : std::min<long long>(2 * lab_cap + 4 * B, 0xFFFFFFFFll);
   long long cand_cap = o.cand_capacity ? o.cand_capacity : std::min<long long>(lab_cap / 2 + 1024, INT32_MAX - 1);
   long long carry_cap = o.frontier_capacity ? o.frontier_capacity : std::min<long long>(lab_cap / 2 + 1024, INT32_MAX - 1);
-  long long sf_cap = std::min<long long>(std::max<long long>(BN, 1024), std::min<long long>(lab_cap / 4 + 1024, INT32_MAX - 1));
   if (pool_cap < 4 * B || pool_cap > 0xFFFFFFFFll || cand_cap < 64 || cand_cap > INT32_MAX - 1 ||
       carry_cap < 64 || carry_cap > INT32_MAX - 1) {
     set_err("vine_solver_create: capacities out of range");
@@ -404,9 +403,8 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
     ALLOC(s->d_lbl, sizeof(float) * BN);
     ALLOC(s->d_lbd, sizeof(unsigned long long) * BN);
     ALLOC(s->d_inq, sizeof(uint32_t) * BN);
-    ALLOC(s->d_qdmin, sizeof(uint32_t) * 3 * B);
-    ALLOC(s->d_sfr[0], sizeof(long long) * sf_cap);
-    ALLOC(s->d_sfr[1], sizeof(long long) * sf_cap);
+    ALLOC(s->d_wq, sizeof(int32_t) * 2 * BN);
+    ALLOC(s->d_qorder, sizeof(int32_t) * B);
   }
   ALLOC(s->d_pool_cd, sizeof(float2) * pool_cap);
   ALLOC(s->d_pool_id, sizeof(int32_t) * pool_cap);
@@ -475,7 +473,8 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   c.lbl = s->d_lbl;
   c.lbd = s->d_lbd;
   c.inq = s->d_inq;
-  c.q_dmin = s->d_qdmin;
+  c.wq = s->d_wq;
+  c.q_order = nullptr;  // set per batch when the slots are sorted
   c.pool_cd = s->d_pool_cd;
   c.pool_id = s->d_pool_id;
   c.pool_cap = (unsigned long long)pool_cap;
@@ -501,9 +500,6 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   c.frec = s->d_frec;
   c.far_cap = (int32_t)far_cap;
   c.far_floor = 0.0625f * g->mean_len;
-  c.sfr[0] = s->d_sfr[0];
-  c.sfr[1] = s->d_sfr[1];
-  c.sf_cap = (int32_t)sf_cap;
   c.ts = s->d_ts;
   c.ct = s->d_ct;
   c.hash_slots = o.hash_slots;
@@ -638,43 +634,23 @@ int grid_for(const vine_solver* s, long long items) {
   return (int)std::max<long long>(4, std::min<long long>(want, s->grid));
 }
 
-vine_status run_sssp(vine_solver* s, int nb, int which, int& sr, vine_stats* st) {
+// Bound searches: one workgroup per query slot runs its searches to the end
+// (dev::k_bounds), so the whole computation is a single launch.
+vine_status run_bounds(vine_solver* s, int nb, vine_stats* st) {
   Ctx& c = s->ctx;
-  vine_status fs = fill_f32(s, (float*)s->d_qdmin, dev::FINF, 3LL * s->B);
-  if (fs != VINE_OK) return fs;
-  hipLaunchKernelGGL(dev::k_sssp_init, dim3((nb + 255) / 256), dim3(256), 0, s->stream, c, nb, which, sr);
-  HIP_TRY(hipGetLastError());
-  float delta = s->opt.bound_delta > 0 ? s->opt.bound_delta
-                                       : 2.0f * (which == 1 ? s->g->mean_len : s->g->mean_delay);
-  const int K = std::max(1, std::min(4, s->opt.rounds_per_sync));
-  long long last = nb;
-  const int r0 = sr;
-  for (;;) {
-    for (int k = 0; k < K; k++) {
-      int gsz = grid_for(s, last * 4);
-      Timer t(s, 2);
-      if (which == 0)
-        hipLaunchKernelGGL(dev::k_sssp<0>, dim3(gsz), dim3(s->block), 0, s->stream, c, sr, delta);
-      else if (which == 2)
-        hipLaunchKernelGGL(dev::k_sssp<2>, dim3(gsz), dim3(s->block), 0, s->stream, c, sr, delta);
-      else
-        hipLaunchKernelGGL(dev::k_sssp<1>, dim3(gsz), dim3(s->block), 0, s->stream, c, sr, delta);
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(t.stop());
-      sr++;
-      st->sssp_rounds++;
-      st->sssp_launches++;
-    }
-    vine_status ps = poll(s);
-    if (ps != VINE_OK) return ps;
-    last = s->h_ct->sf_n[sr % 3];
-    if (last == 0) break;
-    // every round settles at least the closest frontier vertex of each query
-    if ((long long)(sr - r0) > 4LL * s->g->n * 8 + 256) {
-      set_err("bound computation made no progress after %d rounds", sr - r0);
-      return VINE_E_OVERFLOW;
-    }
+  const float dd = s->opt.bound_delta > 0 ? s->opt.bound_delta : 2.0f * s->g->mean_delay;
+  const float dl = s->opt.bound_delta > 0 ? s->opt.bound_delta : 2.0f * s->g->mean_len;
+  c.q_order = nullptr;
+  if (nb <= dev::ORDER_MAX && nb > 1) {
+    c.q_order = s->d_qorder;
+    hipLaunchKernelGGL(dev::k_bound_order, dim3(1), dim3(1024), 0, s->stream, c, nb);
+    HIP_TRY(hipGetLastError());
   }
+  Timer t(s, 2);
+  hipLaunchKernelGGL(dev::k_bounds, dim3(nb), dim3(dev::BOUND_BLOCK), 0, s->stream, c, nb, dd, dl);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(t.stop());
+  st->sssp_launches++;
   return VINE_OK;
 }
 
@@ -713,18 +689,7 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
     if (fs == VINE_OK) fs = fill_f32(s, s->d_lbl, dev::FINF, BN);
     if (fs != VINE_OK) return fs;
     HIP_TRY(hipMemsetAsync(s->d_inq, 0, sizeof(uint32_t) * BN, s->stream));
-    int sr = 0;
-    vine_status ss = VINE_OK;
-    if (c.ellipse) {
-      // forward min delay into lbl, the delay bound search inside
-      // fwd + lbd <= bound, then lbl back to +inf for the len search
-      ss = run_sssp(s, nb, 2, sr, st);
-      if (ss == VINE_OK) ss = run_sssp(s, nb, 0, sr, st);
-      if (ss == VINE_OK) ss = fill_f32(s, s->d_lbl, dev::FINF, BN);
-    } else {
-      ss = run_sssp(s, nb, 0, sr, st);
-    }
-    if (ss == VINE_OK) ss = run_sssp(s, nb, 1, sr, st);
+    vine_status ss = run_bounds(s, nb, st);
     if (ss != VINE_OK) return ss;
   }
   switch (s->opt.prune) {
@@ -824,6 +789,7 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
   st->candidates += (int64_t)s->h_ct->cands;
   st->inserted += (int64_t)s->h_ct->inserted;
   st->sssp_relaxed += (int64_t)s->h_ct->sssp_relaxed;
+  st->sssp_rounds += s->h_ct->sround;
   st->touched += (int64_t)s->h_ct->touched_total;
   st->merge_scanned += (int64_t)s->h_ct->merge_scanned;
   st->merge_chain_max = std::max<int64_t>(st->merge_chain_max, s->h_ct->merge_chain_max);

@@ -58,8 +58,8 @@ struct Counters {
   long long lstart[2];   // first label of the frontier's "new" range, per parity
   int round_e;           // round number read by expand
   int round_m;           // round number read by merge (written by expand)
-  int sf_n[3];           // bound-computation frontier sizes (triple buffered)
-  int sround;            // bound-computation round
+  int sround;            // most bound-search rounds of one query (all searches)
+  int pad0[3];
   unsigned int blocks_done;
   int overflow;
   int range_flag;
@@ -104,7 +104,6 @@ struct Ctx {
                             // (delay, len) distance to t; the len part is the
                             // length of a minimum-delay path
   uint32_t* inq;     // (query, vertex) in the bound-search frontier
-  uint32_t* q_dmin;  // [3][B] smallest frontier distance per query (bound search)
   // pools
   float2* pool_cd;
   int32_t* pool_id;
@@ -139,8 +138,8 @@ struct Ctx {
   int32_t far_cap;    // blocks
   int2* dl;           // [far_cap] drain list: segments {block, records} for the next expand
   float far_floor;    // smallest near-tier reach (key units)
-  long long* sfr[2];
-  int32_t sf_cap;
+  int32_t* wq;       // [B][2][n] bound-search frontier queues
+  int32_t* q_order;  // [B] bound-search dispatch order (null: slot order)
   unsigned long long* ts;  // [TS_CAP] per-round merge-end wall clock
   Counters* ct;
   int32_t hash_slots;
@@ -378,146 +377,221 @@ __global__ void k_init_batch(Ctx c, int nq) {
 }
 
 // ---------------------------------------------------------------------------
-// Lower bounds to t on the transposed graph (VINE_PRUNE_BOUNDS): a batched
-// delta-stepping SSSP.  Each round relaxes, per query, the frontier vertices
-// whose distance is within `delta` of that query's smallest frontier distance
-// and carries the rest, which keeps re-relaxation close to Dijkstra's while a
-// round still covers every query of the batch.  inq marks (query, vertex)
-// pairs present in the frontier (exactly one copy each).
+// Lower bounds to t (VINE_PRUNE_BOUNDS).  One workgroup per query slot runs
+// its searches to completion with its own frontier queues: delta-stepping
+// rounds separated by workgroup barriers, no grid-wide rounds, so a short
+// query's workgroup retires early and a long one is never held up by others.
+// Each round relaxes the frontier vertices whose distance is within `delta`
+// of the round's smallest frontier distance and carries the rest, which keeps
+// re-relaxation close to Dijkstra's.  inq marks (query, vertex) pairs in the
+// frontier (one queue entry each, so a queue never exceeds n entries).
 //   WHICH = 2: forward min delay fwd from s (on the graph itself, into lbl),
 //   limited to radius `bound`.  Run first when c.ellipse is set.
-//   WHICH = 0: delay to t, limited to radius `bound`, or with c.ellipse to
-//   fwd + lbd <= bound.  A label at u has delay >= fwd(u), so outside that
-//   region every label fails the delay test and lbd = +inf prunes it the same.
-//   WHICH = 1: len, restricted to vertices with lbd <= bound: only those can
-//   lie on a feasible completion, so the restricted distance is a valid lower
-//   bound for every label that passes the delay test.
-// Round r reads frontier r&1 and counter sf_n[r%3], appends to (r+1), clears
-// counter (r+2)%3; q_dmin is triple-buffered the same way.
-
-__global__ void k_sssp_init(Ctx c, int nq, int which, int r) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nq) return;
-  // which 2: forward from s (into lbl); 0 and 1: reverse from t
-  long long qt = (long long)q * c.n + (which == 2 ? c.q_src[q] : c.q_dst[q]);
-  if (which) c.lbl[qt] = 0.f;
-  else c.lbd[qt] = pack2(0.f, 0.f);
-  c.inq[qt] = 1;
-  c.q_dmin[(size_t)(r % 3) * c.B + q] = __float_as_uint(0.f);
-  int slot = atomicAdd(&c.ct->sf_n[r % 3], 1);
-  if (slot < c.sf_cap) c.sfr[r & 1][slot] = qt;
-  else atomicOr(&c.ct->overflow, OVF_SSSP);
-}
+//   WHICH = 0: delay to t on the transposed graph, limited to radius `bound`,
+//   or with c.ellipse to fwd + lbd <= bound.  A label at u has delay >=
+//   fwd(u), so outside that region every label fails the delay test and
+//   lbd = +inf prunes it the same.  The packed (delay, len) pair makes the
+//   distance lexicographic; its len part is the length of a min-delay path.
+//   WHICH = 1: len to t, restricted to vertices with lbd <= bound: only those
+//   can lie on a feasible completion, so the restricted distance is a valid
+//   lower bound for every label that passes the delay test.
+constexpr int BOUND_BLOCK = 512;
 
 template <int WHICH>
-__global__ void __launch_bounds__(256) k_sssp(Ctx c, int r, float delta) {
-  const int cur = r & 1, nxt = cur ^ 1;
-  const int n_in = min(c.ct->sf_n[r % 3], c.sf_cap);
-  const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gtid == 0) c.ct->sf_n[(r + 2) % 3] = 0;
-  const uint32_t* dmin_cur = c.q_dmin + (size_t)(r % 3) * c.B;
-  uint32_t* dmin_nxt = c.q_dmin + (size_t)((r + 1) % 3) * c.B;
-  uint32_t* dmin_old = c.q_dmin + (size_t)((r + 2) % 3) * c.B;
-  for (long long q = gtid; q < c.B; q += (long long)gridDim.x * blockDim.x)
-    dmin_old[q] = __float_as_uint(FINF);
+__device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* s_min, int& relaxed, int& rounds) {
+  const int n = c.n;
+  const long long qb = (long long)q * n;
+  int32_t* que[2] = {c.wq + 2 * qb, c.wq + 2 * qb + n};
+  const float bnd = c.q_bound[q];
+  const int tid = threadIdx.x;
   const int lane = lane_id();
-  const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
-  int relaxed = 0;
-  for (long long base = ((long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * WAVE;
-       base < n_in; base += nwaves * WAVE) {
-    long long i = base + lane;
-    int deg = 0, row = 0, q = 0;
-    float du = 0.f, dl2 = 0.f;  // distance (delay for WHICH 0, len for 1); len tie key
-    bool carry = false;
-    long long qv = 0;
-    if (i < n_in) {
-      qv = c.sfr[cur][i];
-      q = (int)(qv / c.n);
-      int u = (int)(qv - (long long)q * c.n);
-      if (WHICH) {
-        du = load_agent(&c.lbl[qv]);
-      } else {
-        unsigned long long pk = __hip_atomic_load(&c.lbd[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        du = hi_of(pk);
-      }
-      if (du > __uint_as_float(dmin_cur[q]) + delta) {
-        carry = true;
-      } else {
-        // leave the frontier first, then read the final distance: a later
-        // decrease by another lane sees inq == 0 and re-queues the vertex
-        c.inq[qv] = 0;
-        __threadfence();
-        if (WHICH) {
-          du = load_agent(&c.lbl[qv]);
-        } else {
-          unsigned long long pk = __hip_atomic_load(&c.lbd[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          du = hi_of(pk);
-          dl2 = lo_of(pk);
-        }
-        const int32_t* rp = (WHICH == 2) ? c.row_ptr : c.rrow_ptr;
-        row = rp[u];
-        deg = rp[u + 1] - row;
-      }
-    }
-    int cslot = wave_alloc32(&c.ct->sf_n[(r + 1) % 3], carry);
-    if (carry) {
-      if (cslot < c.sf_cap) {
-        c.sfr[nxt][cslot] = qv;
-        atomicMin(&dmin_nxt[q], __float_as_uint(du));
-      } else {
-        atomicOr(&c.ct->overflow, OVF_SSSP);
-      }
-    }
-    wave_flat(deg, row, [&](int owner, int pos, bool act) {
-      float odu = __shfl(du, owner, WAVE);
-      float odl = __shfl(dl2, owner, WAVE);
-      int oq = __shfl(q, owner, WAVE);
-      bool push = false;
-      long long wv = 0;
-      float nd = 0.f;
-      if (act) {
-        relaxed++;
-        int4 e = (WHICH == 2) ? c.edges[pos] : c.redges[pos];
-        float bnd = c.q_bound[oq];
-        wv = (long long)oq * c.n + e.x;
-        bool lowered = false;
-        if (WHICH == 2) {
-          nd = odu + __int_as_float(e.z);
-          if (nd <= bnd && nd < c.lbl[wv]) {
-            unsigned old = atomicMin((unsigned*)&c.lbl[wv], __float_as_uint(nd));
-            lowered = __float_as_uint(nd) < old;
-          }
-        } else if (WHICH == 1) {
-          nd = odu + __int_as_float(e.y);
-          if (lbd_of(c, wv) <= bnd && nd < c.lbl[wv]) {
-            unsigned old = atomicMin((unsigned*)&c.lbl[wv], __float_as_uint(nd));
-            lowered = __float_as_uint(nd) < old;
-          }
-        } else {
-          nd = odu + __int_as_float(e.z);
-          const unsigned long long pk = pack2(nd, odl + __int_as_float(e.y));
-          // with the forward search done, only vertices with fwd + lbd <= bound:
-          // every vertex of a feasible s-t path lies there, and so does every
-          // vertex of such a vertex's shortest path to t
-          const float lim = c.ellipse ? bnd - c.lbl[wv] : bnd;
-          if (nd <= lim && pk < c.lbd[wv]) {
-            unsigned long long old = atomicMin(&c.lbd[wv], pk);
-            lowered = pk < old;
-          }
-        }
-        if (lowered) {
-          if (__float_as_uint(nd) < load_agent_u32(&dmin_nxt[oq])) atomicMin(&dmin_nxt[oq], __float_as_uint(nd));
-          push = (atomicExch(&c.inq[wv], 1u) == 0u);
-        }
-      }
-      int slot = wave_alloc32(&c.ct->sf_n[(r + 1) % 3], push);
-      if (push) {
-        if (slot < c.sf_cap) c.sfr[nxt][slot] = wv;
-        else atomicOr(&c.ct->overflow, OVF_SSSP);
-      }
-    });
+  const int32_t* rp = (WHICH == 2) ? c.row_ptr : c.rrow_ptr;
+  const int4* ep = (WHICH == 2) ? c.edges : c.redges;
+  if (tid == 0) {
+    const int root = (WHICH == 2) ? c.q_src[q] : c.q_dst[q];
+    if (WHICH == 0) c.lbd[qb + root] = pack2(0.f, 0.f);
+    else c.lbl[qb + root] = 0.f;
+    c.inq[qb + root] = 1;
+    que[0][0] = root;
+    s_n[0] = 1;
+    s_n[1] = 0;
+    s_min[0] = __float_as_uint(0.f);
+    s_min[1] = __float_as_uint(FINF);
   }
+  __syncthreads();
+  int cur = 0;
+  for (int round = 0;; round++) {
+    const int ncur = s_n[cur];
+    if (ncur == 0) break;
+    // every round settles the smallest frontier vertex for good, so rounds
+    // are bounded by about n; this guard only stops a malformed run
+    if (round > 4 * n + 1024) {
+      if (tid == 0) atomicOr(&c.ct->overflow, OVF_SSSP);
+      break;
+    }
+    rounds = round + 1;
+    const int nxt = cur ^ 1;
+    const float lim_key = __uint_as_float(s_min[cur]) + delta;
+    const int32_t* qin = que[cur];
+    int32_t* qout = que[nxt];
+    for (int base = (tid / WAVE) * WAVE; base < ncur; base += blockDim.x) {
+      const int i = base + lane;
+      int deg = 0, row = 0, u = 0;
+      float du = 0.f, dl2 = 0.f;  // distance (delay for WHICH 0 and 2, len for 1); len tie key
+      bool carry = false;
+      if (i < ncur) {
+        u = qin[i];
+        const long long qv = qb + u;
+        if (WHICH == 0) du = hi_of(__hip_atomic_load(&c.lbd[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        else du = load_agent(&c.lbl[qv]);
+        if (du > lim_key) {
+          carry = true;
+        } else {
+          // leave the frontier first, then read the final distance: a later
+          // decrease by another lane sees inq == 0 and re-queues the vertex.
+          // Only this workgroup touches the query's arrays, so workgroup
+          // ordering is enough (distances are read from L2, where the
+          // atomics land).
+          c.inq[qv] = 0;
+          __threadfence_block();
+          if (WHICH == 0) {
+            const unsigned long long pk = __hip_atomic_load(&c.lbd[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            du = hi_of(pk);
+            dl2 = lo_of(pk);
+          } else {
+            du = load_agent(&c.lbl[qv]);
+          }
+          row = rp[u];
+          deg = rp[u + 1] - row;
+        }
+      }
+      const int cslot = wave_alloc32(&s_n[nxt], carry);
+      if (carry) {
+        if (cslot < n) qout[cslot] = u;
+        else atomicOr(&c.ct->overflow, OVF_SSSP);
+        atomicMin(&s_min[nxt], __float_as_uint(du));
+      }
+      wave_flat(deg, row, [&](int owner, int pos, bool act) {
+        const float odu = __shfl(du, owner, WAVE);
+        const float odl = __shfl(dl2, owner, WAVE);
+        bool push = false;
+        int w = 0;
+        if (act) {
+          relaxed++;
+          const int4 e = ep[pos];
+          w = e.x;
+          const long long wv = qb + w;
+          bool lowered = false;
+          float nd;
+          if (WHICH == 2) {
+            nd = odu + __int_as_float(e.z);
+            if (nd <= bnd && nd < c.lbl[wv]) {
+              const unsigned old = atomicMin((unsigned*)&c.lbl[wv], __float_as_uint(nd));
+              lowered = __float_as_uint(nd) < old;
+            }
+          } else if (WHICH == 1) {
+            nd = odu + __int_as_float(e.y);
+            if (lbd_of(c, wv) <= bnd && nd < c.lbl[wv]) {
+              const unsigned old = atomicMin((unsigned*)&c.lbl[wv], __float_as_uint(nd));
+              lowered = __float_as_uint(nd) < old;
+            }
+          } else {
+            nd = odu + __int_as_float(e.z);
+            const unsigned long long pk = pack2(nd, odl + __int_as_float(e.y));
+            // with the forward search done, only vertices with fwd + lbd <=
+            // bound: every vertex of a feasible s-t path lies there, and so
+            // does every vertex of such a vertex's shortest path to t
+            const float lim = c.ellipse ? bnd - c.lbl[wv] : bnd;
+            if (nd <= lim && pk < c.lbd[wv]) {
+              const unsigned long long old = atomicMin(&c.lbd[wv], pk);
+              lowered = pk < old;
+            }
+          }
+          if (lowered) {
+            if (__float_as_uint(nd) < s_min[nxt]) atomicMin(&s_min[nxt], __float_as_uint(nd));
+            push = (atomicExch(&c.inq[wv], 1u) == 0u);
+          }
+        }
+        const int slot = wave_alloc32(&s_n[nxt], push);
+        if (push) {
+          if (slot < n) qout[slot] = w;
+          else atomicOr(&c.ct->overflow, OVF_SSSP);
+        }
+      });
+    }
+    __syncthreads();
+    if (tid == 0) {
+      s_n[cur] = 0;
+      s_min[cur] = __float_as_uint(FINF);
+    }
+    cur = nxt;
+    __syncthreads();
+  }
+}
+
+// Workgroups are dispatched in blockIdx order, so the slots are taken
+// largest delay bound first (a bound of r searches a ball of radius r): the
+// longest searches start at once instead of in the last wave of workgroups.
+// One workgroup sorts (bound, slot) pairs bitonically in LDS.
+constexpr int ORDER_MAX = 8192;
+
+__global__ void __launch_bounds__(1024) k_bound_order(Ctx c, int nq) {
+  __shared__ float sb[ORDER_MAX];
+  __shared__ int si[ORDER_MAX];
+  int np = 1;
+  while (np < nq) np <<= 1;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    sb[i] = i < nq ? c.q_bound[i] : -FINF;
+    si[i] = i;
+  }
+  __syncthreads();
+  for (int k = 2; k <= np; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        const int l = i ^ j;
+        if (l > i) {
+          // descending by bound (ties by slot) in blocks where (i & k) == 0
+          const bool before = sb[i] > sb[l] || (sb[i] == sb[l] && si[i] < si[l]);
+          if (((i & k) == 0) != before) {
+            const float tb = sb[i];
+            sb[i] = sb[l];
+            sb[l] = tb;
+            const int ti = si[i];
+            si[i] = si[l];
+            si[l] = ti;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < nq; i += blockDim.x) c.q_order[i] = si[i];
+}
+
+__global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float delta_delay, float delta_len) {
+  __shared__ int s_n[2];
+  __shared__ uint32_t s_min[2];
+  if ((int)blockIdx.x >= nq) return;
+  const int q = c.q_order ? c.q_order[blockIdx.x] : (int)blockIdx.x;
+  int relaxed = 0, r0 = 0, r1 = 0, r2 = 0;
+  if (c.ellipse) {
+    wg_search<2>(c, q, delta_delay, s_n, s_min, relaxed, r2);
+    // later reads of fwd must not hit lines cached while it was written
+    __threadfence();
+    __syncthreads();
+  }
+  wg_search<0>(c, q, delta_delay, s_n, s_min, relaxed, r0);
+  __syncthreads();
+  if (c.ellipse) {
+    // lbl held fwd: back to +inf for the len search
+    float* l = c.lbl + (long long)q * c.n;
+    for (int j = threadIdx.x; j < c.n; j += blockDim.x) l[j] = FINF;
+  }
+  __threadfence();
+  __syncthreads();
+  wg_search<1>(c, q, delta_len, s_n, s_min, relaxed, r1);
   wave_add(&c.ct->sssp_relaxed, relaxed);
+  if (threadIdx.x == 0) atomicMax(&c.ct->sround, r0 + r1 + r2);
 }
 
 // ---------------------------------------------------------------------------
