@@ -117,19 +117,37 @@ def main():
 
     # roofline of the dominant kernel (algorithmic bytes / HIP-event duration)
     agg = {k: sum(s[k] for s in stats) for k in stats[0]}
+    # algorithmic bytes per unit of work (each item read or written once):
+    #   expand: 16 B per scanned frontier entry, 24 B per expanded label
+    #           (label + meta), 16 B per relaxed arc, 16 B per candidate pushed
+    #   merge:  16 B per candidate read, 37 B per inserted label (label 16,
+    #           meta 8, front entry 8 + id 4, dead flag 1)
+    #   bounds: 16 B arc + 8 B distance per relaxation
+    #   pull:   8 B chunk header + 8 B drain segment per 64 far records
     kern = {
-        "expand": (agg["ms_expand"], agg["expand_launches"], 20.0 * agg["expanded"] + 12.0 * agg["relaxed"]),
-        "merge": (agg["ms_merge"], agg["merge_launches"], 24.0 * agg["inserted"]),
-        "bound_sssp": (agg["ms_sssp"], agg["sssp_launches"], 20.0 * agg["sssp_relaxed"]),
-        "pull_policy": (agg["ms_pull"], agg["expand_launches"], 16.0 * agg["far_pulled"]),
+        "expand": (agg["ms_expand"], agg["expand_launches"],
+                   16.0 * agg["scanned"] + 24.0 * agg["expanded"] + 16.0 * agg["relaxed"] + 16.0 * agg["candidates"]),
+        "merge": (agg["ms_merge"], agg["merge_launches"], 16.0 * agg["candidates"] + 37.0 * agg["inserted"]),
+        "bounds": (agg["ms_sssp"], agg["sssp_launches"], 24.0 * agg["sssp_relaxed"]),
+        "pull_policy": (agg["ms_pull"], agg["expand_launches"], 16.0 / 64.0 * agg["far_pulled"]),
     }
     dom = max(kern, key=lambda k: kern[k][0])
     ms_k, launches, nbytes = kern[dom]
     avg_s = ms_k / 1000.0 / max(launches, 1)
     per_launch = nbytes / max(launches, 1)
     achieved = per_launch / avg_s if avg_s > 0 else 0.0
+    # HBM bytes per launch of that kernel, from the PMC runs summarised in profiles/
+    traffic, traffic_src = None, None
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path):
+            m = re.match(r"traffic: kernel=(\w+) bytes_per_launch=(\d+)", line.strip())
+            if m and m.group(1) == dom:
+                traffic, traffic_src = float(m.group(2)), os.path.relpath(path, ROOT)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None, "avg_launch_us": avg_s * 1e6,
+                "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
+                "algorithmic_bytes_per_launch": per_launch, "avg_launch_us": avg_s * 1e6,
                 "launches_per_step": launches / args.steps,
                 "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()}}
 
@@ -217,8 +235,8 @@ def main():
                             % (args.plain_delta, args.plain_queries),
                    "parallelism": "single GPU"},
         "latency_us": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-        "work_per_step": {k: agg[k] / args.steps for k in ("rounds", "sssp_rounds", "expanded", "relaxed",
-                                                           "inserted", "sssp_relaxed")},
+        "work_per_step": {k: agg[k] / args.steps for k in ("rounds", "sssp_rounds", "scanned", "expanded", "relaxed",
+                                                           "candidates", "inserted", "sssp_relaxed", "far_pulled")},
         "roofline": roofline,
         "plain": plain,
         "vs_plain": (qps / plain["1.1x"]["qps"]) if plain else None,

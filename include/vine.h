@@ -72,11 +72,11 @@ typedef struct {
   int32_t prune;            /* VINE_PRUNE_* */
   int32_t delta_policy;     /* VINE_DELTA_* */
   float delta_fixed;        /* window for VINE_DELTA_FIXED */
-  int32_t use_graph;        /* capture rounds in hipGraphs (batched serving) */
+  int32_t use_graph;        /* reserved, must be 0 (hipGraph capture of rounds is not built) */
   int32_t rounds_per_sync;  /* rounds between host checks of the frontier size */
   int32_t block_size;       /* threads per workgroup (multiple of 64) */
   int32_t grid_blocks;      /* workgroups per round kernel (0 = auto) */
-  int32_t hash_slots;       /* LDS hash slots per workgroup for candidate pre-merge (0 = off) */
+  int32_t hash_slots;       /* reserved, must be 0 (no LDS candidate table is built) */
   int64_t label_capacity;   /* label log entries (0 = auto from free memory) */
   int64_t cand_capacity;    /* candidate slots per round (0 = auto); half as many list chunks are allocated */
   int64_t pool_capacity;    /* Pareto-set pool entries (0 = auto) */

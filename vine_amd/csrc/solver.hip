@@ -319,6 +319,11 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
     set_err("vine_solver_create: invalid options");
     return VINE_E_ARG;
   }
+  if (o.use_graph || o.hash_slots) {
+    set_err("vine_solver_create: use_graph and hash_slots are reserved (rounds are enqueued "
+            "rounds_per_sync ahead; fronts live in the HBM pool) and must be 0");
+    return VINE_E_ARG;
+  }
   auto* s = new vine_solver();
   s->g = g;
   s->opt = o;
