@@ -64,6 +64,15 @@ enum {
   VINE_DELTA_MLP = 3    /* per-query window predicted by the on-device MLP (vine_solver_set_mlp) */
 };
 
+/* How the label search is scheduled. */
+enum {
+  VINE_ENGINE_AUTO = 0,      /* currently grid-wide rounds (the higher-throughput engine) */
+  VINE_ENGINE_ROUNDS = 1,    /* grid-wide rounds: every query advances one round per kernel sequence */
+  VINE_ENGINE_WORKGROUP = 2  /* one workgroup per query runs its whole search: lower tail latency, lower
+                                throughput (needs VINE_PRUNE_BOUNDS, a fixed or learned window and
+                                out-degrees <= 512) */
+};
+
 typedef struct vine_graph vine_graph;
 typedef struct vine_solver vine_solver;
 
@@ -88,7 +97,8 @@ typedef struct {
   int32_t bound_ball;       /* 0: a forward min-delay search from s first, then the reverse bound
                                searches only inside fwd + lbd <= bound; 1: reverse searches over
                                the whole ball lbd <= bound (no forward search) */
-  int32_t reserved[4];
+  int32_t engine;           /* VINE_ENGINE_* */
+  int32_t reserved[3];
 } vine_options;
 
 typedef struct {

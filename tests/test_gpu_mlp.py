@@ -77,15 +77,17 @@ def test_set_params_rejects_bad_values(gpu):
 
 
 @pytest.mark.gpu
-def test_learned_policy_keeps_answers_exact(gpu):
+@pytest.mark.parametrize("engine", [S.ENGINE_ROUNDS, S.ENGINE_WORKGROUP])
+def test_learned_policy_keeps_answers_exact(gpu, engine):
     """Whatever the network says (random weights here), windows only reorder
     work: answers equal the oracle's."""
     g, q = W.preset("grid64")
     ref = [O.cspp(g, q.src[i], q.dst[i], q.bound[i]) for i in range(len(q))]
+    print("grid64 learned policy engine=%d" % engine, flush=True)
     for seed in (0, 1):
         p = M.init_params(seed)
         with S.Solver(g, max_batch=50, prune=S.PRUNE_BOUNDS, delta_policy=S.DELTA_MLP, delta_fixed=50.0,
-                      record_trace=1) as s:
+                      record_trace=1, engine=engine) as s:
             s.set_mlp(p)
             a = s.solve(q.src, q.dst, q.bound)
             tr = s.trace()

@@ -49,13 +49,19 @@ def test_grid_parity(gpu, grid_case, prune, batch):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("policy,delta", [(S.DELTA_FIXED, 0.0), (S.DELTA_FIXED, 2000.0), (S.DELTA_AR, 0.0)])
-def test_grid_windows(gpu, grid_case, policy, delta):
+@pytest.mark.parametrize("policy,delta,engine", [(S.DELTA_FIXED, 0.0, S.ENGINE_ROUNDS),
+                                                 (S.DELTA_FIXED, 2000.0, S.ENGINE_ROUNDS),
+                                                 (S.DELTA_AR, 0.0, S.ENGINE_AUTO),
+                                                 (S.DELTA_FIXED, 0.0, S.ENGINE_WORKGROUP),
+                                                 (S.DELTA_FIXED, 300.0, S.ENGINE_WORKGROUP),
+                                                 (S.DELTA_FIXED, 2000.0, S.ENGINE_WORKGROUP)])
+def test_grid_windows(gpu, grid_case, policy, delta, engine):
     g, q, ref = grid_case
-    print("grid64 windows policy=%d delta=%g" % (policy, delta), flush=True)
-    with S.Solver(g, max_batch=25, prune=2, delta_policy=policy, delta_fixed=delta) as s:
-        ans = s.solve(q.src, q.dst, q.bound)
-    _compare(g, q, ref, ans)
+    print("grid64 windows policy=%d delta=%g engine=%d" % (policy, delta, engine), flush=True)
+    for batch in (1, 25):
+        with S.Solver(g, max_batch=batch, prune=2, delta_policy=policy, delta_fixed=delta, engine=engine) as s:
+            ans = s.solve(q.src, q.dst, q.bound)
+        _compare(g, q, ref, ans)
 
 
 def _tiny_graphs(count, seed):
@@ -130,8 +136,10 @@ def test_knn_sample_bounds(gpu, bound_ball):
     q = W.hop_queries(g, 30, hop_lo=5, hop_hi=80, f_permille=(1050, 1500), seed=5)
     ref = [O.cspp(g, q.src[i], q.dst[i], q.bound[i], prune_target=2) for i in range(len(q))]
     print("knn20k bounds bound_ball=%d" % bound_ball, flush=True)
-    for pol, dlt in ((S.DELTA_INF, 0.0), (S.DELTA_FIXED, 500.0)):
-        with S.Solver(g, max_batch=16, prune=2, delta_policy=pol, delta_fixed=dlt, bound_ball=bound_ball) as s:
+    for pol, dlt, eng in ((S.DELTA_INF, 0.0, S.ENGINE_AUTO), (S.DELTA_FIXED, 500.0, S.ENGINE_ROUNDS),
+                          (S.DELTA_FIXED, 20.0, S.ENGINE_WORKGROUP), (S.DELTA_FIXED, 500.0, S.ENGINE_WORKGROUP)):
+        with S.Solver(g, max_batch=16, prune=2, delta_policy=pol, delta_fixed=dlt, bound_ball=bound_ball,
+                      engine=eng) as s:
             ans = s.solve(q.src, q.dst, q.bound)
         _compare(g, q, ref, ans)
 
@@ -178,3 +186,18 @@ def test_cpp_facade_matches_oracle(gpu):
         if ref.feasible:
             assert (out[1], out[2]) == (ref.cost, ref.delay)
             check_path(g, np.array(out[3:], np.int64), s, t, out[1], out[2], b)
+
+
+@pytest.mark.gpu
+def test_workgroup_engine_eligibility(gpu, grid_case):
+    """The workgroup engine needs the bounds and a finite window; asking for it
+    otherwise is an error, and AUTO falls back to rounds."""
+    g, q, ref = grid_case
+    for kw in ({"prune": 1, "delta_policy": S.DELTA_FIXED, "delta_fixed": 10.0},
+               {"prune": 2, "delta_policy": S.DELTA_INF},
+               {"prune": 2, "delta_policy": S.DELTA_AR}):
+        with pytest.raises(S.VineError):
+            S.Solver(g, max_batch=4, engine=S.ENGINE_WORKGROUP, **kw)
+    with S.Solver(g, max_batch=4, prune=1, delta_policy=S.DELTA_FIXED, delta_fixed=10.0) as s:
+        ans = s.solve(q.src[:4], q.dst[:4], q.bound[:4])
+    _compare(g, q, ref, ans, idx=range(4))

@@ -2,9 +2,9 @@
 //
 // Replaces the reference's AR(3) frontier-size model (include/AR.hpp,
 // algorithm/cspp.cuh:254-284) as the scheduler of the window width.  Output 0
-// is log2 of the next window width (key units), output 1 is log2 of the near-
-// tier reach in windows.  Inputs are normalised with stored per-feature mean
-// and standard deviation.
+// is log2 of the next window width (key units), output 1 a forecast of
+// log2(1 + expansions) a few rounds ahead (the AR model's forecasting role).
+// Inputs are normalised with stored per-feature mean and standard deviation.
 //
 // Parameter vector (floats, row-major W[out][in]):
 //   W1[32*8] b1[32] W2[32*32] b2[32] W3[2*32] b3[2] xmu[8] xsd[8]
@@ -56,6 +56,30 @@ __device__ __forceinline__ void apply(const float* __restrict__ P, const float* 
 #pragma unroll
     for (int j = 0; j < NH; j++) a += P[OFF_W3 + k * NH + j] * h2[j];
     y[k] = a;
+  }
+}
+
+// The same network evaluated by a whole wave: lane i (mod NH) computes
+// hidden unit i, so each lane keeps a handful of registers.  Every lane passes
+// the same x and receives y.  Sums run in another order than apply(), so the
+// outputs agree with it to rounding, not bit for bit.
+__device__ __forceinline__ void apply_wave(const float* __restrict__ P, const float* x, float* y) {
+  const int lane = (int)__lane_id();
+  const int i = lane & (NH - 1);
+  float a = P[OFF_B1 + i];
+#pragma unroll
+  for (int j = 0; j < NIN; j++) a += P[OFF_W1 + i * NIN + j] * ((x[j] - P[OFF_MU + j]) / P[OFF_SD + j]);
+  const float h1 = fmaxf(a, 0.f);
+  float b = P[OFF_B2 + i];
+#pragma unroll 8
+  for (int j = 0; j < NH; j++) b += P[OFF_W2 + i * NH + j] * __shfl(h1, j, 64);
+  const float h2 = fmaxf(b, 0.f);
+#pragma unroll
+  for (int k = 0; k < NOUT; k++) {
+    float t = lane < NH ? P[OFF_W3 + k * NH + i] * h2 : 0.f;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    y[k] = P[OFF_B3 + k] + t;
   }
 }
 

@@ -24,6 +24,7 @@
 #include "mlp.hpp"
 #include "sched.hpp"
 #include "solver_kernels.hpp"
+#include "solver_wg.hpp"
 #include "vine.h"
 
 using namespace vine;
@@ -64,6 +65,7 @@ struct vine_graph {
   int64_t m = 0;
   float mean_len = 1.f;
   float mean_delay = 1.f;
+  int32_t max_deg = 0;
   int device = 0;
   int32_t* d_row = nullptr;
   int4* d_edges = nullptr;
@@ -102,6 +104,9 @@ struct vine_solver {
   uint32_t* d_inq = nullptr;
   int32_t* d_wq = nullptr;
   int32_t* d_qorder = nullptr;
+  int4* d_wnear = nullptr;
+  unsigned long long* d_qfin = nullptr;
+  bool wg = false;  // workgroup engine
   // pools
   float2* d_pool_cd = nullptr;
   int32_t* d_pool_id = nullptr;
@@ -263,6 +268,7 @@ vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, cons
     g->mean_len = (float)std::max(sl / m, 1e-3);
     g->mean_delay = (float)std::max(sd / m, 1e-3);
   }
+  for (int32_t u = 0; u < n; u++) g->max_deg = std::max<int32_t>(g->max_deg, row_ptr[u + 1] - row_ptr[u]);
   hipError_t err = hipGetDevice(&g->device);
   size_t eb = sizeof(int4) * std::max<int64_t>(m, 1);
   if (err == hipSuccess) err = hipMalloc(&g->d_row, sizeof(int32_t) * (n + 1));
@@ -319,6 +325,20 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
     set_err("vine_solver_create: invalid options");
     return VINE_E_ARG;
   }
+  if (o.engine < 0 || o.engine > 2) {
+    set_err("vine_solver_create: invalid engine");
+    return VINE_E_ARG;
+  }
+  // the workgroup engine needs the bounds (windows keyed by d + lbl), a
+  // finite window and every label's arcs fitting the LDS candidate arena
+  const bool wg_ok = o.prune == VINE_PRUNE_BOUNDS &&
+                     (o.delta_policy == VINE_DELTA_FIXED || o.delta_policy == VINE_DELTA_MLP) &&
+                     g->max_deg <= dev::ARENA / 2;
+  if (o.engine == VINE_ENGINE_WORKGROUP && !wg_ok) {
+    set_err("vine_solver_create: the workgroup engine needs VINE_PRUNE_BOUNDS, a fixed or learned window "
+            "and out-degrees <= %d (graph: %d)", dev::ARENA / 2, g->max_deg);
+    return VINE_E_ARG;
+  }
   if (o.use_graph || o.hash_slots) {
     set_err("vine_solver_create: use_graph and hash_slots are reserved (rounds are enqueued "
             "rounds_per_sync ahead; fronts live in the HBM pool) and must be 0");
@@ -330,6 +350,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   s->B = o.max_batch;
   s->block = o.block_size;
   s->grid = o.grid_blocks ? o.grid_blocks : 1024;
+  s->wg = wg_ok && o.engine == VINE_ENGINE_WORKGROUP;
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     set_err("hipStreamCreate: %s", hipGetErrorString(e));
@@ -411,6 +432,14 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
     ALLOC(s->d_wq, sizeof(int32_t) * 2 * BN);
     ALLOC(s->d_qorder, sizeof(int32_t) * B);
   }
+  long long wn_cap = 0;
+  if (s->wg) {
+    // per query two near lists; each holds at least the largest far chunk so
+    // that a pull into an empty list always makes progress
+    wn_cap = std::max<long long>((long long)dev::FCH << dev::FCLS_MAX, std::min<long long>(carry_cap / B, 1LL << 22));
+    ALLOC(s->d_wnear, sizeof(int4) * 2 * wn_cap * B);
+    ALLOC(s->d_qfin, sizeof(unsigned long long) * B);
+  }
   ALLOC(s->d_pool_cd, sizeof(float2) * pool_cap);
   ALLOC(s->d_pool_id, sizeof(int32_t) * pool_cap);
   ALLOC(s->d_labs, sizeof(int4) * lab_cap);
@@ -480,6 +509,9 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   c.inq = s->d_inq;
   c.wq = s->d_wq;
   c.q_order = nullptr;  // set per batch when the slots are sorted
+  c.wnear = s->d_wnear;
+  c.wn_cap = wn_cap;
+  c.q_fin = s->d_qfin;
   c.pool_cd = s->d_pool_cd;
   c.pool_id = s->d_pool_id;
   c.pool_cap = (unsigned long long)pool_cap;
@@ -711,64 +743,81 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
   hipLaunchKernelGGL(dev::k_stamp_start, dim3(1), dim3(1), 0, s->stream, c);
   HIP_TRY(hipGetLastError());
 
-  // scheduling
-  const bool ar = s->opt.delta_policy == VINE_DELTA_AR;
-  DopRule rule;
-  if (ar) {
-    double mx = s->opt.ar_max_labels;
-    if (mx <= 0) {
-      size_t fb = 0, tb = 0;
-      HIP_TRY(hipMemGetInfo(&fb, &tb));
-      mx = (double)fb / 12.0;  // free memory / sizeof(c_label), algorithm/cspp.cuh:43-47
-    }
-    rule.reset(mx);
-  }
-  const int K = ar ? 1 : std::max(1, s->opt.rounds_per_sync);
   int64_t rounds = 0;
-  long long fn_last = nb;
-  unsigned long long exp_prev = 0;
-  for (;;) {
-    for (int k = 0; k < K; k++) {
-      // the frontier may grow between polls: size the grid for 4x growth
-      // (grid-stride loops keep any size correct)
-      vine_status ls = launch_round_any(s, 0, grid_for(s, fn_last * 4), st);
-      if (ls != VINE_OK) return ls;
-      rounds++;
+  if (s->wg) {
+    // workgroup engine: one launch runs every query's search to the end, in
+    // the bound searches' order (largest bound first); its time is reported
+    // as ms_expand
+    {
+      Timer t(s, 0);
+      hipLaunchKernelGGL((dev::k_search<c_label, c_constraint<c_label>, c_edge>), dim3(nb), dim3(dev::WG_BLOCK), 0,
+                         s->stream, c, nb);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(t.stop());
     }
+    st->expand_launches++;
     vine_status ps = poll(s);
     if (ps != VINE_OK) return ps;
-    const Counters& h = *s->h_ct;
-    int par = h.round_e & 1;
-    // near tier, new labels, far segments queued for the next expand, far tier
-    long long fn = (long long)h.carry_n[par] + ((long long)h.label_ctr - h.lstart[par]) +
-                   (long long)h.drain_n * dev::FCH + h.far_live;
-    st->max_frontier = std::max<int64_t>(st->max_frontier, fn);
-    fn_last = std::max<long long>(fn, 1);
-    if (fn == 0) break;
-    // every round expands at least the minimum-key label of each live query
-    // (or re-keys carried labels once), so rounds are bounded by twice the
-    // number of labels ever created
-    if (rounds > 2 * (int64_t)h.label_ctr + 64) {
-      set_err("solver made no progress after %lld rounds", (long long)rounds);
-      return VINE_E_OVERFLOW;
-    }
+    rounds = s->h_ct->round_e;
+  } else {
+    // scheduling
+    const bool ar = s->opt.delta_policy == VINE_DELTA_AR;
+    DopRule rule;
     if (ar) {
-      // the reference's DOP rule picks how many labels the next round should
-      // expand; a multiplicative controller turns that count into the key
-      // window (DOP >= frontier means the whole frontier, as in the reference)
-      rule.update((int64_t)fn);
-      const double dop = (double)rule.dop();
-      const unsigned long long x = h.expanded - exp_prev;
-      exp_prev = h.expanded;
-      if (dop >= (double)fn) {
-        delta_ar = dev::FINF;
-      } else {
-        if (!(delta_ar < dev::FINF)) delta_ar = s->g->mean_len;
-        double ratio = x > 0 ? dop / (double)x : 4.0;
-        delta_ar = (float)std::min(1e7, std::max(0.25, delta_ar * std::min(4.0, std::max(0.25, std::sqrt(ratio)))));
+      double mx = s->opt.ar_max_labels;
+      if (mx <= 0) {
+        size_t fb = 0, tb = 0;
+        HIP_TRY(hipMemGetInfo(&fb, &tb));
+        mx = (double)fb / 12.0;  // free memory / sizeof(c_label), algorithm/cspp.cuh:43-47
       }
-      vine_status fs = fill_f32(s, s->d_qdelta, delta_ar, s->B);
-      if (fs != VINE_OK) return fs;
+      rule.reset(mx);
+    }
+    const int K = ar ? 1 : std::max(1, s->opt.rounds_per_sync);
+    long long fn_last = nb;
+    unsigned long long exp_prev = 0;
+    for (;;) {
+      for (int k = 0; k < K; k++) {
+        // the frontier may grow between polls: size the grid for 4x growth
+        // (grid-stride loops keep any size correct)
+        vine_status ls = launch_round_any(s, 0, grid_for(s, fn_last * 4), st);
+        if (ls != VINE_OK) return ls;
+        rounds++;
+      }
+      vine_status ps = poll(s);
+      if (ps != VINE_OK) return ps;
+      const Counters& h = *s->h_ct;
+      int par = h.round_e & 1;
+      // near tier, new labels, far segments queued for the next expand, far tier
+      long long fn = (long long)h.carry_n[par] + ((long long)h.label_ctr - h.lstart[par]) +
+                     (long long)h.drain_n * dev::FCH + h.far_live;
+      st->max_frontier = std::max<int64_t>(st->max_frontier, fn);
+      fn_last = std::max<long long>(fn, 1);
+      if (fn == 0) break;
+      // every round expands at least the minimum-key label of each live query
+      // (or re-keys carried labels once), so rounds are bounded by twice the
+      // number of labels ever created
+      if (rounds > 2 * (int64_t)h.label_ctr + 64) {
+        set_err("solver made no progress after %lld rounds", (long long)rounds);
+        return VINE_E_OVERFLOW;
+      }
+      if (ar) {
+        // the reference's DOP rule picks how many labels the next round should
+        // expand; a multiplicative controller turns that count into the key
+        // window (DOP >= frontier means the whole frontier, as in the reference)
+        rule.update((int64_t)fn);
+        const double dop = (double)rule.dop();
+        const unsigned long long x = h.expanded - exp_prev;
+        exp_prev = h.expanded;
+        if (dop >= (double)fn) {
+          delta_ar = dev::FINF;
+        } else {
+          if (!(delta_ar < dev::FINF)) delta_ar = s->g->mean_len;
+          double ratio = x > 0 ? dop / (double)x : 4.0;
+          delta_ar = (float)std::min(1e7, std::max(0.25, delta_ar * std::min(4.0, std::max(0.25, std::sqrt(ratio)))));
+        }
+        vine_status fs = fill_f32(s, s->d_qdelta, delta_ar, s->B);
+        if (fs != VINE_OK) return fs;
+      }
     }
   }
   hipLaunchKernelGGL(dev::k_stamp_round, dim3(1), dim3(1), 0, s->stream, c, (int)rounds);
@@ -875,10 +924,20 @@ vine_status solve_impl(vine_solver* s, int32_t nq, const int32_t* d_src, const i
         }
       }
     }
+    std::vector<unsigned long long> fin;
+    if (s->wg) {
+      // workgroup engine: each query's workgroup stamps its own end
+      fin.resize(nb);
+      HIP_TRY(hipMemcpy(fin.data(), s->d_qfin, sizeof(unsigned long long) * nb, hipMemcpyDeviceToHost));
+    }
     for (int i = 0; i < nb; i++) {
       int r = last[i] + 1;  // the query's last active round ends when the next one starts
       double us = 0.0;
-      if (r >= 1 && r < nr && ts[r] >= tstart) us = (double)(ts[r] - tstart) * 1000.0 / rate_khz;
+      if (s->wg) {
+        if (fin[i] >= tstart) us = (double)(fin[i] - tstart) * 1000.0 / rate_khz;
+      } else if (r >= 1 && r < nr && ts[r] >= tstart) {
+        us = (double)(ts[r] - tstart) * 1000.0 / rate_khz;
+      }
       if (q0 + i < (int32_t)s->latency_us.size()) s->latency_us[q0 + i] = us;
     }
     hipLaunchKernelGGL(dev::k_answers, dim3((nb + 255) / 256), dim3(256), 0, s->stream, c, nb, d_cost + q0,

@@ -140,6 +140,9 @@ struct Ctx {
   float far_floor;    // smallest near-tier reach (key units)
   int32_t* wq;       // [B][2][n] bound-search frontier queues
   int32_t* q_order;  // [B] bound-search dispatch order (null: slot order)
+  int4* wnear;       // workgroup engine: [B][2][wn_cap] near-tier records per query
+  long long wn_cap;
+  unsigned long long* q_fin;  // [B] device clock when the query's search ended (workgroup engine)
   unsigned long long* ts;  // [TS_CAP] per-round merge-end wall clock
   Counters* ct;
   int32_t hash_slots;
@@ -598,6 +601,47 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
 // Expand.  PRUNE: VINE_PRUNE_* ; policy_count > 0 applies the reference's DOP
 // count window (expand the last DOP frontier entries, carry the rest,
 // algorithm/cspp.cuh:179-216).
+// Append a carried entry to bucket fb of query q's far tier: lock-free on the
+// bucket's chunk list (as the candidate lists); a full head chunk is followed
+// by one twice its size.
+__device__ __forceinline__ void far_append(const Ctx& c, Counters* ct, int q, int fb, const int4& crec4) {
+  atomicAdd(&c.far_cnt[q], 1);
+  int* headp = &c.far_head[(long long)q * NBK + fb];
+  const int hv = __hip_atomic_load(headp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  bool done = false;
+  int cls = 0;
+  if (hv >= 0) {
+    const int blk = hv >> 4;
+    cls = (hv & 15) + 1;
+    const int slot = atomicAdd(&c.fhd[blk].x, 1);
+    if (slot < (FCH << (hv & 15))) {
+      c.frec[(long long)blk * FCH + slot] = crec4;
+      done = true;
+    }
+  }
+  if (!done) {
+    cls = min(cls, FCLS_MAX);
+    // blocks come from a bump allocator reset with each batch
+    const int blk = atomicAdd(&ct->far_chunks, 1 << cls);
+    if (blk < 0 || blk > c.far_cap - (1 << cls)) {
+      atomicOr(&ct->overflow, OVF_FAR);
+    } else {
+      c.frec[(long long)blk * FCH] = crec4;
+      const int mine = (blk << 4) | cls;
+      int expected = hv;
+      for (;;) {
+        c.fhd[blk] = make_int2(1, expected);
+        __threadfence();
+        int prev = atomicCAS(headp, expected, mine);
+        if (prev == expected) break;
+        expected = prev;
+      }
+      // a bucket becomes non-empty only through a CAS from -1
+      if (expected < 0) atomicOr(&c.far_occ[(long long)q * NBW + (fb >> 6)], 1ull << (fb & 63));
+    }
+  }
+}
+
 template <typename Label_t, typename Constraint_t, typename Edge_t, int PRUNE>
 __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
   using LT = label_traits<Label_t>;
@@ -714,45 +758,7 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
         atomicOr(&ct->overflow, OVF_CARRY);
       }
     }
-    if (far) {
-      atomicAdd(&c.far_cnt[q], 1);
-      // lock-free append to the bucket's chunk list (as the candidate lists);
-      // a full head chunk is followed by one twice its size
-      int* headp = &c.far_head[(long long)q * NBK + fb];
-      const int hv = __hip_atomic_load(headp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      bool done = false;
-      int cls = 0;
-      if (hv >= 0) {
-        const int blk = hv >> 4;
-        cls = (hv & 15) + 1;
-        const int slot = atomicAdd(&c.fhd[blk].x, 1);
-        if (slot < (FCH << (hv & 15))) {
-          c.frec[(long long)blk * FCH + slot] = crec4;
-          done = true;
-        }
-      }
-      if (!done) {
-        cls = min(cls, FCLS_MAX);
-        // blocks come from a bump allocator reset with each batch
-        const int blk = atomicAdd(&ct->far_chunks, 1 << cls);
-        if (blk < 0 || blk > c.far_cap - (1 << cls)) {
-          atomicOr(&ct->overflow, OVF_FAR);
-        } else {
-          c.frec[(long long)blk * FCH] = crec4;
-          const int mine = (blk << 4) | cls;
-          int expected = hv;
-          for (;;) {
-            c.fhd[blk] = make_int2(1, expected);
-            __threadfence();
-            int prev = atomicCAS(headp, expected, mine);
-            if (prev == expected) break;
-            expected = prev;
-          }
-          // a bucket becomes non-empty only through a CAS from -1
-          if (expected < 0) atomicOr(&c.far_occ[(long long)q * NBW + (fb >> 6)], 1ull << (fb & 63));
-        }
-      }
-    }
+    if (far) far_append(c, ct, q, fb, crec4);
     {
       int nf = far ? 1 : 0;
 #pragma unroll

@@ -15,6 +15,7 @@ from ._lib import lib
 
 PRUNE_NONE, PRUNE_TARGET, PRUNE_BOUNDS = 0, 1, 2
 DELTA_INF, DELTA_FIXED, DELTA_AR, DELTA_MLP = 0, 1, 2, 3
+ENGINE_AUTO, ENGINE_ROUNDS, ENGINE_WORKGROUP = 0, 1, 2
 
 VINE_OK, VINE_E_ARG, VINE_E_OOM, VINE_E_HIP, VINE_E_OVERFLOW, VINE_E_RANGE = 0, -1, -2, -3, -4, -5
 _STATUS = {VINE_E_ARG: "VINE_E_ARG", VINE_E_OOM: "VINE_E_OOM", VINE_E_HIP: "VINE_E_HIP",
@@ -47,7 +48,8 @@ class Options(ctypes.Structure):
         ("time_kernels", ctypes.c_int32),
         ("bound_delta", ctypes.c_float),
         ("bound_ball", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("engine", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
