@@ -211,6 +211,13 @@ vine_status vine_solver_query_stats(const vine_solver* s, int32_t nq, int64_t* e
 int vine_ar_burg(const double* x, int n, int degree, double* coef);
 int64_t vine_dop_sequence(const int64_t* sizes, int n, double max_labels, int64_t* out_dop);
 
+/* The internal vertex order vine_graph_create uses (no counterpart in the
+ * reference, whose CSR keeps the input ids, include/graph.hpp:136-167):
+ * out_order[new id] = caller's id, per connected component along a Hilbert
+ * curve over hop distances from two far-apart vertices, so that neighbours
+ * share cache lines of the per-(query, vertex) arrays.  Host only. */
+int32_t vine_locality_order(int32_t n, const int32_t* row_ptr, const int32_t* dst, int32_t* out_order);
+
 #ifdef __cplusplus
 }
 #endif

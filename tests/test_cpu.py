@@ -185,3 +185,39 @@ def test_window_labels_are_batch_aware():
     pick, budget, cost = label_windows(E, R, round_cost=50.0)
     assert list(pick) == [0, 0, 1] and budget == 100
     assert cost == 100 + 100 + 4000 + 50.0 * 100
+
+
+def test_locality_order_is_a_permutation_that_packs_neighbours():
+    """The internal vertex order (vine_locality_order, used by vine_graph_create):
+    a permutation, each component contiguous, and on the road-like k-NN graph
+    arcs span far fewer ids than under the generator's random ids."""
+    L = _lib.lib()
+    fn = L.vine_locality_order
+    fn.restype = ctypes.c_int32
+    fn.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    g = W.knn(20000, seed=3)
+    row = np.ascontiguousarray(g.row_ptr, np.int32)
+    dst = np.ascontiguousarray(g.dst, np.int32)
+    order = np.zeros(g.n, np.int32)
+    assert fn(g.n, row.ctypes.data, dst.ctypes.data, order.ctypes.data) == 0
+    assert np.array_equal(np.sort(order), np.arange(g.n))
+    new = np.empty(g.n, np.int64)
+    new[order] = np.arange(g.n)
+    src = np.repeat(np.arange(g.n), np.diff(row))
+    before = np.abs(src - dst).mean()
+    after = np.abs(new[src] - new[dst]).mean()
+    assert after < before / 20, (before, after)
+    # vertices of one component are contiguous in the new order
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+    ncomp, lab = connected_components(csr_matrix((np.ones(dst.shape[0]), dst, row), shape=(g.n, g.n)),
+                                      directed=True, connection="weak")
+    comp_in_order = lab[order]
+    changes = np.count_nonzero(comp_in_order[1:] != comp_in_order[:-1])
+    assert changes == ncomp - 1
+    # a graph with isolated vertices and a directed chain
+    row2 = np.array([0, 1, 1, 2, 2], np.int32)
+    dst2 = np.array([1, 3], np.int32)
+    o2 = np.zeros(4, np.int32)
+    assert fn(4, row2.ctypes.data, dst2.ctypes.data, o2.ctypes.data) == 0
+    assert sorted(o2.tolist()) == [0, 1, 2, 3]

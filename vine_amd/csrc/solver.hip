@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "mlp.hpp"
+#include "layout.hpp"
 #include "sched.hpp"
 #include "solver_kernels.hpp"
 #include "solver_wg.hpp"
@@ -67,6 +68,11 @@ struct vine_graph {
   float mean_delay = 1.f;
   int32_t max_deg = 0;
   int device = 0;
+  // vertices are renumbered for locality (layout.cpp): perm[caller id] =
+  // internal id, d_inv the inverse; every array below uses internal ids
+  std::vector<int32_t> perm;
+  int32_t* d_perm = nullptr;
+  int32_t* d_inv = nullptr;
   int32_t* d_row = nullptr;
   int4* d_edges = nullptr;
   int32_t* d_rrow = nullptr;
@@ -228,9 +234,6 @@ vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, cons
       set_err("vine_graph_create: row_ptr not monotone at %d", i);
       return VINE_E_ARG;
     }
-  // packed records {dst, len, delay, 0}; transposed copy for the bound search
-  std::vector<int4> e(std::max<int64_t>(m, 1)), re(std::max<int64_t>(m, 1));
-  std::vector<int32_t> rrow(n + 1, 0);
   for (int64_t p = 0; p < m; p++) {
     if (dst[p] < 0 || dst[p] >= n) {
       set_err("vine_graph_create: arc %lld has invalid head %d", (long long)p, dst[p]);
@@ -240,8 +243,31 @@ vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, cons
       set_err("vine_graph_create: arc %lld has a negative or NaN weight", (long long)p);
       return VINE_E_ARG;
     }
-    rrow[dst[p] + 1]++;
   }
+  // renumber for locality, keeping each vertex's arcs in their order
+  const std::vector<int32_t> order = vine::locality_order(n, row_ptr, dst);
+  std::vector<int32_t> perm(n);
+  for (int32_t i = 0; i < n; i++) perm[order[i]] = i;
+  std::vector<int32_t> prow(n + 1, 0), pdst(std::max<int64_t>(m, 1));
+  std::vector<float> plen(std::max<int64_t>(m, 1)), pdelay(std::max<int64_t>(m, 1));
+  for (int32_t i = 0; i < n; i++) {
+    const int32_t u = order[i];
+    int32_t w = prow[i];
+    for (int32_t p = row_ptr[u]; p < row_ptr[u + 1]; p++, w++) {
+      pdst[w] = perm[dst[p]];
+      plen[w] = len[p];
+      pdelay[w] = delay[p];
+    }
+    prow[i + 1] = w;
+  }
+  row_ptr = prow.data();
+  dst = pdst.data();
+  len = plen.data();
+  delay = pdelay.data();
+  // packed records {dst, len, delay, 0}; transposed copy for the bound search
+  std::vector<int4> e(std::max<int64_t>(m, 1)), re(std::max<int64_t>(m, 1));
+  std::vector<int32_t> rrow(n + 1, 0);
+  for (int64_t p = 0; p < m; p++) rrow[dst[p] + 1]++;
   for (int32_t v = 0; v < n; v++) rrow[v + 1] += rrow[v];
   std::vector<int32_t> fill(rrow.begin(), rrow.end() - 1);
   for (int32_t u = 0; u < n; u++)
@@ -259,6 +285,7 @@ vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, cons
   auto* g = new vine_graph();
   g->n = n;
   g->m = m;
+  g->perm = perm;
   if (m > 0) {
     double sl = 0, sd = 0;
     for (int64_t p = 0; p < m; p++) {
@@ -279,6 +306,10 @@ vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, cons
   if (err == hipSuccess) err = hipMemcpy(g->d_rrow, rrow.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy(g->d_edges, e.data(), eb, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy(g->d_redges, re.data(), eb, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMalloc(&g->d_perm, sizeof(int32_t) * n);
+  if (err == hipSuccess) err = hipMalloc(&g->d_inv, sizeof(int32_t) * n);
+  if (err == hipSuccess) err = hipMemcpy(g->d_perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(g->d_inv, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
   if (err != hipSuccess) {
     set_err("vine_graph_create: %s", hipGetErrorString(err));
     vine_graph_destroy(g);
@@ -294,6 +325,8 @@ vine_status vine_graph_destroy(vine_graph* g) {
   if (g->d_rrow) (void)hipFree(g->d_rrow);
   if (g->d_edges) (void)hipFree(g->d_edges);
   if (g->d_redges) (void)hipFree(g->d_redges);
+  if (g->d_perm) (void)hipFree(g->d_perm);
+  if (g->d_inv) (void)hipFree(g->d_inv);
   delete g;
   return VINE_OK;
 }
@@ -481,6 +514,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   c.edges = g->d_edges;
   c.rrow_ptr = g->d_rrow;
   c.redges = g->d_redges;
+  c.vinv = g->d_inv;
   c.q_src = s->d_qsrc;
   c.q_dst = s->d_qdst;
   c.q_bound = s->d_qbound;
@@ -890,8 +924,12 @@ vine_status solve_impl(vine_solver* s, int32_t nq, const int32_t* d_src, const i
   s->trace.clear();
   for (int32_t q0 = 0; q0 < nq; q0 += s->B) {
     int nb = std::min<int32_t>(s->B, nq - q0);
-    HIP_TRY(hipMemcpyAsync(s->d_qsrc, d_src + q0, sizeof(int32_t) * nb, hipMemcpyDeviceToDevice, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->d_qdst, d_dst + q0, sizeof(int32_t) * nb, hipMemcpyDeviceToDevice, s->stream));
+    // endpoints in internal vertex ids (ids were validated by the caller)
+    hipLaunchKernelGGL(dev::k_map_ids, dim3((nb + 255) / 256), dim3(256), 0, s->stream, d_src + q0, s->g->d_perm,
+                       s->d_qsrc, nb);
+    hipLaunchKernelGGL(dev::k_map_ids, dim3((nb + 255) / 256), dim3(256), 0, s->stream, d_dst + q0, s->g->d_perm,
+                       s->d_qdst, nb);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(s->d_qbound, d_bound + q0, sizeof(float) * nb, hipMemcpyDeviceToDevice, s->stream));
     if (nb < s->B) {
       // idle slots: harmless query 0 -> 0 with bound 0 (never initialised as live)
@@ -1145,7 +1183,8 @@ int64_t vine_solver_pareto(const vine_solver* s, int32_t q, int32_t v, float* ou
     return VINE_E_ARG;
   }
   int2 ps;
-  hipError_t e = hipMemcpy(&ps, s->d_pset + (long long)q * s->g->n + v, sizeof(int2), hipMemcpyDeviceToHost);
+  hipError_t e = hipMemcpy(&ps, s->d_pset + (long long)q * s->g->n + s->g->perm[v], sizeof(int2),
+                           hipMemcpyDeviceToHost);
   if (e != hipSuccess) {
     set_err("vine_solver_pareto: %s", hipGetErrorString(e));
     return VINE_E_HIP;

@@ -85,6 +85,7 @@ struct Ctx {
   const int4* edges;  // {dst, len bits, delay bits, 0}
   const int32_t* rrow_ptr;
   const int4* redges;
+  const int32_t* vinv;  // internal vertex id -> caller's id
   // per query slot
   const int32_t* q_src;
   const int32_t* q_dst;
@@ -1297,16 +1298,23 @@ __global__ void k_answers(Ctx c, int nq, float* cost, float* delay, int32_t* fea
   }
 }
 
+// Paths are written in the caller's vertex ids (vinv: internal -> caller).
 __global__ void k_paths(Ctx c, int nq, const long long* off, int32_t* path) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
   long long o = off[q], e = off[q + 1];
   long long k = e - 1;
-  for (int x = c.q_best_id[q]; x >= 0 && k >= o; x = c.labs[x].w) path[k--] = c.labs[x].x;
+  for (int x = c.q_best_id[q]; x >= 0 && k >= o; x = c.labs[x].w) path[k--] = c.vinv[c.labs[x].x];
 }
 
 __global__ void k_stamp_start(Ctx c) { c.ct->t_start = wall_clock64(); }
 __global__ void k_stamp_round(Ctx c, int r) { c.ts[r & (TS_CAP - 1)] = wall_clock64(); }
+
+// Query endpoints from the caller's vertex ids to internal ones.
+__global__ void k_map_ids(const int32_t* in, const int32_t* perm, int32_t* out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = perm[in[i]];
+}
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
