@@ -163,7 +163,7 @@ vine_status vine_solver_latency(const vine_solver* s, int32_t nq, double* out_us
 /* Per-(query, round) scheduler features of the last solve (record_trace = k
  * records every k-th round of every live query, up to 8M rows): row i is
  * rows[i*10 .. i*10+10) = {query, round, x0..x7} with the MLP inputs x0..x7
- * (see vine_amd/csrc/solver_kernels.hpp, k_policy).  With rows = NULL returns
+ * (see vine_amd/csrc/solver_kernels.hpp, policy_query).  With rows = NULL returns
  * the row count; else copies up to max_rows and returns the number copied.
  * The query column is the index into the solve's query arrays. */
 int64_t vine_solver_trace(const vine_solver* s, float* rows, int64_t max_rows);

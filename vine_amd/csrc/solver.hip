@@ -660,15 +660,10 @@ vine_status launch_round(vine_solver* s, long long dop, int gsz, vine_stats* st)
     HIP_TRY(hipGetLastError());
     HIP_TRY(t.stop());
   }
-  if (s->opt.delta_policy != VINE_DELTA_INF) {
+  if (s->opt.delta_policy != VINE_DELTA_INF || c.stats_on) {
     Timer t(s, 3);
-    hipLaunchKernelGGL((dev::k_pull<PRUNE>), dim3((s->B + 3) / 4), dim3(256), 0, s->stream, c);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(t.stop());
-  }
-  if (c.stats_on) {
-    Timer t(s, 3);
-    hipLaunchKernelGGL(dev::k_policy, dim3((s->B + 255) / 256), dim3(256), 0, s->stream, c, s->nb_active);
+    hipLaunchKernelGGL((dev::k_pull_policy<PRUNE>), dim3((s->B + 3) / 4), dim3(256), 0, s->stream, c,
+                       s->nb_active);
     HIP_TRY(hipGetLastError());
     HIP_TRY(t.stop());
   }

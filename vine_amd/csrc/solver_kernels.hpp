@@ -918,6 +918,8 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
       if (PRUNE == 2) lb = c.lbl[qv];
     }
     if (have && gl == 0) c.cand_head[qv] = -1;
+    // the front's last entry stays in registers across the candidates
+    float2 lastv = (have && cnt > 0) ? c.pool_cd[off + cnt - 1] : make_float2(-1.f, FINF);
     int steps = 0;
     while (__ballot(h >= 0)) {
       if (h >= 0 && ++steps > max_steps) {
@@ -935,6 +937,11 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
       int kmax = n8;
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o, WAVE));
+      // label ids for the whole chunk in one wave-wide allocation; the ids
+      // of candidates that do not survive are marked dead below
+      long long rbase = wave_alloc64_n(&ct->label_ctr, gl == 0 ? n8 : 0);
+      rbase = __shfl(rbase, gbase, WAVE);
+      int used = 0;
       for (int k = 0; k < kmax; k++) {
         const bool act = k < n8;  // group-uniform
         const float cd = __int_as_float(__shfl(myrec.x, gbase + k, WAVE));
@@ -949,8 +956,7 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
         int p = 0;
         float2 prev = make_float2(-1.f, FINF);
         if (act && cnt > 0) {
-          const float2 last = c.pool_cd[off + cnt - 1];
-          scanned++;
+          const float2 last = lastv;
           if (last.x <= cd) {
             p = cnt;
             prev = last;
@@ -1001,8 +1007,7 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
         }
         long long poff = wave_alloc64_n(&ct->pool_ctr, (grow && gl == 0) ? newcap : 0);
         poff = __shfl(poff, gbase, WAVE);
-        long long lid = wave_alloc64(&ct->label_ctr, acc && gl == 0);
-        lid = __shfl(lid, gbase, WAVE);
+        const long long lid = rbase + used;
         bool fits = acc;
         if (grow && (poff < 0 || (unsigned long long)(poff + newcap) > c.pool_cap || poff + newcap > 0xFFFFFFFFll)) {
           if (gl == 0) atomicOr(&ct->overflow, OVF_POOL);
@@ -1068,13 +1073,17 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
             if (!(PRUNE != 0 && v == tq) && __float_as_uint(key) < load_agent_u32(&fmin_nxt[q]))
               atomicMin(&fmin_nxt[q], __float_as_uint(key));
           }
+          if (start + 1 == newcnt) lastv = make_float2(cd, cdl);
           cnt = newcnt;
+          used++;
         }
         if (c.stats_on) {
           wave_add_keyed(c.q_ins, q, fits && gl == 0);
           wave_add_keyed(c.q_near, q, fits && gl == 0);
         }
       }
+      // reserved ids no candidate took: dead, so the next expand skips them
+      if (gl >= used && gl < n8 && rbase + gl < c.label_cap) c.dead[rbase + gl] = 1;
       h = nexth;
     }
     if (have && gl == 0) c.pset[qv] = pset_make(off, cnt, cap);
@@ -1101,14 +1110,12 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
 // bitmap (one bit per bucket) finds the non-empty buckets of a key range
 // 4,096 buckets per load step.  One wave per query.
 template <int PRUNE>
-__global__ void __launch_bounds__(256) k_pull(Ctx c) {
+__device__ void pull_query(const Ctx& c, long long q) {
   Counters* ct = c.ct;
   const int rn = ct->round_e;  // the round that follows
   const int nc = rn & 1;
   uint32_t* fmin = c.q_fmin + (size_t)nc * c.B;
   const int lane = lane_id();
-  const long long q = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-  if (q >= c.B) return;  // whole waves exit together
   const int nb = c.far_next[q];
   if (nb >= NBK) return;
   const float front = __uint_as_float(load_agent_u32(&fmin[q]));
@@ -1230,19 +1237,31 @@ __global__ void __launch_bounds__(256) k_pull(Ctx c) {
 // Output 0 gives the next window as 2^y0, clamped to the range the network is
 // trained on, [2^MLP_LOG2_MIN, 2^MLP_LOG2_MAX]; output 1
 // is a forecast of log2(1 + next round's expansions).
-__global__ void k_policy(Ctx c, int nq) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= c.B) return;
+__device__ void policy_query(const Ctx& c, int q, int nq) {
   Counters* ct = c.ct;
+  const int lane = lane_id();
   const int rn = ct->round_e;
   const int nc = rn & 1;
-  const int exp_ = c.q_exp[q], ins = c.q_ins[q], near = c.q_near[q];
-  c.q_exp[q] = 0;
-  c.q_ins[q] = 0;
-  c.q_near[q] = 0;
-  const long long tot = c.q_exptot[q] + exp_;
-  c.q_exptot[q] = tot;
-  const int far = c.far_cnt[q];
+  // per-query counters: read and reset by lane 0 (it also wrote q_near and
+  // far_cnt in the pull just before), broadcast to the wave
+  int exp_ = 0, ins = 0, near = 0, far = 0;
+  long long tot = 0;
+  if (lane == 0) {
+    exp_ = c.q_exp[q];
+    ins = c.q_ins[q];
+    near = c.q_near[q];
+    far = c.far_cnt[q];
+    c.q_exp[q] = 0;
+    c.q_ins[q] = 0;
+    c.q_near[q] = 0;
+    tot = c.q_exptot[q] + exp_;
+    c.q_exptot[q] = tot;
+  }
+  exp_ = __shfl(exp_, 0, WAVE);
+  ins = __shfl(ins, 0, WAVE);
+  near = __shfl(near, 0, WAVE);
+  far = __shfl(far, 0, WAVE);
+  tot = __shfl(tot, 0, WAVE);
   const bool active = q < nq && (near + far + exp_) > 0;
   if (!active) return;
   const float fmin = __uint_as_float(c.q_fmin[(size_t)nc * c.B + q]);
@@ -1259,7 +1278,7 @@ __global__ void k_policy(Ctx c, int nq) {
   x[5] = log2f(1.f + (float)rn);
   x[6] = log2f(1.f + (float)tot);
   x[7] = c.q_best_id[q] >= 0 ? 1.f : 0.f;
-  if (c.trace && c.trace_every > 0 && rn % c.trace_every == 0) {
+  if (lane == 0 && c.trace && c.trace_every > 0 && rn % c.trace_every == 0) {
     long long row = atomicAdd((unsigned long long*)&ct->trace_n, 1ull);
     if (row < c.trace_cap) {
       float* out = c.trace + row * 10;
@@ -1271,13 +1290,25 @@ __global__ void k_policy(Ctx c, int nq) {
   }
   if (c.policy == 3 && c.mlp) {
     float y[mlp::NOUT];
-    mlp::apply(c.mlp, x, y);
+    mlp::apply_wave(c.mlp, x, y);
     float l = y[0];
     if (!(l == l)) l = log2f(fmaxf(c.far_floor, 1.f));  // NaN output: fall back to a small window
     l = fminf(fmaxf(l, MLP_LOG2_MIN), MLP_LOG2_MAX);
-    c.q_delta[q] = exp2f(l);
-    c.q_fcast[q] = y[1];
+    if (lane == 0) {
+      c.q_delta[q] = exp2f(l);
+      c.q_fcast[q] = y[1];
+    }
   }
+}
+
+// After merge: per query (one wave each) the far-tier pull, then the policy
+// step when statistics are on (learned window or trace).
+template <int PRUNE>
+__global__ void __launch_bounds__(256) k_pull_policy(Ctx c, int nq) {
+  const long long q = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  if (q >= c.B) return;  // whole waves exit together
+  if (c.policy != 0) pull_query<PRUNE>(c, q);
+  if (c.stats_on) policy_query(c, (int)q, nq);
 }
 
 // ---------------------------------------------------------------------------
