@@ -92,7 +92,8 @@ typedef struct {
   int64_t frontier_capacity;/* carried-frontier entries (0 = auto) */
   double ar_max_labels;     /* MAX for the DOP rule (0 = free device memory / 12 B, as the reference) */
   int32_t record_trace;     /* keep per-round frontier statistics (for vine_solver_trace) */
-  int32_t time_kernels;     /* bracket every round kernel with HIP events (vine_stats.ms_*) */
+  int32_t time_kernels;     /* HIP-event timing (vine_stats.ms_*): 0 off, k >= 1 brackets the kernels of
+                               every k-th round (and every bound search); timed_rounds counts them */
   float bound_delta;        /* distance window of the bound search (0 = 2 x mean arc weight) */
   int32_t bound_ball;       /* 0: a forward min-delay search from s first, then the reverse bound
                                searches only inside fwd + lbd <= bound; 1: reverse searches over
@@ -127,7 +128,7 @@ typedef struct {
   int64_t near_carried;     /* frontier entries re-queued in the near tier (summed over rounds) */
   int64_t far_appended;     /* frontier entries parked in far tiers */
   int64_t scanned;          /* frontier entries read by the expand kernels */
-  int64_t reserved[1];
+  int64_t timed_rounds;     /* rounds whose kernels the ms_* figures cover (time_kernels) */
 } vine_stats;
 
 void vine_options_default(vine_options* opt);

@@ -154,6 +154,7 @@ struct vine_solver {
   };
   std::vector<hipEvent_t> evpool;
   size_t ev_used = 0;
+  long long round_seq = 0;  // round launches of the current solve (for sampled timing)
   std::vector<EvRec> evrec;
 
   // last-solve records
@@ -591,7 +592,9 @@ struct Timer {
   bool on;
   hipEvent_t a = nullptr;
   int kind;
-  Timer(vine_solver* s_, int kind_) : s(s_), on(s_->opt.time_kernels != 0), kind(kind_) {
+  // sample = true: this launch belongs to a round picked by time_kernels
+  Timer(vine_solver* s_, int kind_, bool sample = true)
+      : s(s_), on(s_->opt.time_kernels != 0 && sample), kind(kind_) {
     if (on) {
       a = next_event(s);
       if (a) (void)hipEventRecord(a, s->stream);
@@ -647,21 +650,24 @@ vine_status poll(vine_solver* s) {
 template <int PRUNE>
 vine_status launch_round(vine_solver* s, long long dop, int gsz, vine_stats* st) {
   Ctx& c = s->ctx;
+  const int tk = std::max(1, s->opt.time_kernels);
+  const bool sample = s->opt.time_kernels != 0 && (s->round_seq++ % tk) == 0;
+  if (sample) st->timed_rounds++;
   {
-    Timer t(s, 0);
+    Timer t(s, 0, sample);
     hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, PRUNE>), dim3(gsz),
                        dim3(s->block), 0, s->stream, c, dop);
     HIP_TRY(hipGetLastError());
     HIP_TRY(t.stop());
   }
   {
-    Timer t(s, 1);
+    Timer t(s, 1, sample);
     hipLaunchKernelGGL((dev::k_merge<c_label, PRUNE>), dim3(gsz), dim3(s->block), 0, s->stream, c);
     HIP_TRY(hipGetLastError());
     HIP_TRY(t.stop());
   }
   if (s->opt.delta_policy != VINE_DELTA_INF || c.stats_on) {
-    Timer t(s, 3);
+    Timer t(s, 3, sample);
     hipLaunchKernelGGL((dev::k_pull_policy<PRUNE>), dim3((s->B + 3) / 4), dim3(256), 0, s->stream, c,
                        s->nb_active);
     HIP_TRY(hipGetLastError());
@@ -914,6 +920,7 @@ vine_status solve_impl(vine_solver* s, int32_t nq, const int32_t* d_src, const i
                        int32_t* d_plen, vine_stats* st) {
   Ctx& c = s->ctx;
   s->latency_us.assign(nq, 0.0);
+  s->round_seq = 0;
   s->q_expanded.assign(nq, 0);
   s->q_rounds.assign(nq, 0);
   s->trace.clear();

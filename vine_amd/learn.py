@@ -6,8 +6,10 @@
    scheduler features.
 2. Each query is labelled with a window (label_windows): rounds are shared by
    the batch, so a round budget is chosen to minimise labels expanded +
-   ROUND_COST * rounds over the batch, and each query takes its least-work
-   window within the budget.  Every feature row recorded under that window
+   round_cost * rounds over the batch, and each query takes its least-work
+   window within the budget.  round_cost (how many label expansions one
+   round's fixed kernel time is worth) is fitted by least squares to the
+   collection runs themselves: seconds ~ a * rounds + b * expanded + c.  Every feature row recorded under that window
    becomes a sample with target
    log2(window); the second output is trained to forecast log2(1 +
    expansions) TRACE_EVERY rounds ahead from the same rows.
@@ -30,11 +32,10 @@ from . import mlp as M
 from . import solver as S
 from . import workload as W
 
-CANDIDATES = (4.0, 8.0, 12.0, 16.0, 24.0, 32.0, 48.0)
-# labels a round's fixed kernel cost is worth: batched solver on MI355X,
-# about 77 us per round and 68 ns per expanded label (least squares over
-# fixed-window runs of the 4,096-query NY workload)
-ROUND_COST = 1150.0
+CANDIDATES = (4.0, 8.0, 12.0, 16.0, 24.0, 32.0, 48.0, 64.0, 96.0)
+# fallback for round_cost when the fit is degenerate (labels a round's fixed
+# kernel time is worth on MI355X: ~63 us per round, ~19 ns per expansion)
+ROUND_COST = 3400.0
 TRACE_EVERY = 4  # feature rows every 4th round
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(HERE, "models", "delta_mlp.npy")
@@ -48,7 +49,7 @@ def solver_for(dg, batch, policy, delta, trace=0, **kw):
     return S.Solver(dg, max_batch=batch, prune=S.PRUNE_BOUNDS, delta_policy=policy, delta_fixed=delta,
                     rounds_per_sync=16, record_trace=trace, label_capacity=kw.get("label_capacity", 600_000_000),
                     pool_capacity=kw.get("pool_capacity", 1_200_000_000), cand_capacity=100_000_000,
-                    frontier_capacity=200_000_000)
+                    frontier_capacity=400_000_000)
 
 
 def collect(dg, q, batch, candidates=CANDIDATES):
@@ -61,9 +62,24 @@ def collect(dg, q, batch, candidates=CANDIDATES):
             ex, rd = s.query_stats(len(q))
             tr = s.trace()
         out[d] = {"expanded": ex, "rounds": rd, "trace": tr, "cost": a.cost, "feasible": a.feasible,
-                  "seconds": time.time() - t0}
+                  "seconds": time.time() - t0, "batch_rounds": int(a.stats["rounds"]),
+                  "batch_expanded": int(a.stats["expanded"])}
         log("window %g: %.1f s, %d trace rows" % (d, time.time() - t0, tr.shape[0]))
     return out
+
+
+def fit_round_cost(runs):
+    """Least squares seconds ~ a * rounds + b * expanded + c over the
+    collection runs; returns a / b (ROUND_COST if the fit is degenerate)."""
+    ds = sorted(runs)
+    if len(ds) < 3:
+        return ROUND_COST
+    A = np.array([[runs[d]["batch_rounds"], runs[d]["batch_expanded"], 1.0] for d in ds], np.float64)
+    y = np.array([runs[d]["seconds"] for d in ds], np.float64)
+    coef = np.linalg.lstsq(A, y, rcond=None)[0]
+    if not (coef[0] > 0 and coef[1] > 0):
+        return ROUND_COST
+    return float(coef[0] / coef[1])
 
 
 def label_windows(E, R, round_cost=ROUND_COST):
@@ -107,11 +123,11 @@ def rows_to_samples(tr, window_of_query, nq):
     return xs, t, m
 
 
-def build_dataset(runs, batch, nq, max_rows=600_000, seed=0):
+def build_dataset(runs, batch, nq, max_rows=600_000, seed=0, round_cost=ROUND_COST):
     ds = sorted(runs)
     E = np.stack([runs[d]["expanded"] for d in ds], 1).astype(np.float64)
     R = np.stack([runs[d]["rounds"] for d in ds], 1).astype(np.float64)
-    best, budget, cost = label_windows(E, R)
+    best, budget, cost = label_windows(E, R, round_cost)
     log("round budget %d, modelled cost %.3g" % (budget, cost))
     X, T, Mk = [], [], []
     window_of_query = np.asarray(ds, np.float32)[best]
@@ -156,7 +172,9 @@ def main():
     log("training queries: %d (seed %d)" % (len(q), args.seed))
     dg = S.DeviceGraph(g)
     runs = collect(dg, q, args.batch)
-    X, T, Mk, best, cost = build_dataset(runs, args.batch, len(q))
+    round_cost = fit_round_cost(runs)
+    log("fitted round cost: %.0f label expansions per round" % round_cost)
+    X, T, Mk, best, cost = build_dataset(runs, args.batch, len(q), round_cost=round_cost)
     log("dataset: %d rows; best-window histogram %s" % (X.shape[0], np.bincount(best, minlength=len(CANDIDATES))))
     params, curve = train(X, T, Mk, epochs=args.epochs)
     # DAgger-style passes: the network's own trajectories, labelled with each
@@ -178,7 +196,7 @@ def main():
         params, curve = train(X, T, Mk, epochs=args.epochs, seed=it + 1)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.save(args.out, params)
-    meta = {"queries": len(q), "seed": args.seed, "candidates": list(CANDIDATES), "round_cost": ROUND_COST,
+    meta = {"queries": len(q), "seed": args.seed, "candidates": list(CANDIDATES), "round_cost": round_cost,
             "rows": int(X.shape[0]), "loss_curve": [float(c) for c in curve],
             "best_hist": [int(x) for x in np.bincount(best, minlength=len(CANDIDATES))],
             "window_seconds": {str(d): runs[d]["seconds"] for d in runs}}

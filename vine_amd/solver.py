@@ -80,7 +80,7 @@ class Stats(ctypes.Structure):
         ("near_carried", ctypes.c_int64),
         ("far_appended", ctypes.c_int64),
         ("scanned", ctypes.c_int64),
-        ("reserved", ctypes.c_int64 * 1),
+        ("timed_rounds", ctypes.c_int64),
     ]
 
     def as_dict(self):
