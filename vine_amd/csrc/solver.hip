@@ -470,7 +470,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   if (s->wg) {
     // per query two near lists; each holds at least the largest far chunk so
     // that a pull into an empty list always makes progress
-    wn_cap = std::max<long long>((long long)dev::FCH << dev::FCLS_MAX, std::min<long long>(carry_cap / B, 1LL << 22));
+    wn_cap = std::max<long long>((long long)dev::FU << dev::FCLS_MAX, std::min<long long>(carry_cap / B, 1LL << 22));
     ALLOC(s->d_wnear, sizeof(int4) * 2 * wn_cap * B);
     ALLOC(s->d_qfin, sizeof(unsigned long long) * B);
   }
@@ -487,11 +487,11 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   ALLOC(s->d_touched, sizeof(long long) * chunk_cap);
   ALLOC(s->d_carry[0], sizeof(int4) * carry_cap);
   ALLOC(s->d_carry[1], sizeof(int4) * carry_cap);
-  // far tier: blocks of dev::FCH records from a bump allocator (reset per
+  // far tier: units of dev::FU records from a bump allocator (reset per
   // batch), sized for four times the near tier's entries; one drain-list
-  // segment at most per block
-  long long far_cap = std::max<long long>(4 * carry_cap / dev::FCH, 4LL * B + 4096);
-  if (far_cap > (1LL << 27) - 1) far_cap = (1LL << 27) - 1;  // chunk pointers keep 27 bits of block
+  // segment at most per unit
+  long long far_cap = std::max<long long>(4 * carry_cap / dev::FU, 16LL * B + (1LL << dev::FCLS_MAX) + 4096);
+  if (far_cap > (1LL << 27) - 1) far_cap = (1LL << 27) - 1;  // chunk pointers keep 27 bits of unit
   ALLOC(s->d_far_head, sizeof(int32_t) * B * dev::NBK);
   ALLOC(s->d_far_occ, sizeof(unsigned long long) * B * dev::NBW);
   ALLOC(s->d_far_next, sizeof(int32_t) * B);
@@ -500,7 +500,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   ALLOC(s->d_far_bw, sizeof(float) * B);
   ALLOC(s->d_fhd, sizeof(int2) * far_cap);
   ALLOC(s->d_dl, sizeof(int2) * far_cap);
-  ALLOC(s->d_frec, sizeof(int4) * dev::FCH * far_cap);
+  ALLOC(s->d_frec, sizeof(int4) * dev::FU * far_cap);
   ALLOC(s->d_ts, sizeof(unsigned long long) * dev::TS_CAP);
   ALLOC(s->d_ct, sizeof(Counters));
   if (hipHostMalloc((void**)&s->h_ct, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
@@ -824,7 +824,7 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
       int par = h.round_e & 1;
       // near tier, new labels, far segments queued for the next expand, far tier
       long long fn = (long long)h.carry_n[par] + ((long long)h.label_ctr - h.lstart[par]) +
-                     (long long)h.drain_n * dev::FCH + h.far_live;
+                     (long long)h.drain_n * dev::FSEG + h.far_live;
       st->max_frontier = std::max<int64_t>(st->max_frontier, fn);
       fn_last = std::max<long long>(fn, 1);
       if (fn == 0) break;

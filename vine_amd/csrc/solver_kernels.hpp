@@ -39,8 +39,9 @@ constexpr float EXACT_LIMIT = 16777216.0f;  // 2^24
 constexpr int INIT_CAP = 4;                 // first Pareto-set allocation (entries)
 constexpr int TS_CAP = 1 << 20;             // per-round timestamps kept
 constexpr int CCHUNK = 8;                   // candidate records per list chunk
-constexpr int FCH = 64;                     // far-tier block: records per segment (one per lane)
-constexpr int FCLS_MAX = 10;                // far chunks grow 1, 2, 4 .. 1024 blocks along a bucket
+constexpr int FU = 16;                      // far-tier allocation unit (records)
+constexpr int FSEG = 64;                    // records per drain segment (one per lane)
+constexpr int FCLS_MAX = 12;                // far chunks grow 16, 32 .. 65,536 records along a bucket
 constexpr float FAR_MULT = 1.0f;            // near tier covers FAR_MULT windows past the front
 constexpr int NBK = 32768;                  // far-tier key buckets per query
 constexpr int NBW = NBK / 64;               // occupancy words per query
@@ -69,7 +70,7 @@ struct Counters {
   unsigned long long touched_total;  // (query, vertex) merges
   unsigned long long merge_scanned;  // Pareto entries compared in merge
   int merge_chain_max;               // longest candidate list merged by one lane
-  int far_chunks;                    // far-tier blocks handed out (bump)
+  int far_chunks;                    // far-tier units handed out (bump)
   int drain_n;                       // drain-list segments queued for the next expand
   long long far_live;                // records currently in far tiers
   unsigned long long far_pulled;     // records moved far -> near
@@ -123,10 +124,10 @@ struct Ctx {
   int32_t carry_cap;
   // far tier: per query NBK key buckets, bucket b holding the carried entries
   // with key in [base + b*bw, base + (b+1)*bw) (the last one also everything
-  // beyond) as a list of chunks.  A chunk is 2^cls blocks of FCH records
+  // beyond) as a list of chunks.  A chunk is 2^cls units of FU records
   // (cls grows by one along a bucket's list up to FCLS_MAX); a head or next
-  // pointer is (first block << 4) | cls, and the chunk header lives at
-  // fhd[first block].  Buckets below far_next are "pulled": their keys belong
+  // pointer is (first unit << 4) | cls, and the chunk header lives at
+  // fhd[first unit].  Buckets below far_next are "pulled": their keys belong
   // to the near tier.
   int32_t* far_head;  // [B * NBK]
   unsigned long long* far_occ;  // [B * NBW] bit b: bucket b holds a chunk
@@ -134,10 +135,10 @@ struct Ctx {
   int32_t* far_cnt;   // [B] entries in the query's far tier
   float* far_base;    // [B] smallest possible key of the query
   float* far_bw;      // [B] bucket width
-  int2* fhd;          // [far_cap] chunk header {count, next pointer}, at the chunk's first block
-  int4* frec;         // [far_cap * FCH] records
-  int32_t far_cap;    // blocks
-  int2* dl;           // [far_cap] drain list: segments {block, records} for the next expand
+  int2* fhd;          // [far_cap] chunk header {count, next pointer}, at the chunk's first unit
+  int4* frec;         // [far_cap * FU] records
+  int32_t far_cap;    // units
+  int2* dl;           // [far_cap] drain list: segments {first unit, records <= FSEG} for the next expand
   float far_floor;    // smallest near-tier reach (key units)
   int32_t* wq;       // [B][2][n] bound-search frontier queues
   int32_t* q_order;  // [B] bound-search dispatch order (null: slot order)
@@ -615,19 +616,19 @@ __device__ __forceinline__ void far_append(const Ctx& c, Counters* ct, int q, in
     const int blk = hv >> 4;
     cls = (hv & 15) + 1;
     const int slot = atomicAdd(&c.fhd[blk].x, 1);
-    if (slot < (FCH << (hv & 15))) {
-      c.frec[(long long)blk * FCH + slot] = crec4;
+    if (slot < (FU << (hv & 15))) {
+      c.frec[(long long)blk * FU + slot] = crec4;
       done = true;
     }
   }
   if (!done) {
     cls = min(cls, FCLS_MAX);
-    // blocks come from a bump allocator reset with each batch
+    // units come from a bump allocator reset with each batch
     const int blk = atomicAdd(&ct->far_chunks, 1 << cls);
     if (blk < 0 || blk > c.far_cap - (1 << cls)) {
       atomicOr(&ct->overflow, OVF_FAR);
     } else {
-      c.frec[(long long)blk * FCH] = crec4;
+      c.frec[(long long)blk * FU] = crec4;
       const int mine = (blk << 4) | cls;
       int expected = hv;
       for (;;) {
@@ -652,7 +653,7 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
   const int ncarry = min(ct->carry_n[cur], c.carry_cap);
   // input: near-tier carry records, then the far segments pull queued (64
   // lanes each, some empty), then the labels merge inserted last round
-  const long long nd64 = (long long)min(ct->drain_n, c.far_cap) * FCH;
+  const long long nd64 = (long long)min(ct->drain_n, c.far_cap) * FSEG;
   const long long nold = (long long)ncarry + nd64;
   const long long lstart = ct->lstart[cur];
   const long long lend = min((long long)ct->label_ctr, c.label_cap);
@@ -685,9 +686,9 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
           cr = c.carry[cur][i];
         } else {
           const long long j = i - ncarry;
-          const int2 sg = c.dl[j / FCH];  // {block, records}
-          const int l = (int)(j % FCH);
-          if (l < sg.y) cr = c.frec[(long long)sg.x * FCH + l];
+          const int2 sg = c.dl[j / FSEG];  // {first unit, records}
+          const int l = (int)(j % FSEG);
+          if (l < sg.y) cr = c.frec[(long long)sg.x * FU + l];
         }
         id = cr.x;
         q = cr.y;
@@ -1104,8 +1105,8 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
 // non-empty bucket is.  A far entry is therefore moved at most once, and
 // buckets whose keys the upper bound prunes are dropped without being read.
 // The pull itself only chases chunk headers (chunks double in size along a
-// bucket, so even a dense bucket is a short list) and queues each 64-record
-// block as a drain segment; the records are read by the next expand across
+// bucket, so even a dense bucket is a short list) and queues each chunk as
+// drain segments of up to 64 records; the records are read by the next expand across
 // the whole grid, where dead and pruned ones are dropped.  An occupancy
 // bitmap (one bit per bucket) finds the non-empty buckets of a key range
 // 4,096 buckets per load step.  One wave per query.
@@ -1195,13 +1196,13 @@ __device__ void pull_query(const Ctx& c, long long q) {
         while (hv >= 0 && guard++ < c.far_cap) {
           const int blk = hv >> 4;
           const int2 hd = c.fhd[blk];
-          const int cnt = min(hd.x, FCH << (hv & 15));
-          const int nseg = (cnt + FCH - 1) / FCH;
+          const int cnt = min(hd.x, FU << (hv & 15));
+          const int nseg = (cnt + FSEG - 1) / FSEG;
           int base = 0;
           if (lane == 0) base = atomicAdd(&ct->drain_n, nseg);
           base = __shfl(base, 0, WAVE);
           for (int k = lane; k < nseg; k += WAVE) {
-            if (base + k < c.far_cap) c.dl[base + k] = make_int2(blk + k, min(FCH, cnt - k * FCH));
+            if (base + k < c.far_cap) c.dl[base + k] = make_int2(blk + k * (FSEG / FU), min(FSEG, cnt - k * FSEG));
             else atomicOr(&ct->overflow, OVF_FAR);
           }
           queued += cnt;

@@ -150,14 +150,14 @@ __device__ void wg_pull(const Ctx& c, Counters* ct, int q, WgLds& S, int4* near_
       while (hv >= 0 && guard++ < c.far_cap) {
         const int blk = hv >> 4;
         const int2 hd = load_hdr(&c.fhd[blk]);
-        const int cnt = min(hd.x, FCH << (hv & 15));
+        const int cnt = min(hd.x, FU << (hv & 15));
         if (cnt > (int)c.wn_cap - min(S.n_near[cur], (int)c.wn_cap)) {
           // this chunk does not fit: the rest of the bucket stays far
           stop = true;
           break;
         }
         for (int k = lane; k < cnt; k += WAVE) {
-          const int4 rec = c.frec[(long long)blk * FCH + k];
+          const int4 rec = c.frec[(long long)blk * FU + k];
           const float key = __int_as_float(rec.z), rd = __int_as_float(rec.w);
           const bool live = !c.dead[rec.x] && !(PRUNE != 0 && lex_ge(key, rd, best.x, best.y));
           const int slot = wave_alloc32(&S.n_near[cur], live);
