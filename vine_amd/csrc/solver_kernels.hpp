@@ -921,6 +921,9 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
     if (have && gl == 0) c.cand_head[qv] = -1;
     // the front's last entry stays in registers across the candidates
     float2 lastv = (have && cnt > 0) ? c.pool_cd[off + cnt - 1] : make_float2(-1.f, FINF);
+    // smallest key and count of the labels this group inserts, published once
+    uint32_t gkey = __float_as_uint(FINF);
+    int gins = 0;
     int steps = 0;
     while (__ballot(h >= 0)) {
       if (h >= 0 && ++steps > max_steps) {
@@ -1006,8 +1009,11 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
           newcap = INIT_CAP;
           while (newcap < 2 * newcnt) newcap <<= 1;
         }
-        long long poff = wave_alloc64_n(&ct->pool_ctr, (grow && gl == 0) ? newcap : 0);
-        poff = __shfl(poff, gbase, WAVE);
+        long long poff = -1;
+        if (__ballot(grow && gl == 0)) {
+          poff = wave_alloc64_n(&ct->pool_ctr, (grow && gl == 0) ? newcap : 0);
+          poff = __shfl(poff, gbase, WAVE);
+        }
         const long long lid = rbase + used;
         bool fits = acc;
         if (grow && (poff < 0 || (unsigned long long)(poff + newcap) > c.pool_cap || poff + newcap > 0xFFFFFFFFll)) {
@@ -1071,23 +1077,26 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
                 c.q_best_id[q] = (int)lid;
               }
             }
-            if (!(PRUNE != 0 && v == tq) && __float_as_uint(key) < load_agent_u32(&fmin_nxt[q]))
-              atomicMin(&fmin_nxt[q], __float_as_uint(key));
+            if (!(PRUNE != 0 && v == tq)) gkey = min(gkey, __float_as_uint(key));
           }
+          gins++;
           if (start + 1 == newcnt) lastv = make_float2(cd, cdl);
           cnt = newcnt;
           used++;
-        }
-        if (c.stats_on) {
-          wave_add_keyed(c.q_ins, q, fits && gl == 0);
-          wave_add_keyed(c.q_near, q, fits && gl == 0);
         }
       }
       // reserved ids no candidate took: dead, so the next expand skips them
       if (gl >= used && gl < n8 && rbase + gl < c.label_cap) c.dead[rbase + gl] = 1;
       h = nexth;
     }
-    if (have && gl == 0) c.pset[qv] = pset_make(off, cnt, cap);
+    if (have && gl == 0) {
+      c.pset[qv] = pset_make(off, cnt, cap);
+      if (gkey < load_agent_u32(&fmin_nxt[q])) atomicMin(&fmin_nxt[q], gkey);
+      if (c.stats_on && gins) {
+        atomicAdd(&c.q_ins[q], gins);
+        atomicAdd(&c.q_near[q], gins);
+      }
+    }
     steps_max = max(steps_max, steps);
     wave_add(&ct->touched_total, (have && gl == 0) ? 1 : 0);
   }
