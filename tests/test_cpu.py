@@ -222,15 +222,3 @@ def test_locality_order_is_a_permutation_that_packs_neighbours():
     assert fn(4, row2.ctypes.data, dst2.ctypes.data, o2.ctypes.data) == 0
     assert sorted(o2.tolist()) == [0, 1, 2, 3]
 
-
-def test_round_cost_fit_recovers_known_costs():
-    """learn.fit_round_cost: seconds = a * rounds + b * expanded + c gives a / b."""
-    from vine_amd import learn as LR
-    a, b, c0 = 60e-6, 20e-9, 0.1
-    runs = {}
-    for d, (r, e) in {4.0: (27000, 16e6), 12.0: (9800, 30e6), 24.0: (6000, 54e6), 48.0: (3500, 90e6)}.items():
-        runs[d] = {"batch_rounds": r, "batch_expanded": e, "seconds": a * r + b * e + c0}
-    assert abs(LR.fit_round_cost(runs) - a / b) < 1e-6 * (a / b)
-    # a degenerate fit falls back to the default
-    runs2 = {d: dict(v, seconds=1.0) for d, v in runs.items()}
-    assert LR.fit_round_cost(runs2) == LR.ROUND_COST

@@ -7,9 +7,12 @@
 2. Each query is labelled with a window (label_windows): rounds are shared by
    the batch, so a round budget is chosen to minimise labels expanded +
    round_cost * rounds over the batch, and each query takes its least-work
-   window within the budget.  round_cost (how many label expansions one
-   round's fixed kernel time is worth) is fitted by least squares to the
-   collection runs themselves: seconds ~ a * rounds + b * expanded + c.  Every feature row recorded under that window
+   window within the budget.  ROUND_COST (how many label expansions one
+   round's fixed kernel time is worth) was chosen on queries of a separate
+   validation seed.  With --wide the heaviest queries (the ones that set the
+   batch's round count) are also run under wide windows, so the labels can
+   give them wide windows (measured: no gain on the NY workload, the network
+   widens too many queries).  Every feature row recorded under that window
    becomes a sample with target
    log2(window); the second output is trained to forecast log2(1 +
    expansions) TRACE_EVERY rounds ahead from the same rows.
@@ -32,10 +35,12 @@ from . import mlp as M
 from . import solver as S
 from . import workload as W
 
-CANDIDATES = (4.0, 8.0, 12.0, 16.0, 24.0, 32.0, 48.0, 64.0, 96.0)
-# fallback for round_cost when the fit is degenerate (labels a round's fixed
-# kernel time is worth on MI355X: ~63 us per round, ~19 ns per expansion)
-ROUND_COST = 3400.0
+CANDIDATES = (4.0, 8.0, 12.0, 16.0, 24.0, 32.0, 48.0)
+WIDE = (64.0, 96.0, 128.0, 192.0, 256.0)  # run on the heaviest queries only
+HEAVY_FRACTION = 0.01  # queries with the most rounds under the widest base window
+# labels one round's fixed kernel time is worth (chosen among 400, 800,
+# 1,150 and 1,800 on a validation query seed)
+ROUND_COST = 1150.0
 TRACE_EVERY = 4  # feature rows every 4th round
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(HERE, "models", "delta_mlp.npy")
@@ -68,18 +73,36 @@ def collect(dg, q, batch, candidates=CANDIDATES):
     return out
 
 
-def fit_round_cost(runs):
-    """Least squares seconds ~ a * rounds + b * expanded + c over the
-    collection runs; returns a / b (ROUND_COST if the fit is degenerate)."""
-    ds = sorted(runs)
-    if len(ds) < 3:
-        return ROUND_COST
-    A = np.array([[runs[d]["batch_rounds"], runs[d]["batch_expanded"], 1.0] for d in ds], np.float64)
-    y = np.array([runs[d]["seconds"] for d in ds], np.float64)
-    coef = np.linalg.lstsq(A, y, rcond=None)[0]
-    if not (coef[0] > 0 and coef[1] > 0):
-        return ROUND_COST
-    return float(coef[0] / coef[1])
+def collect_wide(dg, q, runs, windows=WIDE, fraction=HEAVY_FRACTION):
+    """The heaviest queries (most rounds under the widest base window) again
+    under wide windows; other queries get +inf work and rounds there, so the
+    labels never pick a wide window for them."""
+    base = runs[max(runs)]["rounds"]
+    k = max(1, int(len(q) * fraction))
+    heavy = np.sort(np.argsort(-base)[:k])
+    sub = q.subset(heavy)
+    log("heavy queries: %d, rounds under window %g: %d .. %d" % (k, max(runs), base[heavy].min(), base[heavy].max()))
+    for d in windows:
+        t0 = time.time()
+        ex = np.full(len(q), np.inf)
+        rd = np.full(len(q), np.inf)
+        try:
+            with solver_for(dg, len(sub), S.DELTA_FIXED, d, trace=TRACE_EVERY,
+                            label_capacity=1_500_000_000, pool_capacity=3_000_000_000) as s:
+                s.solve(sub.src, sub.dst, sub.bound, paths=False)
+                e2, r2 = s.query_stats(len(sub))
+                tr = s.trace()
+        except S.VineError as e:
+            log("window %g on the heavy queries: %s; skipped" % (d, e))
+            continue
+        ex[heavy] = e2
+        rd[heavy] = r2
+        if tr.shape[0]:
+            tr = tr.copy()
+            tr[:, 0] = heavy[tr[:, 0].astype(np.int64)]
+        runs[d] = {"expanded": ex, "rounds": rd, "trace": tr, "seconds": time.time() - t0}
+        log("window %g (heavy): %.1f s, rounds max %d, expanded %.3g" % (d, time.time() - t0, r2.max(), e2.sum()))
+    return runs
 
 
 def label_windows(E, R, round_cost=ROUND_COST):
@@ -166,20 +189,23 @@ def main():
     ap.add_argument("--seed", type=int, default=1000, help="query seed (evaluation uses 0)")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--dagger", type=int, default=2)
+    ap.add_argument("--wide", action="store_true", help="also offer wide windows to the heaviest queries")
     args = ap.parse_args()
     g = W.knn(264346, seed=0)
     q = W.hop_queries(g, args.queries, seed=args.seed)
     log("training queries: %d (seed %d)" % (len(q), args.seed))
     dg = S.DeviceGraph(g)
     runs = collect(dg, q, args.batch)
-    round_cost = fit_round_cost(runs)
-    log("fitted round cost: %.0f label expansions per round" % round_cost)
+    if args.wide:
+        runs = collect_wide(dg, q, runs)
+    round_cost = ROUND_COST
     X, T, Mk, best, cost = build_dataset(runs, args.batch, len(q), round_cost=round_cost)
-    log("dataset: %d rows; best-window histogram %s" % (X.shape[0], np.bincount(best, minlength=len(CANDIDATES))))
+    wins = sorted(runs)
+    log("dataset: %d rows; best-window histogram %s over %s" % (X.shape[0], np.bincount(best, minlength=len(wins)).tolist(), wins))
     params, curve = train(X, T, Mk, epochs=args.epochs)
     # DAgger-style passes: the network's own trajectories, labelled with each
     # query's window, so it also learns the states it reaches itself
-    window_of_query = np.asarray(CANDIDATES, np.float32)[best]
+    window_of_query = np.asarray(wins, np.float32)[best]
     for it in range(args.dagger):
         t0 = time.time()
         try:
@@ -196,9 +222,9 @@ def main():
         params, curve = train(X, T, Mk, epochs=args.epochs, seed=it + 1)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.save(args.out, params)
-    meta = {"queries": len(q), "seed": args.seed, "candidates": list(CANDIDATES), "round_cost": round_cost,
+    meta = {"queries": len(q), "seed": args.seed, "candidates": wins, "round_cost": round_cost,
             "rows": int(X.shape[0]), "loss_curve": [float(c) for c in curve],
-            "best_hist": [int(x) for x in np.bincount(best, minlength=len(CANDIDATES))],
+            "best_hist": [int(x) for x in np.bincount(best, minlength=len(wins))],
             "window_seconds": {str(d): runs[d]["seconds"] for d in runs}}
     with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
         json.dump(meta, f, indent=1)
