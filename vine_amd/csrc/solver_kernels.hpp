@@ -228,6 +228,40 @@ __device__ __forceinline__ void wave_add_keyed(int* arr, int key, bool pred) {
   }
 }
 
+// arr[key] += n summed over the lanes holding each key (one atomic per key).
+// Wave-uniform call.
+__device__ __forceinline__ void wave_add_keyed_n(int* arr, int key, int n, bool pred) {
+  unsigned long long todo = __ballot(pred);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int k = __shfl(key, leader, WAVE);
+    const bool mine = pred && key == k;
+    const unsigned long long same = __ballot(mine) & todo;
+    int v = mine ? n : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    if (lane_id() == leader && v) atomicAdd(&arr[k], v);
+    todo &= ~same;
+  }
+}
+
+// arr[key] = min(arr[key], val) over the lanes holding each key (one atomic
+// per key; vals are float bits of non-negative keys).  Wave-uniform call.
+__device__ __forceinline__ void wave_min_keyed(uint32_t* arr, int key, uint32_t val, bool pred) {
+  unsigned long long todo = __ballot(pred);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int k = __shfl(key, leader, WAVE);
+    const bool mine = pred && key == k;
+    const unsigned long long same = __ballot(mine) & todo;
+    uint32_t v = mine ? val : 0xFFFFFFFFu;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, WAVE));
+    if (lane_id() == leader) atomicMin(&arr[k], v);
+    todo &= ~same;
+  }
+}
+
 // Visit every (item, sub-item) pair of the wave, 64 pairs per step: lane i
 // holds an item with `deg` sub-items starting at `row`.  The wave scans the
 // degrees, then each lane finds its owner by binary search over the exclusive
@@ -606,8 +640,8 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
 // Append a carried entry to bucket fb of query q's far tier: lock-free on the
 // bucket's chunk list (as the candidate lists); a full head chunk is followed
 // by one twice its size.
+// The caller counts the entry in far_cnt[q].
 __device__ __forceinline__ void far_append(const Ctx& c, Counters* ct, int q, int fb, const int4& crec4) {
-  atomicAdd(&c.far_cnt[q], 1);
   int* headp = &c.far_head[(long long)q * NBK + fb];
   const int hv = __hip_atomic_load(headp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   bool done = false;
@@ -753,13 +787,12 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
     const int4 crec4 = make_int4(id, q, __float_as_int(key), __float_as_int(rd));
     if (c.stats_on) wave_add_keyed(c.q_near, q, near);
     if (near) {
-      if (cslot < c.carry_cap) {
-        c.carry[nxt][cslot] = crec4;
-        if (__float_as_uint(key) < load_agent_u32(&fmin_nxt[q])) atomicMin(&fmin_nxt[q], __float_as_uint(key));
-      } else {
-        atomicOr(&ct->overflow, OVF_CARRY);
-      }
+      if (cslot < c.carry_cap) c.carry[nxt][cslot] = crec4;
+      else atomicOr(&ct->overflow, OVF_CARRY);
     }
+    // one atomic per query of the wave for the front key and the far count
+    wave_min_keyed(fmin_nxt, q, __float_as_uint(key), near);
+    wave_add_keyed(c.far_cnt, q, far);
     if (far) far_append(c, ct, q, fb, crec4);
     {
       int nf = far ? 1 : 0;
@@ -1089,13 +1122,13 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
       if (gl >= used && gl < n8 && rbase + gl < c.label_cap) c.dead[rbase + gl] = 1;
       h = nexth;
     }
-    if (have && gl == 0) {
-      c.pset[qv] = pset_make(off, cnt, cap);
-      if (gkey < load_agent_u32(&fmin_nxt[q])) atomicMin(&fmin_nxt[q], gkey);
-      if (c.stats_on && gins) {
-        atomicAdd(&c.q_ins[q], gins);
-        atomicAdd(&c.q_near[q], gins);
-      }
+    if (have && gl == 0) c.pset[qv] = pset_make(off, cnt, cap);
+    // the groups of a wave often belong to one query: one atomic per query
+    const bool lead = have && gl == 0;
+    wave_min_keyed(fmin_nxt, q, gkey, lead && gkey != __float_as_uint(FINF));
+    if (c.stats_on) {
+      wave_add_keyed_n(c.q_ins, q, gins, lead && gins > 0);
+      wave_add_keyed_n(c.q_near, q, gins, lead && gins > 0);
     }
     steps_max = max(steps_max, steps);
     wave_add(&ct->touched_total, (have && gl == 0) ? 1 : 0);

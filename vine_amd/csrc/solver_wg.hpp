@@ -85,7 +85,10 @@ __device__ __forceinline__ void wg_carry(const Ctx& c, Counters* ct, int q, WgLd
     }
   }
   if (pred && to_far) {
-    if (nfn < NBK) far_append(c, ct, q, fb, rec);
+    if (nfn < NBK) {
+      atomicAdd(&c.far_cnt[q], 1);
+      far_append(c, ct, q, fb, rec);
+    }
     else atomicOr(&ct->overflow, OVF_CARRY);
   }
 }
