@@ -83,8 +83,8 @@ typedef struct {
   float delta_fixed;        /* window for VINE_DELTA_FIXED */
   int32_t use_graph;        /* reserved, must be 0 (hipGraph capture of rounds is not built) */
   int32_t rounds_per_sync;  /* rounds between host checks of the frontier size */
-  int32_t block_size;       /* threads per workgroup (multiple of 64) */
-  int32_t grid_blocks;      /* workgroups per round kernel (0 = auto) */
+  int32_t block_size;       /* threads per workgroup of the round kernels (64, 128, 192 or 256) */
+  int32_t grid_blocks;      /* workgroups per round kernel at most (0 = 384) */
   int32_t hash_slots;       /* reserved, must be 0 (no LDS candidate table is built) */
   int64_t label_capacity;   /* label log entries (0 = auto from free memory) */
   int64_t cand_capacity;    /* candidate slots per round (0 = auto); half as many list chunks are allocated */
@@ -99,7 +99,9 @@ typedef struct {
                                searches only inside fwd + lbd <= bound; 1: reverse searches over
                                the whole ball lbd <= bound (no forward search) */
   int32_t engine;           /* VINE_ENGINE_* */
-  int32_t reserved[3];
+  int32_t labels_per_wave;  /* frontier entries a wave takes per step in expand (power of two <= 64;
+                               0 = 64); fewer spread a heavy round's arcs over more waves */
+  int32_t reserved[2];
 } vine_options;
 
 typedef struct {

@@ -56,7 +56,7 @@ def _struct_fields(name):
 
 def test_options_layout_matches_header():
     o = S.default_options()
-    assert o.max_batch == 1 and o.prune == S.PRUNE_BOUNDS and o.block_size == 256
+    assert o.max_batch == 1 and o.prune == S.PRUNE_BOUNDS and o.block_size == 128
     for cname, pystruct in (("vine_options", S.Options), ("vine_stats", S.Stats)):
         hdr = _struct_fields(cname)
         py = [(f[0], f[1]) for f in pystruct._fields_]

@@ -113,6 +113,7 @@ struct vine_solver {
   int4* d_wnear = nullptr;
   unsigned long long* d_qfin = nullptr;
   bool wg = false;  // workgroup engine
+  int lpw = 64;     // frontier entries per wave step in expand
   // pools
   float2* d_pool_cd = nullptr;
   int32_t* d_pool_id = nullptr;
@@ -198,7 +199,7 @@ void vine_options_default(vine_options* o) {
   o->delta_fixed = 0.f;
   o->use_graph = 0;
   o->rounds_per_sync = 1;
-  o->block_size = 256;
+  o->block_size = 128;
   o->grid_blocks = 0;
   o->hash_slots = 0;
   o->ar_max_labels = 0.0;
@@ -354,9 +355,13 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   if (opt) o = *opt;
   else vine_options_default(&o);
   if (o.max_batch < 1 || o.prune < 0 || o.prune > 2 || o.delta_policy < 0 || o.delta_policy > 3 ||
-      o.block_size < 64 || o.block_size > 1024 || o.block_size % 64 || o.rounds_per_sync < 1 ||
+      o.block_size < 64 || o.block_size > 256 || o.block_size % 64 || o.rounds_per_sync < 1 ||
       o.grid_blocks < 0 || o.hash_slots < 0 || !(o.delta_fixed >= 0.f) || o.bound_ball < 0 || o.bound_ball > 1) {
     set_err("vine_solver_create: invalid options");
+    return VINE_E_ARG;
+  }
+  if (o.labels_per_wave < 0 || o.labels_per_wave > 64 || (o.labels_per_wave & (o.labels_per_wave - 1))) {
+    set_err("vine_solver_create: labels_per_wave must be 0 or a power of two <= 64");
     return VINE_E_ARG;
   }
   if (o.engine < 0 || o.engine > 2) {
@@ -383,8 +388,11 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   s->opt = o;
   s->B = o.max_batch;
   s->block = o.block_size;
-  s->grid = o.grid_blocks ? o.grid_blocks : 1024;
+  // about 768 waves: the round kernels are latency-bound, and measured on
+  // MI355X fewer, fuller waves beat a grid that fills every CU
+  s->grid = o.grid_blocks ? o.grid_blocks : 384;
   s->wg = wg_ok && o.engine == VINE_ENGINE_WORKGROUP;
+  s->lpw = o.labels_per_wave ? o.labels_per_wave : 64;
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     set_err("hipStreamCreate: %s", hipGetErrorString(e));
@@ -656,7 +664,7 @@ vine_status launch_round(vine_solver* s, long long dop, int gsz, vine_stats* st)
   {
     Timer t(s, 0, sample);
     hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, PRUNE>), dim3(gsz),
-                       dim3(s->block), 0, s->stream, c, dop);
+                       dim3(s->block), 0, s->stream, c, dop, s->lpw);
     HIP_TRY(hipGetLastError());
     HIP_TRY(t.stop());
   }

@@ -208,6 +208,30 @@ __device__ __forceinline__ long long wave_alloc64_n(unsigned long long* ctr, int
   return (long long)base + (incl - size);
 }
 
+// Workgroup sums of per-thread counts, one atomic per counter and workgroup
+// (the statistics counters are shared by every workgroup of the grid, so
+// per-wave atomics on them serialise).  Every thread must call it: it
+// synchronises the workgroup.  Blocks have at most 256 threads.
+template <int N>
+__device__ __forceinline__ void block_add(unsigned long long* const (&ctr)[N], const long long (&v)[N]) {
+  __shared__ long long red[N][4];
+  const int lane = (int)__lane_id();
+  const int wid = threadIdx.x / 64;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    long long x = v[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) red[k][wid] = x;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < N) {
+    long long sum = 0;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; w++) sum += red[threadIdx.x][w];
+    if (sum) atomicAdd(ctr[threadIdx.x], (unsigned long long)sum);
+  }
+}
+
 __device__ __forceinline__ void wave_add(unsigned long long* ctr, int v) {
   int lane = lane_id();
 #pragma unroll
@@ -679,7 +703,7 @@ __device__ __forceinline__ void far_append(const Ctx& c, Counters* ct, int q, in
 }
 
 template <typename Label_t, typename Constraint_t, typename Edge_t, int PRUNE>
-__global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
+__global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop, int lpw) {
   using LT = label_traits<Label_t>;
   Counters* ct = c.ct;
   const int r = ct->round_e;
@@ -699,13 +723,15 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
   }
   const int lane = lane_id();
   const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
-  int n_exp = 0, n_rel = 0, n_cand = 0;
+  int n_exp = 0, n_rel = 0, n_cand = 0, n_far = 0, n_near = 0, n_scan = 0;
   const uint32_t* fmin_cur = c.q_fmin + (size_t)cur * c.B;
   uint32_t* fmin_nxt = c.q_fmin + (size_t)nxt * c.B;
 
-  for (long long base = ((long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * WAVE;
-       base < fn; base += nwaves * WAVE) {
-    long long i = base + lane;
+  // lpw frontier entries per wave step (lanes >= lpw only help with the arcs):
+  // fewer entries per wave spread a heavy round's arcs over more waves
+  for (long long base = ((long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * lpw;
+       base < fn; base += nwaves * lpw) {
+    const long long i = lane < lpw ? base + lane : fn;
     int id = -1, deg = 0, row = 0, q = 0;
     Label_t lab;
     bool carry = false, expd = false;
@@ -794,17 +820,9 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
     wave_min_keyed(fmin_nxt, q, __float_as_uint(key), near);
     wave_add_keyed(c.far_cnt, q, far);
     if (far) far_append(c, ct, q, fb, crec4);
-    {
-      int nf = far ? 1 : 0;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) nf += __shfl_xor(nf, o, WAVE);
-      if (lane == 0 && nf) {
-        atomicAdd((unsigned long long*)&ct->far_live, (unsigned long long)nf);
-        atomicAdd(&ct->far_appended, (unsigned long long)nf);
-      }
-      wave_add(&ct->near_carried, near ? 1 : 0);
-      wave_add(&ct->scanned, i < fn ? 1 : 0);
-    }
+    n_far += far ? 1 : 0;
+    n_near += near ? 1 : 0;
+    n_scan += i < fn ? 1 : 0;
     float ld = lab.d, ldl = lab.delay;
     wave_flat(deg, row, [&](int owner, int pos, bool act) {
       Label_t src = LT::unpack(0, make_float2(__shfl(ld, owner, WAVE), __shfl(ldl, owner, WAVE)));
@@ -883,9 +901,10 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop) {
       }
     });
   }
-  wave_add(&ct->expanded, n_exp);
-  wave_add(&ct->relaxed, n_rel);
-  wave_add(&ct->cands, n_cand);
+  unsigned long long* const ctrs[7] = {&ct->expanded, &ct->relaxed,      &ct->cands,  (unsigned long long*)&ct->far_live,
+                                       &ct->far_appended, &ct->near_carried, &ct->scanned};
+  const long long vals[7] = {n_exp, n_rel, n_cand, n_far, n_far, n_near, n_scan};
+  block_add<7>(ctrs, vals);
 }
 
 // ---------------------------------------------------------------------------
@@ -929,7 +948,7 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
   const unsigned long long gmask = ((1ull << MG) - 1ull) << gbase;
   constexpr int GPW = WAVE / MG;  // groups per wave
   const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
-  int n_ins = 0, scanned = 0, steps_max = 0;
+  int n_ins = 0, scanned = 0, steps_max = 0, n_touch = 0;
 
   for (long long base = ((long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * GPW;
        base < tn; base += nwaves * GPW) {
@@ -1131,13 +1150,15 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
       wave_add_keyed_n(c.q_near, q, gins, lead && gins > 0);
     }
     steps_max = max(steps_max, steps);
-    wave_add(&ct->touched_total, (have && gl == 0) ? 1 : 0);
+    n_touch += (have && gl == 0) ? 1 : 0;
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) steps_max = max(steps_max, __shfl_xor(steps_max, o, WAVE));
-  if (lane == 0 && steps_max > 0) atomicMax(&ct->merge_chain_max, steps_max);
-  wave_add(&ct->merge_scanned, scanned);
-  wave_add(&ct->inserted, n_ins);
+  if (lane == 0 && steps_max > 0 && steps_max > load_agent_u32((const uint32_t*)&ct->merge_chain_max))
+    atomicMax(&ct->merge_chain_max, steps_max);
+  unsigned long long* const ctrs[3] = {&ct->touched_total, &ct->merge_scanned, &ct->inserted};
+  const long long vals[3] = {n_touch, scanned, n_ins};
+  block_add<3>(ctrs, vals);
 }
 
 // ---------------------------------------------------------------------------

@@ -49,7 +49,8 @@ class Options(ctypes.Structure):
         ("bound_delta", ctypes.c_float),
         ("bound_ball", ctypes.c_int32),
         ("engine", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("labels_per_wave", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
