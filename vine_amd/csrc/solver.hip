@@ -95,8 +95,6 @@ struct vine_solver {
   int32_t* d_qbest_id = nullptr;
   uint32_t* d_qfmin = nullptr;
   int32_t* d_qlast = nullptr;
-  float* d_qfmax = nullptr;
-  int32_t* d_qnfront = nullptr;
   int32_t *d_qexp = nullptr, *d_qins = nullptr, *d_qnear = nullptr;
   long long* d_qexptot = nullptr;
   float* d_qfcast = nullptr;
@@ -454,8 +452,6 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   ALLOC(s->d_qbest_id, sizeof(int32_t) * B);
   ALLOC(s->d_qfmin, sizeof(uint32_t) * 2 * B);
   ALLOC(s->d_qlast, sizeof(int32_t) * B);
-  ALLOC(s->d_qfmax, sizeof(float) * B);
-  ALLOC(s->d_qnfront, sizeof(int32_t) * B);
   ALLOC(s->d_qexp, sizeof(int32_t) * B);
   ALLOC(s->d_qins, sizeof(int32_t) * B);
   ALLOC(s->d_qnear, sizeof(int32_t) * B);
@@ -532,8 +528,6 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   c.q_best_id = s->d_qbest_id;
   c.q_fmin = s->d_qfmin;
   c.q_last = s->d_qlast;
-  c.q_fmax = s->d_qfmax;
-  c.q_nfront = s->d_qnfront;
   c.q_exp = s->d_qexp;
   c.q_ins = s->d_qins;
   c.q_near = s->d_qnear;
@@ -582,7 +576,6 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   c.far_floor = 0.0625f * g->mean_len;
   c.ts = s->d_ts;
   c.ct = s->d_ct;
-  c.hash_slots = o.hash_slots;
   c.ellipse = o.bound_ball ? 0 : 1;
   *out = s;
   return VINE_OK;
@@ -656,7 +649,7 @@ vine_status poll(vine_solver* s) {
 }
 
 template <int PRUNE>
-vine_status launch_round(vine_solver* s, long long dop, int gsz, vine_stats* st) {
+vine_status launch_round(vine_solver* s, int gsz, vine_stats* st) {
   Ctx& c = s->ctx;
   const int tk = std::max(1, s->opt.time_kernels);
   const bool sample = s->opt.time_kernels != 0 && (s->round_seq++ % tk) == 0;
@@ -664,7 +657,7 @@ vine_status launch_round(vine_solver* s, long long dop, int gsz, vine_stats* st)
   {
     Timer t(s, 0, sample);
     hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, PRUNE>), dim3(gsz),
-                       dim3(s->block), 0, s->stream, c, dop, s->lpw);
+                       dim3(s->block), 0, s->stream, c, s->lpw);
     HIP_TRY(hipGetLastError());
     HIP_TRY(t.stop());
   }
@@ -686,14 +679,14 @@ vine_status launch_round(vine_solver* s, long long dop, int gsz, vine_stats* st)
   return VINE_OK;
 }
 
-vine_status launch_round_any(vine_solver* s, long long dop, int gsz, vine_stats* st) {
+vine_status launch_round_any(vine_solver* s, int gsz, vine_stats* st) {
   switch (s->opt.prune) {
     case 0:
-      return launch_round<0>(s, dop, gsz, st);
+      return launch_round<0>(s, gsz, st);
     case 1:
-      return launch_round<1>(s, dop, gsz, st);
+      return launch_round<1>(s, gsz, st);
     default:
-      return launch_round<2>(s, dop, gsz, st);
+      return launch_round<2>(s, gsz, st);
   }
 }
 
@@ -822,7 +815,7 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
       for (int k = 0; k < K; k++) {
         // the frontier may grow between polls: size the grid for 4x growth
         // (grid-stride loops keep any size correct)
-        vine_status ls = launch_round_any(s, 0, grid_for(s, fn_last * 4), st);
+        vine_status ls = launch_round_any(s, grid_for(s, fn_last * 4), st);
         if (ls != VINE_OK) return ls;
         rounds++;
       }

@@ -60,11 +60,8 @@ struct Counters {
   int round_e;           // round number read by expand
   int round_m;           // round number read by merge (written by expand)
   int sround;            // most bound-search rounds of one query (all searches)
-  int pad0[3];
-  unsigned int blocks_done;
   int overflow;
   int range_flag;
-  int pad;
   unsigned long long t_start;  // device wall clock at batch start
   unsigned long long expanded, relaxed, cands, inserted, sssp_relaxed;
   unsigned long long touched_total;  // (query, vertex) merges
@@ -96,8 +93,6 @@ struct Ctx {
   int32_t* q_best_id;
   uint32_t* q_fmin;  // [2][B] float bits
   int32_t* q_last;   // last round with live frontier labels
-  float* q_fmax;     // [B] max key among expanded labels this round (stats)
-  int32_t* q_nfront; // [B] live frontier labels this round (stats)
   // per (query, vertex)
   int2* pset;        // {pool offset, cnt | capcode << 26}
   int32_t* cand_head;
@@ -147,7 +142,6 @@ struct Ctx {
   unsigned long long* q_fin;  // [B] device clock when the query's search ended (workgroup engine)
   unsigned long long* ts;  // [TS_CAP] per-round merge-end wall clock
   Counters* ct;
-  int32_t hash_slots;
   int32_t ellipse;  // reverse bound searches restricted to fwd + dist <= bound (fwd in lbl)
   // scheduler: per-query round statistics, MLP, feature trace
   int32_t stats_on;
@@ -431,8 +425,6 @@ __global__ void k_init_batch(Ctx c, int nq) {
   }
   c.q_fmin[c.B + q] = __float_as_uint(FINF);
   c.q_last[q] = -1;
-  c.q_fmax[q] = 0.f;
-  c.q_nfront[q] = 0;
   c.q_exp[q] = 0;
   c.q_ins[q] = 0;
   c.q_near[q] = 0;
@@ -658,9 +650,6 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
 }
 
 // ---------------------------------------------------------------------------
-// Expand.  PRUNE: VINE_PRUNE_* ; policy_count > 0 applies the reference's DOP
-// count window (expand the last DOP frontier entries, carry the rest,
-// algorithm/cspp.cuh:179-216).
 // Append a carried entry to bucket fb of query q's far tier: lock-free on the
 // bucket's chunk list (as the candidate lists); a full head chunk is followed
 // by one twice its size.
@@ -702,8 +691,13 @@ __device__ __forceinline__ void far_append(const Ctx& c, Counters* ct, int q, in
   }
 }
 
+// Expand (PRUNE: VINE_PRUNE_*).  Every frontier entry is checked; the live
+// ones with key <= the query's smallest key + window are expanded, the rest
+// carried to the near or far tier.  The window plays the role of the
+// reference's DOP count (expand the last DOP entries, carry the rest,
+// algorithm/cspp.cuh:179-216) in key units.
 template <typename Label_t, typename Constraint_t, typename Edge_t, int PRUNE>
-__global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop, int lpw) {
+__global__ void __launch_bounds__(256) k_expand(Ctx c, int lpw) {
   using LT = label_traits<Label_t>;
   Counters* ct = c.ct;
   const int r = ct->round_e;
@@ -779,9 +773,7 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop, int lpw) {
         }
         if (live) {
           c.q_last[q] = r;
-          bool in_window;
-          if (dop > 0) in_window = (i >= fn - dop);
-          else in_window = !(key > __uint_as_float(fmin_cur[q]) + c.q_delta[q]);
+          const bool in_window = !(key > __uint_as_float(fmin_cur[q]) + c.q_delta[q]);
           if (in_window) {
             if (!fresh) {
               int4 L = c.labs[id];
@@ -804,7 +796,7 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, long long dop, int lpw) {
     // else the query's far tier (not scanned again until pulled)
     bool far = false;
     int fb = 0;
-    if (carry && dop <= 0) {
+    if (carry) {
       fb = far_bucket(c, q, key);
       far = fb >= c.far_next[q];
     }
