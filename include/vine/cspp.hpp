@@ -23,6 +23,7 @@
 
 #include "vine.h"
 #include "vine/label.hpp"
+#include "vine_workload.h"
 
 namespace vine {
 
@@ -149,6 +150,72 @@ class cspp {
     return st;
   }
   Graph<Edge_t>* g_;
+};
+
+// read_graph (include/io.hpp:22-91): replaces *g with the graph in file_name.
+// graph_type 2 = COLA road network (arcs as listed), 1 = SNAP edge list
+// (undirected; the reference's "N?M?" layout or the shipped MatrixMarket
+// files).  Returns a vine_status (the reference returns 0 and crashes on a
+// missing file); the message is in vine_last_error().
+template <typename Edge_t>
+int read_graph(const char* file_name, Graph<Edge_t>* g, int graph_type) {
+  vine_host_graph* h = nullptr;
+  const int st = vine_read_graph(file_name, graph_type, &h);
+  if (st != 0) return st;
+  const int32_t n = vine_host_graph_n(h);
+  const int32_t* row = vine_host_graph_row_ptr(h);
+  const int32_t* dst = vine_host_graph_dst(h);
+  const float* len = vine_host_graph_len(h);
+  const float* delay = vine_host_graph_delay(h);
+  *g = Graph<Edge_t>(n);
+  for (int32_t u = 0; u < n; u++)
+    for (int32_t p = row[u]; p < row[u + 1]; p++) {
+      Edge_t e;
+      e.len = len[p];
+      e.delay = delay[p];
+      g->add_edge(u, dst[p], e);
+    }
+  vine_host_graph_free(h);
+  return VINE_OK;
+}
+
+// task (include/task.hpp:9-79): the query list of a run.  read_query with
+// graph_type 2 (COLA rows "u v lower upper limit") or 1 (a header line, then
+// "id,src,snk,delay,hop,cost"); load_query is the COLA form.  delay_l_ holds
+// the delay bounds; hop_l_ and cost_l_ are filled by the CSV form.
+class task {
+ public:
+  task() = default;
+  explicit task(int n) : ntask(n) {}
+
+  int read_query(const char* file_name, int graph_type) {
+    const int64_t n = vine_read_queries(file_name, graph_type, 0, nullptr, nullptr, nullptr, nullptr);
+    if (n < 0) return (int)n;
+    std::vector<int32_t> src((size_t)n), dst((size_t)n), aux(2 * (size_t)n);
+    std::vector<float> bound((size_t)n);
+    const int64_t got = vine_read_queries(file_name, graph_type, n, src.data(), dst.data(), bound.data(),
+                                          aux.data());
+    if (got != n) return got < 0 ? (int)got : VINE_E_ARG;
+    for (int64_t i = 0; i < n; i++) {
+      source_l_.push_back(src[i]);
+      sink_l_.push_back(dst[i]);
+      delay_l_.push_back((int)bound[i]);
+      if (graph_type == VINE_FILE_SNAP) {
+        hop_l_.push_back(aux[2 * i]);
+        cost_l_.push_back(aux[2 * i + 1]);
+      }
+    }
+    ntask = (int)source_l_.size();
+    return VINE_OK;
+  }
+  int load_query(const char* file_name) { return read_query(file_name, VINE_FILE_COLA); }
+
+  int ntask = 0;
+  std::vector<int> source_l_;
+  std::vector<int> sink_l_;
+  std::vector<int> delay_l_;
+  std::vector<int> hop_l_;
+  std::vector<int> cost_l_;
 };
 
 }  // namespace vine

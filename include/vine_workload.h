@@ -89,6 +89,35 @@ int vine_gen_queries_hops(const vine_host_graph* g, int32_t nq, int32_t hop_lo,
  * vertex 3-NN geometric graph, 3 = 1,524,453 vertex 3-NN, 4 = R-MAT scale 17. */
 int vine_gen_preset_graph(int32_t id, uint64_t seed, vine_host_graph** out);
 
+/* ---- Graph and query files ------------------------------------------------
+ * The input formats of the reference's read_graph(file, g, graph_type)
+ * (include/io.hpp:22-91) and task::read_query(file, graph_type) /
+ * task::load_query(file) (include/task.hpp:27-76); the format numbers are the
+ * reference's graph_type values.  Vertex ids are 0-based, weights and bounds
+ * integers in [0, 2^24).  Errors return a negative vine_status (VINE_E_ARG for
+ * an unreadable file or bad content) with file:line in vine_last_error(). */
+#define VINE_FILE_SNAP 1 /* SNAP edge list, undirected */
+#define VINE_FILE_COLA 2 /* COLA road network, directed arcs */
+
+/* Graph.  VINE_FILE_COLA: the vertex count, then lines "u: v w c v w c ... -1",
+ *   each triple an arc u->v with len w and delay c, kept directed as listed.
+ * VINE_FILE_SNAP: each row is an undirected edge, stored as both arcs with the
+ *   same (len, delay).  Either the reference's layout ("N?M?": two counts each
+ *   followed by one separator character, then M rows "u v w c") or the
+ *   MatrixMarket one of the shipped inputs/tech-*.mtx.txt ('%' comment lines,
+ *   "N N M", then M rows "id u v w c"), told apart by a leading '%'.
+ * The CSR keeps the file's arc order per vertex (adjacency insertion order). */
+int vine_read_graph(const char* path, int32_t format, vine_host_graph** out);
+
+/* Queries.  VINE_FILE_COLA: rows "u v lower upper limit", bound = limit,
+ *   aux = (lower, upper).  VINE_FILE_SNAP: a header line, then rows
+ *   "id,src,snk,delay,hop,cost", bound = delay, aux = (hop, cost).
+ * Returns the number of queries in the file and writes the first min(count,
+ * cap) of them (out_aux: 2 per query, may be NULL; all outputs may be NULL when
+ * cap = 0).  Vertex ids are checked against a graph only when solved. */
+int64_t vine_read_queries(const char* path, int32_t format, int64_t cap, int32_t* out_src,
+                          int32_t* out_dst, float* out_bound, int32_t* out_aux);
+
 #ifdef __cplusplus
 }
 #endif

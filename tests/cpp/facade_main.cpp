@@ -1,13 +1,53 @@
 // Builds a 4x4 grid through the C++ facade (include/vine/cspp.hpp), solves a
 // few tasks and prints one line per task: feasible cost delay path...
-// With argument "args" it only checks the argument errors (no GPU needed).
+// With argument "args" it only checks the argument errors (no GPU needed);
+// with "files ..." it reads a graph and a query file (see run_files).
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "vine/cspp.hpp"
 
+// files <graph> <graph_type> <queries> <graph_type> [solve]: reads both files
+// through read_graph / task::read_query, prints "graph N M" and "tasks n",
+// then with "solve" one line per task as above (path omitted).
+static int run_files(char** argv, bool solve) {
+  using namespace vine;
+  Graph<c_edge> g(1);
+  int st = read_graph(argv[2], &g, std::atoi(argv[3]));
+  if (st != VINE_OK) {
+    std::printf("error %d %s\n", st, vine_last_error());
+    return 2;
+  }
+  task tasks;
+  st = tasks.read_query(argv[4], std::atoi(argv[5]));
+  if (st != VINE_OK) {
+    std::printf("error %d %s\n", st, vine_last_error());
+    return 2;
+  }
+  std::printf("graph %d %lld\ntasks %d\n", g.N, (long long)g.M, tasks.ntask);
+  if (!solve) return 0;
+  std::vector<c_label> src;
+  std::vector<c_constraint<c_label>> C;
+  for (int i = 0; i < tasks.ntask; i++) {
+    src.push_back(c_label(tasks.source_l_[i]));
+    C.push_back(c_constraint<c_label>((float)tasks.delay_l_[i]));
+  }
+  cspp<c_label, c_edge> solver(&g, tasks.ntask, src, tasks.sink_l_, C);
+  st = solver.solve();
+  if (st != VINE_OK) {
+    std::printf("error %d %s\n", st, solver.last_error.c_str());
+    return 2;
+  }
+  for (int i = 0; i < tasks.ntask; i++)
+    std::printf("%d %.0f %.0f\n", solver.feasible_l[i], solver.cost_l[i], solver.delay_l[i]);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   using namespace vine;
+  if (argc >= 6 && std::strcmp(argv[1], "files") == 0)
+    return run_files(argv, argc > 6 && std::strcmp(argv[6], "solve") == 0);
   Graph<c_edge> g(16);
   for (int r = 0; r < 4; r++)
     for (int c = 0; c < 4; c++) {

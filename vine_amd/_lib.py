@@ -57,5 +57,8 @@ def _declare(L):
     _sig(L, "vine_gen_queries_hops", c_i32,
          [c_vp, c_i32, c_i32, c_i32, c_i32, c_i32, c_u64, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp])
     _sig(L, "vine_gen_preset_graph", c_i32, [c_i32, c_u64, pp])
+    _sig(L, "vine_read_graph", c_i32, [ctypes.c_char_p, c_i32, pp])
+    _sig(L, "vine_read_queries", c_i64, [ctypes.c_char_p, c_i32, c_i64, c_vp, c_vp, c_vp, c_vp])
+    _sig(L, "vine_last_error", ctypes.c_char_p, [])
     if hasattr(L, "vine_declare_solver"):
         pass

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "mlp.hpp"
+#include "errors.hpp"
 #include "layout.hpp"
 #include "sched.hpp"
 #include "solver_kernels.hpp"
@@ -34,22 +35,13 @@ using vine::dev::Ctx;
 
 namespace {
 
-thread_local std::string g_err;
-
-void set_err(const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
-  g_err = buf;
-}
+using vine::set_error;
 
 #define HIP_TRY(expr)                                                              \
   do {                                                                             \
     hipError_t e_ = (expr);                                                        \
     if (e_ != hipSuccess) {                                                        \
-      set_err("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
       return VINE_E_HIP;                                                           \
     }                                                                              \
   } while (0)
@@ -169,7 +161,7 @@ static vine_status dev_alloc(vine_solver* s, void** p, size_t bytes) {
   if (bytes == 0) bytes = 16;
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
-    set_err("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
     (void)hipGetLastError();
     return VINE_E_OOM;
   }
@@ -203,13 +195,11 @@ void vine_options_default(vine_options* o) {
   o->ar_max_labels = 0.0;
 }
 
-const char* vine_last_error(void) { return g_err.c_str(); }
-
 int32_t vine_device_count(void) {
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
   if (e != hipSuccess) {
-    set_err("hipGetDeviceCount: %s", hipGetErrorString(e));
+    set_error("hipGetDeviceCount: %s", hipGetErrorString(e));
     return VINE_E_HIP;
   }
   return n;
@@ -218,29 +208,29 @@ int32_t vine_device_count(void) {
 vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, const int32_t* dst,
                               const float* len, const float* delay, vine_graph** out) {
   if (!out || n <= 0 || m < 0 || !row_ptr || (m > 0 && (!dst || !len || !delay))) {
-    set_err("vine_graph_create: invalid arguments");
+    set_error("vine_graph_create: invalid arguments");
     return VINE_E_ARG;
   }
   if (m >= (int64_t)INT32_MAX) {
-    set_err("vine_graph_create: %lld arcs exceed the 32-bit CSR offsets", (long long)m);
+    set_error("vine_graph_create: %lld arcs exceed the 32-bit CSR offsets", (long long)m);
     return VINE_E_ARG;
   }
   if (row_ptr[0] != 0 || row_ptr[n] != m) {
-    set_err("vine_graph_create: row_ptr must start at 0 and end at m");
+    set_error("vine_graph_create: row_ptr must start at 0 and end at m");
     return VINE_E_ARG;
   }
   for (int32_t i = 0; i < n; i++)
     if (row_ptr[i + 1] < row_ptr[i]) {
-      set_err("vine_graph_create: row_ptr not monotone at %d", i);
+      set_error("vine_graph_create: row_ptr not monotone at %d", i);
       return VINE_E_ARG;
     }
   for (int64_t p = 0; p < m; p++) {
     if (dst[p] < 0 || dst[p] >= n) {
-      set_err("vine_graph_create: arc %lld has invalid head %d", (long long)p, dst[p]);
+      set_error("vine_graph_create: arc %lld has invalid head %d", (long long)p, dst[p]);
       return VINE_E_ARG;
     }
     if (!(len[p] >= 0.f) || !(delay[p] >= 0.f)) {
-      set_err("vine_graph_create: arc %lld has a negative or NaN weight", (long long)p);
+      set_error("vine_graph_create: arc %lld has a negative or NaN weight", (long long)p);
       return VINE_E_ARG;
     }
   }
@@ -311,7 +301,7 @@ vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, cons
   if (err == hipSuccess) err = hipMemcpy(g->d_perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy(g->d_inv, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
   if (err != hipSuccess) {
-    set_err("vine_graph_create: %s", hipGetErrorString(err));
+    set_error("vine_graph_create: %s", hipGetErrorString(err));
     vine_graph_destroy(g);
     return err == hipErrorOutOfMemory ? VINE_E_OOM : VINE_E_HIP;
   }
@@ -346,7 +336,7 @@ vine_status vine_solver_destroy(vine_solver* s) {
 
 vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vine_solver** out) {
   if (!g || !out) {
-    set_err("vine_solver_create: null argument");
+    set_error("vine_solver_create: null argument");
     return VINE_E_ARG;
   }
   vine_options o;
@@ -355,15 +345,15 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   if (o.max_batch < 1 || o.prune < 0 || o.prune > 2 || o.delta_policy < 0 || o.delta_policy > 3 ||
       o.block_size < 64 || o.block_size > 256 || o.block_size % 64 || o.rounds_per_sync < 1 ||
       o.grid_blocks < 0 || o.hash_slots < 0 || !(o.delta_fixed >= 0.f) || o.bound_ball < 0 || o.bound_ball > 1) {
-    set_err("vine_solver_create: invalid options");
+    set_error("vine_solver_create: invalid options");
     return VINE_E_ARG;
   }
   if (o.labels_per_wave < 0 || o.labels_per_wave > 64 || (o.labels_per_wave & (o.labels_per_wave - 1))) {
-    set_err("vine_solver_create: labels_per_wave must be 0 or a power of two <= 64");
+    set_error("vine_solver_create: labels_per_wave must be 0 or a power of two <= 64");
     return VINE_E_ARG;
   }
   if (o.engine < 0 || o.engine > 2) {
-    set_err("vine_solver_create: invalid engine");
+    set_error("vine_solver_create: invalid engine");
     return VINE_E_ARG;
   }
   // the workgroup engine needs the bounds (windows keyed by d + lbl), a
@@ -372,12 +362,12 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
                      (o.delta_policy == VINE_DELTA_FIXED || o.delta_policy == VINE_DELTA_MLP) &&
                      g->max_deg <= dev::ARENA / 2;
   if (o.engine == VINE_ENGINE_WORKGROUP && !wg_ok) {
-    set_err("vine_solver_create: the workgroup engine needs VINE_PRUNE_BOUNDS, a fixed or learned window "
+    set_error("vine_solver_create: the workgroup engine needs VINE_PRUNE_BOUNDS, a fixed or learned window "
             "and out-degrees <= %d (graph: %d)", dev::ARENA / 2, g->max_deg);
     return VINE_E_ARG;
   }
   if (o.use_graph || o.hash_slots) {
-    set_err("vine_solver_create: use_graph and hash_slots are reserved (rounds are enqueued "
+    set_error("vine_solver_create: use_graph and hash_slots are reserved (rounds are enqueued "
             "rounds_per_sync ahead; fronts live in the HBM pool) and must be 0");
     return VINE_E_ARG;
   }
@@ -393,20 +383,20 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   s->lpw = o.labels_per_wave ? o.labels_per_wave : 64;
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
-    set_err("hipStreamCreate: %s", hipGetErrorString(e));
+    set_error("hipStreamCreate: %s", hipGetErrorString(e));
     delete s;
     return VINE_E_HIP;
   }
   for (auto& ev : s->ev)
     if (hipEventCreate(&ev) != hipSuccess) {
-      set_err("hipEventCreate failed");
+      set_error("hipEventCreate failed");
       vine_solver_destroy(s);
       return VINE_E_HIP;
     }
   const long long N = g->n, B = s->B, BN = N * B;
   size_t freeb = 0, totalb = 0;
   if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
-    set_err("hipMemGetInfo failed");
+    set_error("hipMemGetInfo failed");
     vine_solver_destroy(s);
     return VINE_E_HIP;
   }
@@ -414,7 +404,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   size_t per_qv = sizeof(int2) + sizeof(int32_t) + (o.prune == 2 ? 4 * sizeof(float) : 0);
   double remaining = (double)freeb - (double)per_qv * BN - 64.0 * (1 << 20);
   if (remaining <= 0) {
-    set_err("vine_solver_create: %lld query slots x %lld vertices need %.1f GB, %.1f GB free",
+    set_error("vine_solver_create: %lld query slots x %lld vertices need %.1f GB, %.1f GB free",
             B, N, (double)per_qv * BN / 1e9, (double)freeb / 1e9);
     vine_solver_destroy(s);
     return VINE_E_OOM;
@@ -431,7 +421,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
     lab_cap = std::max<long long>(lab_cap, 4 * B + 1024);
   }
   if (lab_cap > INT32_MAX - 1 || lab_cap < B + 1) {
-    set_err("vine_solver_create: label_capacity must be in [max_batch+1, 2^31-1)");
+    set_error("vine_solver_create: label_capacity must be in [max_batch+1, 2^31-1)");
     vine_solver_destroy(s);
     return VINE_E_ARG;
   }
@@ -440,7 +430,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   long long carry_cap = o.frontier_capacity ? o.frontier_capacity : std::min<long long>(lab_cap / 2 + 1024, INT32_MAX - 1);
   if (pool_cap < 4 * B || pool_cap > 0xFFFFFFFFll || cand_cap < 64 || cand_cap > INT32_MAX - 1 ||
       carry_cap < 64 || carry_cap > INT32_MAX - 1) {
-    set_err("vine_solver_create: capacities out of range");
+    set_error("vine_solver_create: capacities out of range");
     vine_solver_destroy(s);
     return VINE_E_ARG;
   }
@@ -508,7 +498,7 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   ALLOC(s->d_ts, sizeof(unsigned long long) * dev::TS_CAP);
   ALLOC(s->d_ct, sizeof(Counters));
   if (hipHostMalloc((void**)&s->h_ct, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
-    set_err("hipHostMalloc failed");
+    set_error("hipHostMalloc failed");
     vine_solver_destroy(s);
     return VINE_E_OOM;
   }
@@ -640,7 +630,7 @@ vine_status poll(vine_solver* s) {
   HIP_TRY(hipMemcpyAsync(s->h_ct, s->d_ct, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (s->h_ct->overflow) {
-    set_err("device capacity exceeded (flags 0x%x: 1 labels, 2 candidates, 4 touched, 8 pool, "
+    set_error("device capacity exceeded (flags 0x%x: 1 labels, 2 candidates, 4 touched, 8 pool, "
             "16 carry, 32 bound frontier); raise the capacities in vine_options",
             s->h_ct->overflow);
     return VINE_E_OVERFLOW;
@@ -754,7 +744,7 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
   }
   s->nb_active = nb;
   if (s->opt.delta_policy == VINE_DELTA_MLP && !c.mlp) {
-    set_err("VINE_DELTA_MLP needs parameters: call vine_solver_set_mlp first");
+    set_error("VINE_DELTA_MLP needs parameters: call vine_solver_set_mlp first");
     return VINE_E_ARG;
   }
   if (s->opt.prune == 2) {
@@ -833,7 +823,7 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
       // (or re-keys carried labels once), so rounds are bounded by twice the
       // number of labels ever created
       if (rounds > 2 * (int64_t)h.label_ctr + 64) {
-        set_err("solver made no progress after %lld rounds", (long long)rounds);
+        set_error("solver made no progress after %lld rounds", (long long)rounds);
         return VINE_E_OVERFLOW;
       }
       if (ar) {
@@ -859,7 +849,7 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
   hipLaunchKernelGGL(dev::k_stamp_round, dim3(1), dim3(1), 0, s->stream, c, (int)rounds);
   HIP_TRY(hipGetLastError());
   if (s->h_ct->range_flag) {
-    set_err("a label sum reached 2^24; float32 sums are no longer exact");
+    set_error("a label sum reached 2^24; float32 sums are no longer exact");
     return VINE_E_RANGE;
   }
   vine_status es = resolve_events(s, st);
@@ -895,11 +885,11 @@ vine_status check_queries_host(const vine_solver* s, int32_t nq, const int32_t* 
                                const float* bound) {
   for (int32_t i = 0; i < nq; i++) {
     if (src[i] < 0 || src[i] >= s->g->n || dst[i] < 0 || dst[i] >= s->g->n) {
-      set_err("query %d: vertex id out of range [0, %d)", i, s->g->n);
+      set_error("query %d: vertex id out of range [0, %d)", i, s->g->n);
       return VINE_E_ARG;
     }
     if (!(bound[i] >= 0.f) || !(bound[i] < dev::EXACT_LIMIT)) {
-      set_err("query %d: bound %g must be in [0, 2^24)", i, (double)bound[i]);
+      set_error("query %d: bound %g must be in [0, 2^24)", i, (double)bound[i]);
       return VINE_E_ARG;
     }
   }
@@ -1018,7 +1008,7 @@ vine_status vine_solve_device(vine_solver* s, int32_t nq, const int32_t* d_src, 
                               const float* d_bound, float* d_cost, float* d_delay, int32_t* d_feasible,
                               vine_stats* out_stats) {
   if (!s || nq < 0 || (nq > 0 && (!d_src || !d_dst || !d_bound || !d_cost || !d_delay || !d_feasible))) {
-    set_err("vine_solve_device: invalid arguments");
+    set_error("vine_solve_device: invalid arguments");
     return VINE_E_ARG;
   }
   vine_stats st{};
@@ -1035,7 +1025,7 @@ vine_status vine_solve_device(vine_solver* s, int32_t nq, const int32_t* d_src, 
     HIP_TRY(hipMemcpyAsync(&bad, s->d_out_plen, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     if (bad != INT32_MAX) {
-      set_err("query %d: vertex id out of range or bound not in [0, 2^24)", bad);
+      set_error("query %d: vertex id out of range or bound not in [0, 2^24)", bad);
       return VINE_E_ARG;
     }
   }
@@ -1051,7 +1041,7 @@ vine_status vine_solve(vine_solver* s, int32_t nq, const int32_t* src, const int
                        int32_t* out_path, int64_t path_cap, vine_stats* out_stats) {
   if (!s || nq < 0 || (nq > 0 && (!src || !dst || !bound || !out_cost || !out_delay || !out_feasible)) ||
       (out_path && (!out_path_off || path_cap < 0))) {
-    set_err("vine_solve: invalid arguments");
+    set_error("vine_solve: invalid arguments");
     return VINE_E_ARG;
   }
   vine_status v = check_queries_host(s, nq, src, dst, bound);
@@ -1095,7 +1085,7 @@ vine_status vine_solve(vine_solver* s, int32_t nq, const int32_t* src, const int
         hipError_t e = hipMalloc(&d_path, sizeof(int32_t) * need);
         if (e != hipSuccess) {
           (void)hipFree(d_off);
-          set_err("hipMalloc for paths: %s", hipGetErrorString(e));
+          set_error("hipMalloc for paths: %s", hipGetErrorString(e));
           return VINE_E_OOM;
         }
         std::vector<long long> off(nb + 1);
@@ -1112,7 +1102,7 @@ vine_status vine_solve(vine_solver* s, int32_t nq, const int32_t* src, const int
         (void)hipFree(d_off);
         (void)hipFree(d_path);
         if (e != hipSuccess) {
-          set_err("path extraction: %s", hipGetErrorString(e));
+          set_error("path extraction: %s", hipGetErrorString(e));
           return VINE_E_HIP;
         }
       }
@@ -1127,7 +1117,7 @@ vine_status vine_solve(vine_solver* s, int32_t nq, const int32_t* src, const int
   st.ms_kernels = st.ms_expand + st.ms_merge + st.ms_sssp + st.ms_pull;
   if (out_stats) *out_stats = st;
   if (path_overflow) {
-    set_err("vine_solve: path_cap %lld too small (need %lld)", (long long)path_cap,
+    set_error("vine_solve: path_cap %lld too small (need %lld)", (long long)path_cap,
             (long long)out_path_off[nq]);
     return VINE_E_OVERFLOW;
   }
@@ -1161,7 +1151,7 @@ vine_status vine_solver_query_stats(const vine_solver* s, int32_t nq, int64_t* e
 
 vine_status vine_solver_set_mlp(vine_solver* s, const float* params, int64_t nparams) {
   if (!s || !params || nparams != vine::mlp::NPARAM) {
-    set_err("vine_solver_set_mlp: invalid arguments");
+    set_error("vine_solver_set_mlp: invalid arguments");
     return VINE_E_ARG;
   }
   if (!s->d_mlp || s->mlp_nparams != nparams) {
@@ -1182,14 +1172,14 @@ vine_status vine_solver_set_mlp(vine_solver* s, const float* params, int64_t npa
 
 int64_t vine_solver_pareto(const vine_solver* s, int32_t q, int32_t v, float* out_pairs, int64_t cap) {
   if (!s || q < 0 || q >= s->B || v < 0 || v >= s->g->n || (cap > 0 && !out_pairs)) {
-    set_err("vine_solver_pareto: invalid arguments");
+    set_error("vine_solver_pareto: invalid arguments");
     return VINE_E_ARG;
   }
   int2 ps;
   hipError_t e = hipMemcpy(&ps, s->d_pset + (long long)q * s->g->n + s->g->perm[v], sizeof(int2),
                            hipMemcpyDeviceToHost);
   if (e != hipSuccess) {
-    set_err("vine_solver_pareto: %s", hipGetErrorString(e));
+    set_error("vine_solver_pareto: %s", hipGetErrorString(e));
     return VINE_E_HIP;
   }
   // the stored front holds exactly its live entries (sorted by d)
@@ -1198,7 +1188,7 @@ int64_t vine_solver_pareto(const vine_solver* s, int32_t q, int32_t v, float* ou
   if (slots > 0) {
     e = hipMemcpy(tmp.data(), s->d_pool_cd + (uint32_t)ps.x, sizeof(float2) * slots, hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
-      set_err("vine_solver_pareto: %s", hipGetErrorString(e));
+      set_error("vine_solver_pareto: %s", hipGetErrorString(e));
       return VINE_E_HIP;
     }
   }

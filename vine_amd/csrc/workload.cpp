@@ -15,13 +15,7 @@
 #include <thread>
 #include <vector>
 
-struct vine_host_graph {
-  int32_t n = 0;
-  std::vector<int32_t> row_ptr;
-  std::vector<int32_t> dst;
-  std::vector<float> len;
-  std::vector<float> delay;
-};
+#include "host_graph.hpp"
 
 namespace {
 

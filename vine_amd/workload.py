@@ -16,6 +16,7 @@ Standard workloads:
             and bound factor U[1.05, 2.0].
 """
 import ctypes
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -46,17 +47,18 @@ class Queries:
     src: np.ndarray  # int32[q]
     dst: np.ndarray  # int32[q]
     bound: np.ndarray  # float32[q]
-    mindelay: np.ndarray  # int64[q], -1 if unreachable
+    mindelay: np.ndarray  # int64[q], -1 if unreachable (None for queries read from a file)
     hops: np.ndarray = field(default=None)  # int32[q] (hop-distance queries only)
+    aux: np.ndarray = field(default=None)  # int32[q, 2] (read from a file: (lower, upper) or (hop, cost))
 
     def __len__(self):
         return int(self.src.shape[0])
 
     def subset(self, idx):
         idx = np.asarray(idx)
+        take = lambda a: None if a is None else a[idx].copy()
         return Queries(self.src[idx].copy(), self.dst[idx].copy(), self.bound[idx].copy(),
-                       self.mindelay[idx].copy(),
-                       None if self.hops is None else self.hops[idx].copy())
+                       take(self.mindelay), take(self.hops), take(self.aux))
 
 
 def _take(h, name="", seed=0):
@@ -112,6 +114,46 @@ def from_csr(row_ptr, dst, len_, delay, name="custom"):
                                           dst.ctypes.data, len_.ctypes.data, delay.ctypes.data,
                                           ctypes.byref(h)), "vine_host_graph_from_csr")
     return _take(h, name)
+
+
+FILE_SNAP = 1  # SNAP edge list (the reference's graph_type 1), undirected
+FILE_COLA = 2  # COLA road network (graph_type 2), directed arcs
+
+
+def _read_error(what, rc):
+    msg = lib().vine_last_error()
+    raise ValueError("%s failed (status %d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+def read_graph(path, fmt):
+    """Graph file in the reference's formats (read_graph, include/io.hpp:22-91):
+    FILE_COLA ("u: v w c ... -1" lines, arcs as listed) or FILE_SNAP (undirected
+    rows, the reference's "N?M?" layout or the shipped MatrixMarket files)."""
+    h = ctypes.c_void_p()
+    rc = lib().vine_read_graph(str(path).encode(), fmt, ctypes.byref(h))
+    if rc != 0:
+        _read_error("vine_read_graph", rc)
+    return _take(h, os.path.basename(str(path)))
+
+
+def read_queries(path, fmt):
+    """Query file (task::read_query / load_query, include/task.hpp:27-76):
+    FILE_COLA rows "u v lower upper limit" (bound = limit, aux = (lower, upper))
+    or FILE_SNAP CSV rows "id,src,snk,delay,hop,cost" after a header line
+    (bound = delay, aux = (hop, cost))."""
+    path = str(path).encode()
+    nq = lib().vine_read_queries(path, fmt, 0, None, None, None, None)
+    if nq < 0:
+        _read_error("vine_read_queries", nq)
+    src = np.zeros(nq, np.int32)
+    dst = np.zeros(nq, np.int32)
+    bound = np.zeros(nq, np.float32)
+    aux = np.zeros((nq, 2), np.int32)
+    got = lib().vine_read_queries(path, fmt, nq, src.ctypes.data, dst.ctypes.data, bound.ctypes.data,
+                                  aux.ctypes.data)
+    if got != nq:
+        _read_error("vine_read_queries", got)
+    return Queries(src, dst, bound, None, aux=aux)
 
 
 class _HostHandle:
