@@ -25,6 +25,11 @@ static int run_files(char** argv, bool solve) {
     std::printf("error %d %s\n", st, vine_last_error());
     return 2;
   }
+  if (std::atoi(argv[5]) == 1)  // main.cu:19-26 folds SNAP task ids into range
+    for (int i = 0; i < tasks.ntask; i++) {
+      tasks.source_l_[i] %= g.N;
+      tasks.sink_l_[i] %= g.N;
+    }
   std::printf("graph %d %lld\ntasks %d\n", g.N, (long long)g.M, tasks.ntask);
   if (!solve) return 0;
   std::vector<c_label> src;
