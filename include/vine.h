@@ -221,6 +221,56 @@ int64_t vine_dop_sequence(const int64_t* sizes, int n, double max_labels, int64_
  * share cache lines of the per-(query, vertex) arrays.  Host only. */
 int32_t vine_locality_order(int32_t n, const int32_t* row_ptr, const int32_t* dst, int32_t* out_order);
 
+/* ---- Other label families ------------------------------------------------
+ * The reference compiles one label family at a time (CSP in include/label.hpp:
+ * 46-169, 248-461 and include/constraint.hpp:21-80, 106-186); here each is a
+ * family id of a generic label-correcting GPU search (functors in
+ * include/vine/families.hpp).  Per query: limits[3*i .. 3*i+3) =
+ * (delay_L, hop_L, maxcost_L) (hop_L and maxcost_L are read by the hop
+ * families only; the reference's defaults are 100 and 10000), and the answer
+ * out_label[4*i .. 4*i+4) = (d, delay, hop, maxcost) of the best label in t's
+ * Pareto set (+inf when t's set is empty; unused fields 0):
+ *   VINE_FAMILY_CSP, VINE_FAMILY_2OSP (CSP == 1, 4): min (d, delay); delay <= delay_L
+ *   VINE_FAMILY_HOP (CSP == 2): + hop count, hop <= hop_L; min (d, delay, hop)
+ *   VINE_FAMILY_HOP_MAXCOST (CSP == 3): + max arc len on the path <= maxcost_L;
+ *     min (d, delay, hop, maxcost)
+ *   VINE_FAMILY_SOCIAL (CSP == 5): products from (1, 1), larger dominates,
+ *     delay <= delay_L; max (d, delay); arc weights must lie in [0, 1]
+ *   VINE_FAMILY_QOS (default branch): d additive, delay = bottleneck (min arc
+ *     delay, 21474836 at the source), bottleneck >= delay_L; min d, then max
+ *     bottleneck.
+ * No counterpart in the reference for the answer order: it never extracts an
+ * answer for these families. */
+typedef enum {
+  VINE_FAMILY_CSP = 1,
+  VINE_FAMILY_HOP = 2,
+  VINE_FAMILY_HOP_MAXCOST = 3,
+  VINE_FAMILY_2OSP = 4,
+  VINE_FAMILY_SOCIAL = 5,
+  VINE_FAMILY_QOS = 6
+} vine_family;
+
+typedef struct vine_family_solver vine_family_solver;
+
+/* Device memory for max_batch queries per batch and label_capacity labels
+ * (0 = 2^25); about 37 B per label plus 8 B per (query slot, vertex). */
+vine_status vine_family_solver_create(const vine_graph* g, int32_t family, int32_t max_batch,
+                                      int64_t label_capacity, vine_family_solver** out);
+vine_status vine_family_solver_destroy(vine_family_solver* s);
+
+/* Host buffers; queries run in batches of max_batch.  Paths as in vine_solve
+ * (skipped when out_path is NULL; out_path_off may then still receive the
+ * offsets).  out_counters (optional, 4 entries): rounds, labels expanded,
+ * arcs relaxed, labels inserted, summed over batches. */
+vine_status vine_family_solve(vine_family_solver* s, int32_t nq, const int32_t* src, const int32_t* dst,
+                              const float* limits, float* out_label, int32_t* out_feasible,
+                              int64_t* out_path_off, int32_t* out_path, int64_t path_cap,
+                              int64_t* out_counters);
+
+/* Pareto set of vertex v for query q of the last batch: up to cap labels as
+ * (d, delay, hop, maxcost); returns the set size or a negative status. */
+int64_t vine_family_front(const vine_family_solver* s, int32_t q, int32_t v, float* out_labels, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,11 @@ def lib():
         L.oracle_ar_burg.argtypes = [vp, i32, i32, vp]
         L.oracle_ar_lsq.restype = i32
         L.oracle_ar_lsq.argtypes = [vp, i32, i32, vp]
+        L.oracle_family.restype = i32
+        L.oracle_family.argtypes = [i32, i32, vp, vp, vp, vp, i32, i32, vp, vp, vp, i32, vp, vp, i32, vp, vp,
+                                    ctypes.c_int64]
+        L.oracle_family_bruteforce.restype = i32
+        L.oracle_family_bruteforce.argtypes = [i32, i32, vp, vp, vp, vp, i32, i32, vp, vp, i32]
         _lib = L
     return _lib
 
@@ -98,6 +103,45 @@ def cspp(g, s, t, bound, prune_target=False, path_cap=None, pareto_cap=4096, tra
     return Result(rc == 1, cost.value, dly.value,
                   path[:plen.value].copy() if plen.value <= path_cap else None,
                   par[:2 * k].reshape(k, 2).copy(), counters, tr)
+
+
+@dataclass
+class FamilyResult:
+    feasible: bool
+    best: np.ndarray  # float32[4]: d, delay, hop, maxcost
+    path: np.ndarray
+    front: np.ndarray  # (k, 4) float32, t's Pareto set in the answer order
+    counters: dict
+
+
+def family(g, fam, s, t, limits, front_cap=65536, max_labels=1 << 28):
+    """oracle_family: the label family `fam` (1..6 as vine_family) for one query;
+    limits = (delay_L, hop_L, maxcost_L)."""
+    lim = np.asarray(limits, np.float32)
+    best = np.zeros(4, np.float32)
+    path = np.zeros(g.n, np.int32)
+    plen = ctypes.c_int32()
+    front = np.zeros(4 * front_cap, np.float32)
+    flen = ctypes.c_int32()
+    cnt = np.zeros(4, np.int64)
+    rc = lib().oracle_family(int(fam), *_g(g), int(s), int(t), lim.ctypes.data, best.ctypes.data, path.ctypes.data,
+                             g.n, ctypes.byref(plen), front.ctypes.data, front_cap, ctypes.byref(flen),
+                             cnt.ctypes.data, int(max_labels))
+    if rc < 0:
+        raise RuntimeError("oracle_family failed (%d)" % rc)
+    k = min(flen.value, front_cap)
+    return FamilyResult(rc == 1, best, path[:plen.value].copy(), front[:4 * k].reshape(k, 4).copy(),
+                        dict(zip(("rounds", "expanded", "relaxed", "inserted"), (int(x) for x in cnt))))
+
+
+def family_bruteforce(g, fam, s, t, limits, cap=100000):
+    """t's non-dominated set over simple paths (tiny graphs), answer order."""
+    lim = np.asarray(limits, np.float32)
+    out = np.zeros(4 * cap, np.float32)
+    m = lib().oracle_family_bruteforce(int(fam), *_g(g), int(s), int(t), lim.ctypes.data, out.ctypes.data, cap)
+    if m < 0:
+        raise RuntimeError("oracle_family_bruteforce failed (%d)" % m)
+    return out[:4 * min(m, cap)].reshape(-1, 4).copy()
 
 
 def dijkstra_lex(g, s, w1="len", w2="delay"):

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "mlp.hpp"
+#include "device_graph.hpp"
 #include "errors.hpp"
 #include "layout.hpp"
 #include "sched.hpp"
@@ -52,24 +53,6 @@ double now_ms() {
 }
 
 }  // namespace
-
-struct vine_graph {
-  int32_t n = 0;
-  int64_t m = 0;
-  float mean_len = 1.f;
-  float mean_delay = 1.f;
-  int32_t max_deg = 0;
-  int device = 0;
-  // vertices are renumbered for locality (layout.cpp): perm[caller id] =
-  // internal id, d_inv the inverse; every array below uses internal ids
-  std::vector<int32_t> perm;
-  int32_t* d_perm = nullptr;
-  int32_t* d_inv = nullptr;
-  int32_t* d_row = nullptr;
-  int4* d_edges = nullptr;
-  int32_t* d_rrow = nullptr;
-  int4* d_redges = nullptr;
-};
 
 struct vine_solver {
   const vine_graph* g = nullptr;
@@ -286,6 +269,10 @@ vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, cons
     g->mean_delay = (float)std::max(sd / m, 1e-3);
   }
   for (int32_t u = 0; u < n; u++) g->max_deg = std::max<int32_t>(g->max_deg, row_ptr[u + 1] - row_ptr[u]);
+  for (int64_t p = 0; p < m; p++) {
+    g->max_len = std::max(g->max_len, len[p]);
+    g->max_delay = std::max(g->max_delay, delay[p]);
+  }
   hipError_t err = hipGetDevice(&g->device);
   size_t eb = sizeof(int4) * std::max<int64_t>(m, 1);
   if (err == hipSuccess) err = hipMalloc(&g->d_row, sizeof(int32_t) * (n + 1));

@@ -92,7 +92,8 @@ EXPORTS = (
     "vine_options_default", "vine_last_error", "vine_device_count", "vine_graph_create",
     "vine_graph_destroy", "vine_solver_create", "vine_solver_destroy", "vine_solve",
     "vine_solve_device", "vine_solver_latency", "vine_solver_trace", "vine_solver_set_mlp",
-    "vine_solver_pareto", "vine_solver_query_stats",
+    "vine_solver_pareto", "vine_solver_query_stats", "vine_family_solver_create", "vine_family_solver_destroy",
+    "vine_family_solve", "vine_family_front",
 )
 
 _declared = False
@@ -119,6 +120,10 @@ def _L():
             "vine_solver_query_stats": (i32, [vp, i32, vp, vp]),
             "vine_solver_set_mlp": (i32, [vp, vp, i64]),
             "vine_solver_pareto": (i64, [vp, i32, i32, vp, i64]),
+            "vine_family_solver_create": (i32, [vp, i32, i32, i64, P(vp)]),
+            "vine_family_solver_destroy": (i32, [vp]),
+            "vine_family_solve": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, i64, vp]),
+            "vine_family_front": (i64, [vp, i32, i32, vp, i64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -276,6 +281,88 @@ class Solver:
     def set_mlp(self, params):
         p = np.ascontiguousarray(params, np.float32)
         _check(_L().vine_solver_set_mlp(self.h, p.ctypes.data, p.shape[0]))
+
+
+# label families (vine_family in include/vine.h; the reference's CSP values)
+FAMILY_CSP, FAMILY_HOP, FAMILY_HOP_MAXCOST, FAMILY_2OSP, FAMILY_SOCIAL, FAMILY_QOS = 1, 2, 3, 4, 5, 6
+# the reference's constructor defaults for hop_L and maxcost_L (include/constraint.hpp:31, 61)
+DEFAULT_HOP_L, DEFAULT_MAXCOST_L = 100, 10000.0
+
+
+@dataclass
+class FamilyAnswer:
+    label: np.ndarray  # float32[nq, 4]: d, delay, hop, maxcost of the best label at t (+inf if infeasible)
+    feasible: np.ndarray
+    path_off: np.ndarray
+    path: np.ndarray
+    counters: dict
+
+    def path_of(self, i):
+        return self.path[self.path_off[i]:self.path_off[i + 1]]
+
+
+class FamilySolver:
+    """Generic label-correcting search for one label family (vine_family_solve)."""
+
+    def __init__(self, graph, family, max_batch=64, label_capacity=0):
+        self.graph = graph if isinstance(graph, DeviceGraph) else DeviceGraph(graph)
+        h = ctypes.c_void_p()
+        _check(_L().vine_family_solver_create(self.graph.h, int(family), int(max_batch), int(label_capacity),
+                                              ctypes.byref(h)))
+        self.h = h
+        self.family = family
+
+    def close(self):
+        if getattr(self, "h", None):
+            _L().vine_family_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def solve(self, src, dst, delay_L, hop_L=DEFAULT_HOP_L, maxcost_L=DEFAULT_MAXCOST_L, paths=True):
+        src = np.ascontiguousarray(src, np.int32)
+        dst = np.ascontiguousarray(dst, np.int32)
+        nq = int(src.shape[0])
+        lim = np.zeros((nq, 3), np.float32)
+        lim[:, 0] = delay_L
+        lim[:, 1] = hop_L
+        lim[:, 2] = maxcost_L
+        out = np.zeros((nq, 4), np.float32)
+        feas = np.zeros(nq, np.int32)
+        off = np.zeros(nq + 1, np.int64)
+        cnt = np.zeros(4, np.int64)
+        path_cap = max(1, nq * 64)
+        while True:
+            path = np.zeros(path_cap if paths else 1, np.int32)
+            rc = _L().vine_family_solve(self.h, nq, src.ctypes.data, dst.ctypes.data, lim.ctypes.data,
+                                        out.ctypes.data, feas.ctypes.data, off.ctypes.data,
+                                        path.ctypes.data if paths else None, path_cap if paths else 0,
+                                        cnt.ctypes.data)
+            if rc == VINE_E_OVERFLOW and paths and off[-1] > path_cap:
+                path_cap = int(off[-1])
+                continue
+            _check(rc)
+            break
+        return FamilyAnswer(out, feas.astype(bool), off, path[:off[-1]] if paths else None,
+                            dict(zip(("rounds", "expanded", "relaxed", "inserted"), (int(x) for x in cnt))))
+
+    def front(self, q, v, cap=1 << 12):
+        """Pareto set of vertex v for query q of the last batch, (k, 4) float32."""
+        out = np.zeros(4 * cap, np.float32)
+        n = _L().vine_family_front(self.h, int(q), int(v), out.ctypes.data, cap)
+        if n < 0:
+            _check(int(n))
+        return out[:4 * min(n, cap)].reshape(-1, 4).copy()
 
 
 class c_label:
