@@ -41,6 +41,8 @@ def main():
     ap.add_argument("--delta", type=float, default=None,
                     help="window (fixed) or first window (mlp); default 12 and 4 key units")
     ap.add_argument("--model", default=os.path.join(ROOT, "vine_amd", "models", "delta_mlp.npy"))
+    ap.add_argument("--concurrency", type=int, default=3,
+                    help="internal solvers on separate HIP streams (vine_options.concurrency)")
     ap.add_argument("--plain-delta", type=float, default=12.0, help="window of the plain single-query solver")
     ap.add_argument("--plain-queries", type=int, default=100, help="per bound factor")
     ap.add_argument("--cpu-queries", type=int, default=32)
@@ -79,11 +81,13 @@ def main():
     dg = S.DeviceGraph(g)
     solver = S.Solver(dg, max_batch=args.batch, prune=S.PRUNE_BOUNDS, delta_policy=policy,
                       delta_fixed=args.delta, rounds_per_sync=16, time_kernels=TIME_EVERY,
+                      concurrency=args.concurrency,
                       label_capacity=1_000_000_000, pool_capacity=2_000_000_000,
                       cand_capacity=100_000_000, frontier_capacity=400_000_000)
     if policy == S.DELTA_MLP:
         solver.set_mlp(np.load(args.model))
-    log("solver ready (batch %d, policy %s, window %g)" % (args.batch, args.policy, args.delta))
+    log("solver ready (batch %d, policy %s, window %g, concurrency %d)"
+        % (args.batch, args.policy, args.delta, args.concurrency))
     nq = len(qb)
     d_src = torch.from_numpy(qb.src).to(dev)
     d_dst = torch.from_numpy(qb.dst).to(dev)
@@ -236,6 +240,7 @@ def main():
         "config": {"workload": "ny_batch: 264,346-vertex 3-NN graph (%d arcs), %d queries, hop-distance "
                                "log-uniform [10,2000], bound U[1.05,2.0] x min delay" % (g.m, nq),
                    "batch": args.batch, "policy": args.policy, "delta": args.delta, "prune": "bounds",
+                   "concurrency": args.concurrency,
                    "plain": "1 query per solve, fixed window %g, random pairs bound 1.1x (%d queries)"
                             % (args.plain_delta, args.plain_queries),
                    "parallelism": "single GPU"},

@@ -101,7 +101,13 @@ typedef struct {
   int32_t engine;           /* VINE_ENGINE_* */
   int32_t labels_per_wave;  /* frontier entries a wave takes per step in expand (power of two <= 64;
                                0 = 64); fewer spread a heavy round's arcs over more waves */
-  int32_t reserved[2];
+  int32_t concurrency;      /* 0 or 1: one stream.  k in 2..16: k internal solvers on k HIP streams,
+                               each with ceil(max_batch / k) slots and 1/k of every explicit
+                               capacity (automatic ones are sized for the whole batch); a solve
+                               gives query i to part i % k and runs the parts from k host threads,
+                               so one part's long tail overlaps the others' rounds.  With k > 1,
+                               vine_solver_pareto's q is the query index (nq <= max_batch) */
+  int32_t reserved[1];
 } vine_options;
 
 typedef struct {

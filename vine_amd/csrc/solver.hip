@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mlp.hpp"
@@ -138,7 +139,19 @@ struct vine_solver {
   std::vector<int32_t> q_rounds;
   int last_batch_q = 0;
   int nb_active = 0;  // queries in the current batch
+
+  // concurrency > 1: the parts do all the work, query i going to part
+  // i % parts.size() as its (i / parts.size())-th query
+  std::vector<vine_solver*> parts;
+  std::vector<void*> split_bufs;  // gathered device query/answer arrays, per part
+  int32_t split_cap = 0;          // queries each split_bufs entry holds
 };
+
+namespace {
+vine_status create_single(const vine_graph* g, const vine_options& o, double mem_frac, long long want_slots,
+                          vine_solver** out);
+vine_status create_split(const vine_graph* g, const vine_options& o, vine_solver** out);
+}  // namespace
 
 static vine_status dev_alloc(vine_solver* s, void** p, size_t bytes) {
   if (bytes == 0) bytes = 16;
@@ -310,6 +323,8 @@ vine_status vine_graph_destroy(vine_graph* g) {
 
 vine_status vine_solver_destroy(vine_solver* s) {
   if (!s) return VINE_OK;
+  for (vine_solver* p : s->parts) vine_solver_destroy(p);
+  for (void* p : s->split_bufs) (void)hipFree(p);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (void* p : s->allocs) (void)hipFree(p);
   if (s->h_ct) (void)hipHostFree(s->h_ct);
@@ -358,6 +373,22 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
             "rounds_per_sync ahead; fronts live in the HBM pool) and must be 0");
     return VINE_E_ARG;
   }
+  if (o.concurrency < 0 || o.concurrency > 16) {
+    set_error("vine_solver_create: concurrency must lie in [0, 16]");
+    return VINE_E_ARG;
+  }
+  if (o.concurrency > 1) return create_split(g, o, out);
+  return create_single(g, o, 0.6, o.max_batch, out);
+}
+
+}  // extern "C"
+
+namespace {
+
+// One solver on one stream.  Automatic capacities are sized for want_slots
+// query slots, capped by mem_frac of the free memory.
+vine_status create_single(const vine_graph* g, const vine_options& o, double mem_frac, long long want_slots,
+                          vine_solver** out) {
   auto* s = new vine_solver();
   s->g = g;
   s->opt = o;
@@ -366,6 +397,9 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   // about 768 waves: the round kernels are latency-bound, and measured on
   // MI355X fewer, fuller waves beat a grid that fills every CU
   s->grid = o.grid_blocks ? o.grid_blocks : 384;
+  const bool wg_ok = o.prune == VINE_PRUNE_BOUNDS &&
+                     (o.delta_policy == VINE_DELTA_FIXED || o.delta_policy == VINE_DELTA_MLP) &&
+                     g->max_deg <= dev::ARENA / 2;
   s->wg = wg_ok && o.engine == VINE_ENGINE_WORKGROUP;
   s->lpw = o.labels_per_wave ? o.labels_per_wave : 64;
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -401,10 +435,10 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   long long lab_cap = o.label_capacity;
   if (!lab_cap) {
     // sized to the problem (64 labels per (query, vertex) pair plus slack),
-    // capped by 60% of the free memory
+    // capped by mem_frac of the free memory
     double per = 25.0 + 2 * 12.0 + 0.25 * 144.0 + 0.5 * 32.0 + 0.25 * 16.0;
-    double want = 64.0 * (double)BN + (double)(1 << 22);
-    lab_cap = (long long)std::min(std::min(remaining * 0.6 / per, want), (double)INT32_MAX - 1);
+    double want = 64.0 * (double)(N * want_slots) + (double)(1 << 22);
+    lab_cap = (long long)std::min(std::min(remaining * mem_frac / per, want), (double)INT32_MAX - 1);
     lab_cap = std::max<long long>(lab_cap, 4 * B + 1024);
   }
   if (lab_cap > INT32_MAX - 1 || lab_cap < B + 1) {
@@ -558,7 +592,38 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   return VINE_OK;
 }
 
-}  // extern "C"
+// `concurrency` solvers, each with its share of the query slots, on their
+// own streams.  Explicit capacities are divided among them; automatic ones
+// are sized as for the whole batch (the heaviest queries may meet in one
+// part), each part capped by an equal share of 60% of the free memory.
+vine_status create_split(const vine_graph* g, const vine_options& o, vine_solver** out) {
+  const int k = o.concurrency;
+  auto* s = new vine_solver();
+  s->g = g;
+  s->opt = o;
+  s->B = o.max_batch;
+  vine_options oc = o;
+  oc.concurrency = 0;
+  oc.max_batch = (o.max_batch + k - 1) / k;
+  auto share = [k](int64_t v, int64_t lo) { return v ? std::max<int64_t>((v + k - 1) / k, lo) : 0; };
+  oc.label_capacity = share(o.label_capacity, oc.max_batch + 1);
+  oc.pool_capacity = share(o.pool_capacity, 4 * (int64_t)oc.max_batch);
+  oc.cand_capacity = share(o.cand_capacity, 64);
+  oc.frontier_capacity = share(o.frontier_capacity, 64);
+  for (int i = 0; i < k; i++) {
+    vine_solver* p = nullptr;
+    vine_status r = create_single(g, oc, 0.6 / (k - 0.6 * i), o.max_batch, &p);
+    if (r != VINE_OK) {
+      vine_solver_destroy(s);
+      return r;
+    }
+    s->parts.push_back(p);
+  }
+  *out = s;
+  return VINE_OK;
+}
+
+}  // namespace
 
 // ---------------------------------------------------------------------------
 namespace {
@@ -893,15 +958,19 @@ __global__ void k_check_queries(const int32_t* src, const int32_t* dst, const fl
 }
 
 // Solve nq queries whose arrays are in device memory; answers to device memory.
+// Per-query records (latency, work, trace rows) go to index rec_base + i of
+// records sized rec_total; rec_base = 0 starts a new solve's records.
 vine_status solve_impl(vine_solver* s, int32_t nq, const int32_t* d_src, const int32_t* d_dst,
                        const float* d_bound, float* d_cost, float* d_delay, int32_t* d_feas,
-                       int32_t* d_plen, vine_stats* st) {
+                       int32_t* d_plen, vine_stats* st, int32_t rec_base, int32_t rec_total) {
   Ctx& c = s->ctx;
-  s->latency_us.assign(nq, 0.0);
-  s->round_seq = 0;
-  s->q_expanded.assign(nq, 0);
-  s->q_rounds.assign(nq, 0);
-  s->trace.clear();
+  if (rec_base == 0) {
+    s->latency_us.assign(rec_total, 0.0);
+    s->round_seq = 0;
+    s->q_expanded.assign(rec_total, 0);
+    s->q_rounds.assign(rec_total, 0);
+    s->trace.clear();
+  }
   for (int32_t q0 = 0; q0 < nq; q0 += s->B) {
     int nb = std::min<int32_t>(s->B, nq - q0);
     // endpoints in internal vertex ids (ids were validated by the caller)
@@ -917,7 +986,7 @@ vine_status solve_impl(vine_solver* s, int32_t nq, const int32_t* d_src, const i
       HIP_TRY(hipMemsetAsync(s->d_qdst + nb, 0, sizeof(int32_t) * (s->B - nb), s->stream));
       HIP_TRY(hipMemsetAsync(s->d_qbound + nb, 0, sizeof(float) * (s->B - nb), s->stream));
     }
-    vine_status bs = solve_batch(s, nb, q0, st);
+    vine_status bs = solve_batch(s, nb, rec_base + q0, st);
     if (bs != VINE_OK) return bs;
     // per-query finish times: merge-end clock of the last round with live labels
     std::vector<int32_t> last(nb);
@@ -936,9 +1005,10 @@ vine_status solve_impl(vine_solver* s, int32_t nq, const int32_t* d_src, const i
       std::vector<long long> ex(nb);
       HIP_TRY(hipMemcpy(ex.data(), s->d_qexptot, sizeof(long long) * nb, hipMemcpyDeviceToHost));
       for (int i = 0; i < nb; i++) {
-        if (q0 + i < (int32_t)s->q_expanded.size()) {
-          s->q_expanded[q0 + i] = ex[i];
-          s->q_rounds[q0 + i] = last[i] + 1;
+        const int32_t at = rec_base + q0 + i;
+        if (at < (int32_t)s->q_expanded.size()) {
+          s->q_expanded[at] = ex[i];
+          s->q_rounds[at] = last[i] + 1;
         }
       }
     }
@@ -956,7 +1026,7 @@ vine_status solve_impl(vine_solver* s, int32_t nq, const int32_t* d_src, const i
       } else if (r >= 1 && r < nr && ts[r] >= tstart) {
         us = (double)(ts[r] - tstart) * 1000.0 / rate_khz;
       }
-      if (q0 + i < (int32_t)s->latency_us.size()) s->latency_us[q0 + i] = us;
+      if (rec_base + q0 + i < (int32_t)s->latency_us.size()) s->latency_us[rec_base + q0 + i] = us;
     }
     hipLaunchKernelGGL(dev::k_answers, dim3((nb + 255) / 256), dim3(256), 0, s->stream, c, nb, d_cost + q0,
                        d_delay + q0, d_feas + q0, d_plen ? d_plen + q0 : nullptr);
@@ -987,6 +1057,243 @@ vine_status ensure_io(vine_solver* s, int32_t nq) {
   return VINE_OK;
 }
 
+// ---- concurrency > 1: query i is part (i % k)'s query (i / k)
+__global__ void k_part_gather(int nq, int k, int p, const int32_t* src, const int32_t* dst, const float* bound,
+                              int32_t* psrc, int32_t* pdst, float* pbound) {
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long i = j * k + p;
+  if (i >= nq) return;
+  psrc[j] = src[i];
+  pdst[j] = dst[i];
+  pbound[j] = bound[i];
+}
+
+__global__ void k_part_scatter(int nq, int k, int p, const float* pcost, const float* pdelay, const int32_t* pfeas,
+                               float* cost, float* delay, int32_t* feas) {
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long i = j * k + p;
+  if (i >= nq) return;
+  cost[i] = pcost[j];
+  delay[i] = pdelay[j];
+  feas[i] = pfeas[j];
+}
+
+int32_t part_size(int32_t nq, int k, int p) { return p < nq ? (nq - p + k - 1) / k : 0; }
+
+void merge_stats(vine_stats& a, const vine_stats& b) {
+  a.rounds += b.rounds;
+  a.sssp_rounds += b.sssp_rounds;
+  a.expanded += b.expanded;
+  a.relaxed += b.relaxed;
+  a.candidates += b.candidates;
+  a.inserted += b.inserted;
+  a.sssp_relaxed += b.sssp_relaxed;
+  a.max_frontier = std::max(a.max_frontier, b.max_frontier);
+  a.batches += b.batches;
+  a.ms_kernels += b.ms_kernels;
+  a.ms_expand += b.ms_expand;
+  a.ms_merge += b.ms_merge;
+  a.ms_sssp += b.ms_sssp;
+  a.expand_launches += b.expand_launches;
+  a.merge_launches += b.merge_launches;
+  a.sssp_launches += b.sssp_launches;
+  a.touched += b.touched;
+  a.merge_scanned += b.merge_scanned;
+  a.merge_chain_max = std::max(a.merge_chain_max, b.merge_chain_max);
+  a.far_pulled += b.far_pulled;
+  a.ms_pull += b.ms_pull;
+  a.near_carried += b.near_carried;
+  a.far_appended += b.far_appended;
+  a.scanned += b.scanned;
+  a.timed_rounds += b.timed_rounds;
+}
+
+// Per-query records of the parts, in the caller's query order.
+void gather_records(vine_solver* s, int32_t nq) {
+  const int k = (int)s->parts.size();
+  s->latency_us.assign(nq, 0.0);
+  s->q_expanded.assign(nq, 0);
+  s->q_rounds.assign(nq, 0);
+  s->trace.clear();
+  for (int p = 0; p < k; p++) {
+    const vine_solver* c = s->parts[p];
+    const int32_t np = part_size(nq, k, p);
+    for (int32_t j = 0; j < np; j++) {
+      const int32_t i = j * k + p;
+      if (j < (int32_t)c->latency_us.size()) s->latency_us[i] = c->latency_us[j];
+      if (j < (int32_t)c->q_expanded.size()) {
+        s->q_expanded[i] = c->q_expanded[j];
+        s->q_rounds[i] = c->q_rounds[j];
+      }
+    }
+    const size_t at = s->trace.size();
+    s->trace.insert(s->trace.end(), c->trace.begin(), c->trace.end());
+    for (size_t r = at; r < s->trace.size(); r += 10) s->trace[r] = s->trace[r] * (float)k + (float)p;
+  }
+}
+
+// Runs fn(p) for every part with queries on its own host thread; the first
+// failing part's status and message become the call's.
+template <typename Fn>
+vine_status run_parts(vine_solver* s, int32_t nq, Fn fn) {
+  const int k = (int)s->parts.size();
+  std::vector<vine_status> rc(k, VINE_OK);
+  std::vector<std::string> msg(k);
+  std::vector<std::thread> th;
+  for (int p = 0; p < k; p++) {
+    if (part_size(nq, k, p) == 0) continue;
+    th.emplace_back([&, p] {
+      rc[p] = fn(p);
+      if (rc[p] != VINE_OK) msg[p] = vine_last_error();
+    });
+  }
+  for (auto& t : th) t.join();
+  for (int p = 0; p < k; p++)
+    if (rc[p] != VINE_OK && rc[p] != VINE_E_OVERFLOW) {
+      set_error("part %d of %d (queries %d, %d + %d, ...): %s", p, k, p, p, k, msg[p].c_str());
+      return rc[p];
+    }
+  for (int p = 0; p < k; p++)
+    if (rc[p] == VINE_E_OVERFLOW) {
+      set_error("part %d of %d: %s", p, k, msg[p].c_str());
+      return rc[p];
+    }
+  return VINE_OK;
+}
+
+vine_status solve_split_device(vine_solver* s, int32_t nq, const int32_t* d_src, const int32_t* d_dst,
+                               const float* d_bound, float* d_cost, float* d_delay, int32_t* d_feas,
+                               vine_stats* out_stats) {
+  const int k = (int)s->parts.size();
+  const int32_t need = (nq + k - 1) / k;
+  if (need > s->split_cap) {
+    for (void* p : s->split_bufs) (void)hipFree(p);
+    s->split_bufs.assign(k, nullptr);
+    s->split_cap = 0;
+    for (int p = 0; p < k; p++) {
+      hipError_t e = hipMalloc(&s->split_bufs[p], (size_t)24 * need);
+      if (e != hipSuccess) {
+        set_error("hipMalloc of %zu bytes for the query parts: %s", (size_t)24 * need, hipGetErrorString(e));
+        return VINE_E_OOM;
+      }
+    }
+    s->split_cap = need;
+  }
+  std::vector<vine_stats> st(k);
+  const double t0 = now_ms();
+  vine_status r = run_parts(s, nq, [&](int p) -> vine_status {
+    vine_solver* c = s->parts[p];
+    const int32_t np = part_size(nq, k, p);
+    char* base = (char*)s->split_bufs[p];
+    int32_t* psrc = (int32_t*)base;
+    int32_t* pdst = psrc + s->split_cap;
+    float* pbound = (float*)(pdst + s->split_cap);
+    float* pcost = pbound + s->split_cap;
+    float* pdelay = pcost + s->split_cap;
+    int32_t* pfeas = (int32_t*)(pdelay + s->split_cap);
+    const dim3 grid((np + 255) / 256);
+    hipLaunchKernelGGL(k_part_gather, grid, dim3(256), 0, c->stream, nq, k, p, d_src, d_dst, d_bound, psrc, pdst,
+                       pbound);
+    HIP_TRY(hipGetLastError());
+    vine_status v = vine_solve_device(c, np, psrc, pdst, pbound, pcost, pdelay, pfeas, &st[p]);
+    if (v != VINE_OK) return v;
+    hipLaunchKernelGGL(k_part_scatter, grid, dim3(256), 0, c->stream, nq, k, p, pcost, pdelay, pfeas, d_cost,
+                       d_delay, d_feas);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return VINE_OK;
+  });
+  if (r != VINE_OK) return r;
+  vine_stats acc{};
+  for (int p = 0; p < k; p++) merge_stats(acc, st[p]);
+  acc.ms_total = now_ms() - t0;
+  if (out_stats) *out_stats = acc;
+  gather_records(s, nq);
+  return VINE_OK;
+}
+
+vine_status solve_split_host(vine_solver* s, int32_t nq, const int32_t* src, const int32_t* dst, const float* bound,
+                             float* out_cost, float* out_delay, int32_t* out_feasible, int64_t* out_path_off,
+                             int32_t* out_path, int64_t path_cap, vine_stats* out_stats) {
+  const int k = (int)s->parts.size();
+  struct Part {
+    std::vector<int32_t> src, dst, feas, path;
+    std::vector<float> bound, cost, delay;
+    std::vector<int64_t> off;
+    vine_stats st{};
+    bool path_overflow = false;
+  };
+  std::vector<Part> part(k);
+  const bool want_off = out_path_off != nullptr;
+  for (int p = 0; p < k; p++) {
+    Part& P = part[p];
+    const int32_t np = part_size(nq, k, p);
+    for (int32_t j = 0; j < np; j++) {
+      const int32_t i = j * k + p;
+      P.src.push_back(src[i]);
+      P.dst.push_back(dst[i]);
+      P.bound.push_back(bound[i]);
+    }
+    P.cost.resize(np);
+    P.delay.resize(np);
+    P.feas.resize(np);
+    if (want_off) P.off.resize(np + 1);
+    if (out_path) P.path.resize(std::max<int64_t>(path_cap, 1));
+  }
+  const double t0 = now_ms();
+  vine_status r = run_parts(s, nq, [&](int p) -> vine_status {
+    Part& P = part[p];
+    const int32_t np = (int32_t)P.src.size();
+    vine_status v = vine_solve(s->parts[p], np, P.src.data(), P.dst.data(), P.bound.data(), P.cost.data(),
+                               P.delay.data(), P.feas.data(), want_off ? P.off.data() : nullptr,
+                               out_path ? P.path.data() : nullptr, out_path ? path_cap : 0, &P.st);
+    // a part whose paths alone overflow has its answers and offsets; the
+    // total is checked below
+    if (v == VINE_E_OVERFLOW && out_path && P.off[np] > path_cap) {
+      P.path_overflow = true;
+      return VINE_OK;
+    }
+    return v;
+  });
+  if (r != VINE_OK) return r;
+  vine_stats acc{};
+  for (int p = 0; p < k; p++) {
+    const Part& P = part[p];
+    merge_stats(acc, P.st);
+    for (size_t j = 0; j < P.src.size(); j++) {
+      const size_t i = j * k + p;
+      out_cost[i] = P.cost[j];
+      out_delay[i] = P.delay[j];
+      out_feasible[i] = P.feas[j];
+    }
+  }
+  if (want_off) {
+    out_path_off[0] = 0;
+    for (int32_t i = 0; i < nq; i++) {
+      const Part& P = part[i % k];
+      const int32_t j = i / k;
+      out_path_off[i + 1] = out_path_off[i] + (P.off[j + 1] - P.off[j]);
+    }
+  }
+  acc.ms_total = now_ms() - t0;
+  if (out_stats) *out_stats = acc;
+  gather_records(s, nq);
+  if (out_path) {
+    bool overflow = out_path_off[nq] > path_cap;
+    for (const Part& P : part) overflow = overflow || P.path_overflow;
+    if (overflow) {
+      set_error("vine_solve: path_cap %lld too small (need %lld)", (long long)path_cap, (long long)out_path_off[nq]);
+      return VINE_E_OVERFLOW;
+    }
+    for (int32_t i = 0; i < nq; i++) {
+      const Part& P = part[i % k];
+      const int32_t j = i / k;
+      std::copy(P.path.begin() + P.off[j], P.path.begin() + P.off[j + 1], out_path + out_path_off[i]);
+    }
+  }
+  return VINE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -998,6 +1305,8 @@ vine_status vine_solve_device(vine_solver* s, int32_t nq, const int32_t* d_src, 
     set_error("vine_solve_device: invalid arguments");
     return VINE_E_ARG;
   }
+  if (!s->parts.empty()) return solve_split_device(s, nq, d_src, d_dst, d_bound, d_cost, d_delay, d_feasible,
+                                                   out_stats);
   vine_stats st{};
   double t0 = now_ms();
   if (nq > 0) {
@@ -1016,7 +1325,7 @@ vine_status vine_solve_device(vine_solver* s, int32_t nq, const int32_t* d_src, 
       return VINE_E_ARG;
     }
   }
-  vine_status r = solve_impl(s, nq, d_src, d_dst, d_bound, d_cost, d_delay, d_feasible, nullptr, &st);
+  vine_status r = solve_impl(s, nq, d_src, d_dst, d_bound, d_cost, d_delay, d_feasible, nullptr, &st, 0, nq);
   st.ms_total = now_ms() - t0;
   st.ms_kernels = st.ms_expand + st.ms_merge + st.ms_sssp + st.ms_pull;
   if (out_stats) *out_stats = st;
@@ -1033,6 +1342,9 @@ vine_status vine_solve(vine_solver* s, int32_t nq, const int32_t* src, const int
   }
   vine_status v = check_queries_host(s, nq, src, dst, bound);
   if (v != VINE_OK) return v;
+  if (!s->parts.empty())
+    return solve_split_host(s, nq, src, dst, bound, out_cost, out_delay, out_feasible, out_path_off, out_path,
+                            path_cap, out_stats);
   vine_stats st{};
   double t0 = now_ms();
   if (nq == 0) {
@@ -1055,7 +1367,7 @@ vine_status vine_solve(vine_solver* s, int32_t nq, const int32_t* src, const int
   for (int32_t q0 = 0; q0 < nq; q0 += s->B) {
     int nb = std::min<int32_t>(s->B, nq - q0);
     v = solve_impl(s, nb, s->d_in_q + q0, s->d_in_q + nq + q0, s->d_in_bound + q0, s->d_out_cost + q0,
-                   s->d_out_delay + q0, s->d_out_feas + q0, s->d_out_plen + q0, &st);
+                   s->d_out_delay + q0, s->d_out_feas + q0, s->d_out_plen + q0, &st, q0, nq);
     if (v != VINE_OK) return v;
     if (want_path || out_path_off) {
       HIP_TRY(hipMemcpy(plen.data() + q0, s->d_out_plen + q0, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
@@ -1141,6 +1453,11 @@ vine_status vine_solver_set_mlp(vine_solver* s, const float* params, int64_t npa
     set_error("vine_solver_set_mlp: invalid arguments");
     return VINE_E_ARG;
   }
+  for (vine_solver* p : s->parts) {
+    vine_status r = vine_solver_set_mlp(p, params, nparams);
+    if (r != VINE_OK) return r;
+  }
+  if (!s->parts.empty()) return VINE_OK;
   if (!s->d_mlp || s->mlp_nparams != nparams) {
     if (s->d_mlp) {
       auto it = std::find(s->allocs.begin(), s->allocs.end(), (void*)s->d_mlp);
@@ -1161,6 +1478,10 @@ int64_t vine_solver_pareto(const vine_solver* s, int32_t q, int32_t v, float* ou
   if (!s || q < 0 || q >= s->B || v < 0 || v >= s->g->n || (cap > 0 && !out_pairs)) {
     set_error("vine_solver_pareto: invalid arguments");
     return VINE_E_ARG;
+  }
+  if (!s->parts.empty()) {
+    const int k = (int)s->parts.size();
+    return vine_solver_pareto(s->parts[q % k], q / k, v, out_pairs, cap);
   }
   int2 ps;
   hipError_t e = hipMemcpy(&ps, s->d_pset + (long long)q * s->g->n + s->g->perm[v], sizeof(int2),

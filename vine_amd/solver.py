@@ -50,7 +50,8 @@ class Options(ctypes.Structure):
         ("bound_ball", ctypes.c_int32),
         ("engine", ctypes.c_int32),
         ("labels_per_wave", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("concurrency", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 
