@@ -683,7 +683,7 @@ vine_status poll(vine_solver* s) {
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (s->h_ct->overflow) {
     set_error("device capacity exceeded (flags 0x%x: 1 labels, 2 candidates, 4 touched, 8 pool, "
-            "16 carry, 32 bound frontier); raise the capacities in vine_options",
+            "16 carry, 32 bound frontier, 64 far tier (frontier_capacity)); raise the capacities in vine_options",
             s->h_ct->overflow);
     return VINE_E_OVERFLOW;
   }
