@@ -651,43 +651,79 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
 
 // ---------------------------------------------------------------------------
 // Append a carried entry to bucket fb of query q's far tier: lock-free on the
-// bucket's chunk list (as the candidate lists); a full head chunk is followed
-// by one twice its size.
-// The caller counts the entry in far_cnt[q].
+// bucket's chunk list (as the candidate lists).  Entries go into the head
+// chunk while it has room.  The entry that takes slot 3/4 of a head chunk
+// also links an empty successor twice its size in front, so that the many
+// waves appending to one bucket move on to it instead of each allocating a
+// chunk of their own when the head fills up.  Entries that still find the
+// head full (or the bucket empty) retry once on a newer head, then are
+// grouped by bucket within the wave, and one lane per bucket allocates a
+// chunk just large enough for them.  Called by the lanes with an entry to
+// park (any subset of the wave).  The caller counts the entry in far_cnt[q].
+__device__ __forceinline__ int far_fit_class(int n) {
+  int k = 0;
+  while ((FU << k) < n && k < FCLS_MAX) k++;
+  return k;
+}
+
+__device__ __forceinline__ void far_link(const Ctx& c, int* headp, int q, int fb, int blk, int cls, int count,
+                                         int expected) {
+  const int head = (blk << 4) | cls;
+  for (;;) {
+    c.fhd[blk] = make_int2(count, expected);
+    __threadfence();  // records and header before the link
+    const int prev = atomicCAS(headp, expected, head);
+    if (prev == expected) break;
+    expected = prev;
+  }
+  // a bucket becomes non-empty only through a CAS from -1
+  if (expected < 0) atomicOr(&c.far_occ[(long long)q * NBW + (fb >> 6)], 1ull << (fb & 63));
+}
+
 __device__ __forceinline__ void far_append(const Ctx& c, Counters* ct, int q, int fb, const int4& crec4) {
   int* headp = &c.far_head[(long long)q * NBK + fb];
-  const int hv = __hip_atomic_load(headp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int hv = __hip_atomic_load(headp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   bool done = false;
-  int cls = 0;
-  if (hv >= 0) {
-    const int blk = hv >> 4;
-    cls = (hv & 15) + 1;
+  for (int attempt = 0; attempt < 2 && hv >= 0 && !done; attempt++) {
+    const int blk = hv >> 4, cls = hv & 15;
+    const int cap = FU << cls;
     const int slot = atomicAdd(&c.fhd[blk].x, 1);
-    if (slot < (FU << (hv & 15))) {
+    if (slot < cap) {
       c.frec[(long long)blk * FU + slot] = crec4;
       done = true;
+      if (slot == cap - cap / 4 && cls < FCLS_MAX) {
+        // successor of this chunk, linked empty
+        const int ncls = cls + 1;
+        const int nb = atomicAdd(&ct->far_chunks, 1 << ncls);
+        if (nb < 0 || nb > c.far_cap - (1 << ncls)) atomicOr(&ct->overflow, OVF_FAR);
+        else far_link(c, headp, q, fb, nb, ncls, 0, hv);
+      }
+    } else {
+      const int hv2 = __hip_atomic_load(headp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (hv2 == hv) break;
+      hv = hv2;
     }
   }
-  if (!done) {
-    cls = min(cls, FCLS_MAX);
-    // units come from a bump allocator reset with each batch
-    const int blk = atomicAdd(&ct->far_chunks, 1 << cls);
-    if (blk < 0 || blk > c.far_cap - (1 << cls)) {
-      atomicOr(&ct->overflow, OVF_FAR);
+  const int lane = lane_id();
+  unsigned long long pend = __ballot(!done);
+  while (pend) {
+    const int leader = __ffsll((long long)pend) - 1;
+    const int lq = __shfl(q, leader, WAVE), lfb = __shfl(fb, leader, WAVE);
+    const bool mine = !done && q == lq && fb == lfb;
+    const unsigned long long mm = __ballot(mine);
+    const int n_same = __popcll(mm);
+    const int ncls = far_fit_class(n_same);
+    int blk = 0;
+    if (lane == leader) blk = atomicAdd(&ct->far_chunks, 1 << ncls);  // bump allocator, reset per batch
+    blk = __shfl(blk, leader, WAVE);
+    if (blk < 0 || blk > c.far_cap - (1 << ncls)) {
+      if (lane == leader) atomicOr(&ct->overflow, OVF_FAR);
     } else {
-      c.frec[(long long)blk * FU] = crec4;
-      const int mine = (blk << 4) | cls;
-      int expected = hv;
-      for (;;) {
-        c.fhd[blk] = make_int2(1, expected);
-        __threadfence();
-        int prev = atomicCAS(headp, expected, mine);
-        if (prev == expected) break;
-        expected = prev;
-      }
-      // a bucket becomes non-empty only through a CAS from -1
-      if (expected < 0) atomicOr(&c.far_occ[(long long)q * NBW + (fb >> 6)], 1ull << (fb & 63));
+      if (mine) c.frec[(long long)blk * FU + __popcll(mm & ((1ull << lane) - 1ull))] = crec4;
+      if (lane == leader) far_link(c, headp, q, fb, blk, ncls, n_same, hv);
     }
+    if (mine) done = true;
+    pend &= ~mm;
   }
 }
 

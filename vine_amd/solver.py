@@ -83,6 +83,7 @@ class Stats(ctypes.Structure):
         ("far_appended", ctypes.c_int64),
         ("scanned", ctypes.c_int64),
         ("timed_rounds", ctypes.c_int64),
+        ("far_units", ctypes.c_int64),
     ]
 
     def as_dict(self):
