@@ -41,8 +41,10 @@ def main():
     ap.add_argument("--delta", type=float, default=None,
                     help="window (fixed) or first window (mlp); default 12 and 4 key units")
     ap.add_argument("--model", default=os.path.join(ROOT, "vine_amd", "models", "delta_mlp.npy"))
-    ap.add_argument("--concurrency", type=int, default=3,
-                    help="internal solvers on separate HIP streams (vine_options.concurrency)")
+    ap.add_argument("--concurrency", type=int, default=None,
+                    help="internal solvers on separate HIP streams (vine_options.concurrency); default: the "
+                         "autotuned choice cached in vine_amd/models/autotune.json, else 3")
+    ap.add_argument("--no-tuned", action="store_true", help="ignore the autotune cache (library launch defaults)")
     ap.add_argument("--plain-delta", type=float, default=12.0, help="window of the plain single-query solver")
     ap.add_argument("--plain-queries", type=int, default=100, help="per bound factor")
     ap.add_argument("--cpu-queries", type=int, default=32)
@@ -59,6 +61,7 @@ def main():
     import numpy as np
     import torch
 
+    from vine_amd import autotune as AT
     from vine_amd import solver as S
     from vine_amd import workload as W
 
@@ -79,15 +82,19 @@ def main():
     if args.delta is None:
         args.delta = 4.0 if args.policy == "mlp" else 12.0
     dg = S.DeviceGraph(g)
-    solver = S.Solver(dg, max_batch=args.batch, prune=S.PRUNE_BOUNDS, delta_policy=policy,
-                      delta_fixed=args.delta, rounds_per_sync=16, time_kernels=TIME_EVERY,
-                      concurrency=args.concurrency,
-                      label_capacity=1_000_000_000, pool_capacity=2_000_000_000,
-                      cand_capacity=100_000_000, frontier_capacity=400_000_000)
+    base = dict(max_batch=args.batch, prune=S.PRUNE_BOUNDS, delta_policy=policy, delta_fixed=args.delta,
+                rounds_per_sync=16, **AT.BENCH_CAPACITIES)
+    launch = {"concurrency": 3}
+    tuned = None if args.no_tuned else AT.cached_options(g, **base)
+    if tuned:
+        launch = tuned
+    if args.concurrency is not None:
+        launch["concurrency"] = args.concurrency
+    log("launch options: %s (%s)" % (launch, "autotune cache" if tuned else "defaults"))
+    solver = S.Solver(dg, time_kernels=TIME_EVERY, **base, **launch)
     if policy == S.DELTA_MLP:
         solver.set_mlp(np.load(args.model))
-    log("solver ready (batch %d, policy %s, window %g, concurrency %d)"
-        % (args.batch, args.policy, args.delta, args.concurrency))
+    log("solver ready (batch %d, policy %s, window %g)" % (args.batch, args.policy, args.delta))
     nq = len(qb)
     d_src = torch.from_numpy(qb.src).to(dev)
     d_dst = torch.from_numpy(qb.dst).to(dev)
@@ -216,11 +223,14 @@ def main():
         wall = time.perf_counter() - t1
         done = sum(1 for o in out if o[1])
         agree = all(o[2] for o in out)
+        busy = sum(o[0] for o in out)
         cpu = {"value": len(idx) / wall, "unit": "queries/s", "cores": threads, "kind": "port",
+               "single_core_value": len(idx) / busy,
                "sample": "%d random queries of the batched workload, oracle (restated cspp_cpu with the same "
                          "lower-bound pruning, whole-frontier rounds) on %d threads; %d finished, %d stopped at "
                          "%.0f s CPU each and counted at that time (so the CPU rate is an upper bound); "
-                         "finished answers agree with the GPU: %s"
+                         "finished answers agree with the GPU: %s; single_core_value = queries / summed "
+                         "per-query time (one core, sequential)"
                          % (len(idx), threads, done, len(idx) - done, args.cpu_limit, agree)}
         log("cpu: %.2f queries/s (%d/%d finished)" % (cpu["value"], done, len(idx)))
 
@@ -240,7 +250,7 @@ def main():
         "config": {"workload": "ny_batch: 264,346-vertex 3-NN graph (%d arcs), %d queries, hop-distance "
                                "log-uniform [10,2000], bound U[1.05,2.0] x min delay" % (g.m, nq),
                    "batch": args.batch, "policy": args.policy, "delta": args.delta, "prune": "bounds",
-                   "concurrency": args.concurrency,
+                   "launch": launch, "launch_source": "autotune cache" if tuned else "defaults",
                    "plain": "1 query per solve, fixed window %g, random pairs bound 1.1x (%d queries)"
                             % (args.plain_delta, args.plain_queries),
                    "parallelism": "single GPU"},

@@ -222,3 +222,22 @@ def test_locality_order_is_a_permutation_that_packs_neighbours():
     assert fn(4, row2.ctypes.data, dst2.ctypes.data, o2.ctypes.data) == 0
     assert sorted(o2.tolist()) == [0, 1, 2, 3]
 
+
+
+def test_autotune_cache_round_trip(tmp_path):
+    """The autotuner's cache key follows the graph, batch size and policy, and a
+    cached choice is returned unchanged (no GPU: the search itself is not run)."""
+    from vine_amd import autotune as AT
+
+    g = W.grid(8, 8, seed=1)
+    h = W.grid(8, 8, seed=2)
+    k = AT.graph_key(g, 64, S.DELTA_MLP)
+    assert k == AT.graph_key(W.grid(8, 8, seed=1), 64, S.DELTA_MLP)
+    assert k != AT.graph_key(h, 64, S.DELTA_MLP) and k != AT.graph_key(g, 32, S.DELTA_MLP)
+    assert k != AT.graph_key(g, 64, S.DELTA_FIXED)
+    cache = tmp_path / "tune.json"
+    assert AT.cached_options(g, cache=str(cache), max_batch=64, delta_policy=S.DELTA_MLP) is None
+    cache.write_text(json.dumps({k: {"options": {"concurrency": 2, "block_size": 64, "grid_blocks": 768}}}))
+    got = AT.cached_options(g, cache=str(cache), max_batch=64, delta_policy=S.DELTA_MLP)
+    assert got == {"concurrency": 2, "block_size": 64, "grid_blocks": 768}
+    assert AT.cached_options(h, cache=str(cache), max_batch=64, delta_policy=S.DELTA_MLP) is None
