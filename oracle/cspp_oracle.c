@@ -40,6 +40,7 @@
  * id (algorithm/cspp_cpu.hpp:111-115); constraint delay <= bound
  * (include/constraint.hpp:99-104).
  */
+#define _POSIX_C_SOURCE 199309L /* clock_gettime */
 #include <float.h>
 #include <math.h>
 #include <stdint.h>
@@ -53,6 +54,14 @@ typedef struct {
   float delay;
   int32_t parent;
 } olabel;
+
+/* CPU time of the calling thread in seconds (queries solved on several
+ * threads each get their own time limit) */
+static double thread_cpu_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_THREAD_CPUTIME_ID, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
 
 typedef struct {
   int32_t* a;
@@ -98,7 +107,7 @@ enum { OC_X = 0, OC_E, OC_I, OC_ROUNDS, OC_MAXF, OC_SUMF, OC_NCOUNT };
  *      when (d + lbl, delay + lbd) is lexicographically >= (bd, bdl).
  * out_pareto receives t's Pareto set sorted by d as (d, delay) pairs.
  * frontier_trace (optional, length trace_cap) receives the frontier size after
- * every round.  time_limit_s > 0 aborts with -4 once that much CPU time is spent
+ * every round.  time_limit_s > 0 aborts with -4 once the calling thread has spent that much CPU time
  * (checked once per round). */
 int oracle_cspp(int32_t n, const int32_t* row_ptr, const int32_t* dst, const float* len,
                 const float* delay, int32_t s, int32_t t, float bound, int32_t prune_target,
@@ -158,10 +167,10 @@ int oracle_cspp(int32_t n, const int32_t* row_ptr, const int32_t* dst, const flo
     bdl = 0.0f;
   }
   int32_t rounds = 0;
-  clock_t t_begin = clock();
+  const double t_begin = thread_cpu_s();
   while (frontier.n > 0) {
     rounds++;
-    if (time_limit_s > 0 && (double)(clock() - t_begin) / CLOCKS_PER_SEC > time_limit_s) {
+    if (time_limit_s > 0 && thread_cpu_s() - t_begin > time_limit_s) {
       rc = -4; /* time limit reached: no answer */
       goto done;
     }
