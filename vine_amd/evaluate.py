@@ -45,6 +45,7 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--model", default=L.DEFAULT_MODEL)
     ap.add_argument("--engine", type=int, default=S.ENGINE_AUTO)
+    ap.add_argument("--concurrency", type=int, default=1, help="internal solvers on separate HIP streams")
     ap.add_argument("--label-capacity", type=float, default=6e8)
     ap.add_argument("--ar-max-labels", type=float, default=0.0,
                     help="MAX of the AR rule (0 = the reference's: free memory / 12 B)")
@@ -63,10 +64,12 @@ def main(argv=None):
     for text in args.policies:
         policy, delta = parse_policy(text)
         cap = int(args.label_capacity)
-        line = {"workload": args.workload, "policy": text, "queries": len(qs), "batch": args.batch}
+        line = {"workload": args.workload, "policy": text, "queries": len(qs), "batch": args.batch,
+                "concurrency": args.concurrency}
         try:
             with S.Solver(dg, max_batch=args.batch, prune=S.PRUNE_BOUNDS, delta_policy=policy, delta_fixed=delta,
                           rounds_per_sync=16 if policy != S.DELTA_AR else 1, engine=args.engine,
+                          concurrency=args.concurrency,
                           label_capacity=cap, pool_capacity=2 * cap, cand_capacity=100_000_000,
                           frontier_capacity=cap // 2, ar_max_labels=args.ar_max_labels) as s:
                 if policy == S.DELTA_MLP:
