@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void k_fam_expand(FamCtx c, int cur) {
         }
       if (dominated) continue;
       const int id = atomicAdd(&c.ct->label_ctr, 1);
-      if (id >= c.cap) {
+      if (id < 0 || id >= c.cap) {  // the counter keeps counting (and may wrap) once full
         atomicOr(&c.ct->overflow, 1);
         continue;
       }
@@ -349,7 +349,14 @@ vine_status run_batch(vine_family_solver* s, int nb, int64_t* counters) {
   int cur = 0;
   const int sync = 8;
   const int grid = 2048;
+  // every family here converges (dominance discards cycles); the cap only
+  // turns a runaway search into an error
+  const long long max_rounds = 64LL * n + 1024;
   for (long long r = 0;; r++) {
+    if (r > max_rounds) {
+      set_error("vine_family_solve: no convergence after %lld rounds", r);
+      return VINE_E_HIP;
+    }
     hipLaunchKernelGGL(k_fam_expand<Label_t>, dim3(grid), dim3(kBlock), 0, st, c, cur);
     hipLaunchKernelGGL(k_fam_merge<Label_t>, dim3(grid), dim3(kBlock), 0, st, c, cur);
     hipLaunchKernelGGL(k_fam_swap, dim3(1), dim3(64), 0, st, c, cur);
