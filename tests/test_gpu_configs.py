@@ -5,13 +5,16 @@ oracle.
 
 C4: R-MAT scale 17 (131,072 vertices, hubs of hundreds of arcs), 500 pairs
 from the largest component, bound 1.3x the min delay.
-C2: NY-scale 3-NN graph (264,346 vertices), 1,000 random pairs, bound 1.1x.
+C2: NY-scale 3-NN graph (264,346 vertices), 1,000 random pairs, bounds 1.1x and 1.5x.
+C3: NE-scale 3-NN graph (1,524,453 vertices), the first 300 of its 1,000 pairs,
+bound 1.2x, learned window against a fixed one, split over three streams.
 """
 import numpy as np
 import pytest
 
 from helpers import check_path
 from oracle import oracle as O
+from vine_amd import learn as L
 from vine_amd import solver as S
 from vine_amd import workload as W
 
@@ -70,3 +73,34 @@ def test_ny_random_pairs_full(gpu):
     # the CPU oracle on the shortest queries (long random pairs take it minutes)
     idx = np.argsort(q11.bound)[:8]
     assert _oracle_sample(g, q11, a, idx, time_limit=8.0) >= 6
+
+
+@pytest.mark.gpu
+def test_ny_random_pairs_loose_bounds(gpu):
+    g, _, q15 = W.preset("ny")
+    print("ny: %d queries at 1.5x" % len(q15), flush=True)
+    a = _solve(g, q15, 1000, delta_policy=S.DELTA_FIXED, delta_fixed=12.0)
+    b = _solve(g, q15, 1000, delta_policy=S.DELTA_FIXED, delta_fixed=40.0, concurrency=2)
+    assert np.array_equal(a.cost, b.cost) and np.array_equal(a.delay, b.delay)
+    assert np.array_equal(a.feasible, b.feasible)
+    _check_all(g, q15, a)
+    idx = np.argsort(q15.bound)[:6]
+    assert _oracle_sample(g, q15, a, idx, time_limit=8.0) >= 4
+
+
+@pytest.mark.gpu
+def test_ne_sample_learned_vs_fixed(gpu):
+    g, q = W.preset("ne")
+    q = q.subset(np.arange(300))
+    params = np.load(L.DEFAULT_MODEL)
+    print("ne: n=%d m=%d, %d queries at 1.2x" % (g.n, g.m, len(q)), flush=True)
+    dg = S.DeviceGraph(g)
+    a = _solve(dg, q, 300, delta_policy=S.DELTA_FIXED, delta_fixed=12.0, label_capacity=300_000_000)
+    with S.Solver(dg, max_batch=300, prune=S.PRUNE_BOUNDS, rounds_per_sync=16, delta_policy=S.DELTA_MLP,
+                  delta_fixed=4.0, concurrency=3, label_capacity=300_000_000) as s:
+        s.set_mlp(params)
+        b = s.solve(q.src, q.dst, q.bound)
+    assert np.array_equal(a.cost, b.cost) and np.array_equal(a.delay, b.delay)
+    assert np.array_equal(a.feasible, b.feasible)
+    _check_all(g, q, b)
+    assert b.stats["expanded"] < a.stats["expanded"]
