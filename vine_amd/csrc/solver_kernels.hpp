@@ -787,36 +787,47 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, int lpw) {
       } else {
         id = (int)(lstart + (i - nold));
       }
-      if (id >= 0 && !c.dead[id]) {
-        bool live = true;
-        if (fresh) {
-          int4 L = c.labs[id];
-          int2 meta = c.lmeta[id];
+      if (id >= 0) {
+        // A tail round is a chain of dependent loads, so every load this
+        // entry may need is issued as soon as its address is known, in two
+        // steps (by id, then by query and vertex), whether or not the entry
+        // turns out dead, pruned or outside the window.
+        const bool dead = c.dead[id] != 0;
+        const int4 L = c.labs[id];
+        const int2 meta = fresh ? c.lmeta[id] : make_int2(q, __float_as_int(key));
+        // an id reserved by merge but never filled is dead and holds stale
+        // contents: its query and vertex are used only clamped into range,
+        // for loads whose values are then ignored
+        const int qq = min(max(meta.x, 0), c.B - 1);
+        const int vv = min(max(L.x, 0), c.n - 1);
+        const float2 b = c.q_best[qq];
+        const float lim = __uint_as_float(fmin_cur[qq]) + c.q_delta[qq];
+        const int qdst = c.q_dst[qq];
+        const float qbound = c.q_bound[qq];
+        const float lbdv = (PRUNE == 2 && fresh) ? lbd_of(c, (long long)qq * c.n + vv) : 0.f;
+        const int row0 = c.row_ptr[vv];
+        const int row1 = c.row_ptr[vv + 1];
+        bool live = !dead;
+        if (live) {
           q = meta.x;
           key = __int_as_float(meta.y);
           lab = LT::unpack(L.x, make_float2(__int_as_float(L.y), __int_as_float(L.z)));
+        }
+        if (live && fresh) {
           rd = lab.delay;
-          if (PRUNE != 0 && lab.vid == c.q_dst[q]) live = false;  // answers are not extended
+          if (PRUNE != 0 && lab.vid == qdst) live = false;  // answers are not extended
           if (live && PRUNE == 2) {
-            rd = lab.delay + lbd_of(c, (long long)q * c.n + lab.vid);
-            if (rd > c.q_bound[q]) live = false;
+            rd = lab.delay + lbdv;
+            if (rd > qbound) live = false;
           }
         }
         // (key, rd) is (d, delay) without bounds and (d + lbl, delay + lbd) with them
-        if (live && PRUNE != 0) {
-          float2 b = c.q_best[q];
-          if (lex_ge(key, rd, b.x, b.y)) live = false;
-        }
+        if (live && PRUNE != 0 && lex_ge(key, rd, b.x, b.y)) live = false;
         if (live) {
           c.q_last[q] = r;
-          const bool in_window = !(key > __uint_as_float(fmin_cur[q]) + c.q_delta[q]);
-          if (in_window) {
-            if (!fresh) {
-              int4 L = c.labs[id];
-              lab = LT::unpack(L.x, make_float2(__int_as_float(L.y), __int_as_float(L.z)));
-            }
-            row = c.row_ptr[lab.vid];
-            deg = c.row_ptr[lab.vid + 1] - row;
+          if (!(key > lim)) {
+            row = row0;
+            deg = row1 - row0;
             n_exp++;
             expd = true;
           } else {
@@ -852,50 +863,52 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, int lpw) {
     n_near += near ? 1 : 0;
     n_scan += i < fn ? 1 : 0;
     float ld = lab.d, ldl = lab.delay;
+    // the owner's query bound and best answer travel with the label
+    const float obound = (i < fn && id >= 0) ? c.q_bound[q] : 0.f;
+    const float2 obest = (PRUNE != 0 && i < fn && id >= 0) ? c.q_best[q] : make_float2(0.f, 0.f);
     wave_flat(deg, row, [&](int owner, int pos, bool act) {
+      // shuffles by every lane (a lane reads its owner's registers)
       Label_t src = LT::unpack(0, make_float2(__shfl(ld, owner, WAVE), __shfl(ldl, owner, WAVE)));
       const int oq = __shfl(q, owner, WAVE);
       const int oid = __shfl(id, owner, WAVE);
+      const float2 b = make_float2(__shfl(obest.x, owner, WAVE), __shfl(obest.y, owner, WAVE));
+      const float bound = __shfl(obound, owner, WAVE);
       bool ok = false;
       Label_t son;
       long long qv = 0;
+      int h = -1;
       if (act) {
         n_rel++;
         int4 e = c.edges[pos];
         Edge_t ed(__int_as_float(e.y), __int_as_float(e.z));
         src.expand(e.x, ed, &son);
-        Constraint_t C(c.q_bound[oq]);
+        Constraint_t C(bound);
         ok = C.pass_constraints_check(son);
         qv = (long long)oq * c.n + son.vid;
-        if (ok && PRUNE == 1 && son.vid != c.q_dst[oq]) {
-          float2 b = c.q_best[oq];
-          if (lex_ge(son.d, son.delay, b.x, b.y)) ok = false;
-        }
+        // everything the checks and the append read at (query, vertex), at once
+        const int2 ps = c.pset[qv];
+        h = __hip_atomic_load(&c.cand_head[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const float lbdv = PRUNE == 2 ? lbd_of(c, qv) : 0.f;
+        const float lblv = PRUNE == 2 ? c.lbl[qv] : 0.f;
+        if (ok && PRUNE == 1 && son.vid != c.q_dst[oq] && lex_ge(son.d, son.delay, b.x, b.y)) ok = false;
         if (ok && PRUNE == 2) {
-          float rd = son.delay + lbd_of(c, qv);
-          if (rd > C.delay_L) {
-            ok = false;
-          } else {
-            float2 b = c.q_best[oq];
-            if (lex_ge(son.d + c.lbl[qv], rd, b.x, b.y)) ok = false;
-          }
+          const float rd = son.delay + lbdv;
+          if (rd > C.delay_L || lex_ge(son.d + lblv, rd, b.x, b.y)) ok = false;
         }
-        if (ok) {
-          // read-only dominance check against the vertex's front (no writer
-          // runs now); candidates of this round are checked against each
-          // other in merge
-          int2 ps = c.pset[qv];
-          if (front_dominates<Label_t>(c.pool_cd + (uint32_t)ps.x, pset_cnt(ps), son.vid, son)) ok = false;
-        }
+        // read-only dominance check against the vertex's front (no writer
+        // runs now); candidates of this round are checked against each
+        // other in merge
+        if (ok && front_dominates<Label_t>(c.pool_cd + (uint32_t)ps.x, pset_cnt(ps), son.vid, son)) ok = false;
       }
       // lock-free append to the (query, vertex) chunk list: claim a slot in
-      // the head chunk, or publish a new head chunk holding the record
+      // the head chunk, or publish a new head chunk holding the record (a
+      // head read before other lanes published theirs is still a chunk of
+      // this round's list; the CAS below links behind whatever is current)
       bool first = false;
       if (ok) {
         n_cand++;
         float2 pk = LT::pack(son);
         int4 rec = make_int4(__float_as_int(pk.x), __float_as_int(pk.y), oid, 0);
-        int h = __hip_atomic_load(&c.cand_head[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         bool done = false;
         if (h >= 0) {
           int slot = atomicAdd(&c.chd[h].x, 1);
