@@ -1026,10 +1026,12 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
       int n8 = 0, nexth = -1;
       int4 myrec = make_int4(0, 0, -1, 0);
       if (h >= 0) {
-        int2 hd = c.chd[h];
+        // the chunk's records are read with its header, not after it (lanes
+        // past the count read slots that are ignored)
+        const int2 hd = c.chd[h];
+        if (gl < CCHUNK) myrec = c.crec[(long long)h * CCHUNK + gl];
         n8 = min(hd.x, CCHUNK);
         nexth = hd.y;
-        if (gl < n8) myrec = c.crec[(long long)h * CCHUNK + gl];
       }
       int kmax = n8;
 #pragma unroll
