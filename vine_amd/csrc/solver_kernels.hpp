@@ -1223,21 +1223,26 @@ __device__ void pull_query(const Ctx& c, long long q) {
   const int nc = rn & 1;
   uint32_t* fmin = c.q_fmin + (size_t)nc * c.B;
   const int lane = lane_id();
+  // the query's state in one step of loads (a pull is a chain of dependent
+  // reads: state, occupancy words, bucket heads, chunk headers)
   const int nb = c.far_next[q];
-  if (nb >= NBK) return;
   const float front = __uint_as_float(load_agent_u32(&fmin[q]));
-  const float reach = fmaxf(FAR_MULT * c.q_delta[q], c.far_floor);
-  if (c.far_cnt[q] <= 0) {
+  const float qdelta = c.q_delta[q];
+  const int fcnt = c.far_cnt[q];
+  const float2 best = c.q_best[q];
+  const float fbase = c.far_base[q], fbw = c.far_bw[q];
+  if (nb >= NBK) return;
+  const float reach = fmaxf(FAR_MULT * qdelta, c.far_floor);
+  if (fcnt <= 0) {
     // nothing parked: only keep the near/far boundary up with the front
     if (lane == 0 && front < FINF) c.far_next[q] = max(nb, far_bucket(c, (int)q, front + reach) + 1);
     return;
   }
-  const float2 best = c.q_best[q];
-  if (PRUNE != 0 && c.far_base[q] + (float)nb * c.far_bw[q] > best.x) {
+  if (PRUNE != 0 && fbase + (float)nb * fbw > best.x) {
     // every parked key exceeds t's best answer: the whole far tier is
     // pruned without reading it (its chunks are reclaimed at the next batch)
     if (lane == 0) {
-      const int left = c.far_cnt[q];
+      const int left = fcnt;
       c.far_cnt[q] = 0;
       c.far_next[q] = NBK;
       atomicAdd((unsigned long long*)&ct->far_live, (unsigned long long)(-(long long)left));
