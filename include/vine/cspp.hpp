@@ -10,9 +10,11 @@
 // always 0, and every task uses its own constraint (the reference passes
 // C_l[0] for all tasks, algorithm/cspp.cuh:196,234).
 //
-// The C ABI carries the c_label / c_constraint / c_edge instantiation; other
-// label types need the kernel templates in vine_amd/csrc/solver_kernels.hpp
-// compiled for them.
+// cspp<c_label, c_edge> runs the batched solver (vine_solve); the other label
+// families of vine/families.hpp (hop-bounded, hop + max arc length, 2-OSP,
+// social, QoS) run the generic search (vine_family_solve), with the answer's
+// fields in label_l.  A new label type needs a family_traits specialisation
+// and the kernel templates of vine_amd/csrc/families.hip compiled for it.
 #pragma once
 
 #include <cstdint>
@@ -22,6 +24,7 @@
 #include <vector>
 
 #include "vine.h"
+#include "vine/families.hpp"
 #include "vine/label.hpp"
 #include "vine_workload.h"
 
@@ -67,11 +70,14 @@ class Graph {
   std::vector<std::vector<std::pair<int, Edge_t>>> adj_;
 };
 
+// Label_t: c_label (the batched solver of vine_solve) or any label family of
+// vine/families.hpp (the generic search of vine_family_solve); the
+// constraint type is the family's c_constraint<Label_t>.
 template <typename Label_t, typename Edge_t>
 class cspp {
-  static_assert(std::is_same<Label_t, c_label>::value && std::is_same<Edge_t, c_edge>::value,
-                "the C ABI carries the c_label / c_edge instantiation");
-  using Constraint_t = c_constraint<Label_t>;
+  static_assert(std::is_same<Edge_t, c_edge>::value, "the C ABI carries c_edge arcs");
+  using Traits = family_traits<Label_t>;
+  using Constraint_t = typename Traits::constraint;
 
  public:
   cspp(Graph<Edge_t>* g, int ntask, std::vector<Label_t>& source_state, std::vector<int>& sink_l,
@@ -92,6 +98,11 @@ class cspp {
     vine_status st = vine_graph_create(g_->N, (int64_t)dst.size(), row.data(), dst.data(), len.data(),
                                        delay.data(), &dg);
     if (st != VINE_OK) return fail(st);
+    if (Traits::family != VINE_FAMILY_CSP) {
+      const int fst = solve_family(dg);
+      vine_graph_destroy(dg);
+      return fst;
+    }
     vine_solver* sv = nullptr;
     st = vine_solver_create(dg, &options, &sv);
     if (st != VINE_OK) {
@@ -142,11 +153,55 @@ class cspp {
   std::vector<int> feasible_l;
   std::vector<std::vector<int>> path_l;
   std::vector<int> size_of_result_l;
+  // other families: the answer's (d, delay, hop, maxcost) per task
+  std::vector<float> label_l;
   std::string last_error;
 
  private:
   int fail(vine_status st) {
     last_error = vine_last_error();
+    return st;
+  }
+
+  // the generic label-correcting search for the other families
+  int solve_family(vine_graph* dg) {
+    vine_family_solver* fs = nullptr;
+    const int batch = options.max_batch > 1 ? options.max_batch : (ntask < 64 ? (ntask > 0 ? ntask : 1) : 64);
+    vine_status st = vine_family_solver_create(dg, Traits::family, batch, 0, &fs);
+    if (st != VINE_OK) return fail(st);
+    std::vector<int32_t> src(ntask), snk(sink_l.begin(), sink_l.end());
+    std::vector<float> lim(3 * (size_t)ntask);
+    for (int i = 0; i < ntask; i++) {
+      src[i] = init_state[i].vid;
+      Traits::limits(C_l[i], &lim[3 * (size_t)i]);
+    }
+    label_l.assign(4 * (size_t)ntask, 0.f);
+    std::vector<int32_t> feas(ntask, 0);
+    std::vector<int64_t> off(ntask + 1, 0);
+    std::vector<int32_t> path((size_t)ntask * 64 + 1);
+    st = vine_family_solve(fs, ntask, src.data(), snk.data(), lim.data(), label_l.data(), feas.data(), off.data(),
+                           path.data(), (int64_t)path.size(), nullptr);
+    if (st == VINE_E_OVERFLOW && off[ntask] > (int64_t)path.size()) {
+      path.resize(off[ntask]);
+      st = vine_family_solve(fs, ntask, src.data(), snk.data(), lim.data(), label_l.data(), feas.data(),
+                             off.data(), path.data(), (int64_t)path.size(), nullptr);
+    }
+    if (st == VINE_OK) {
+      cost_l.assign(ntask, 0.f);
+      delay_l.assign(ntask, 0.f);
+      feasible_l.assign(feas.begin(), feas.end());
+      path_l.assign(ntask, {});
+      size_of_result_l.assign(ntask, 0);
+      for (int i = 0; i < ntask; i++) {
+        cost_l[i] = label_l[4 * (size_t)i];
+        delay_l[i] = label_l[4 * (size_t)i + 1];
+        path_l[i].assign(path.begin() + off[i], path.begin() + off[i + 1]);
+        size_of_result_l[i] = (int)path_l[i].size();
+      }
+    } else {
+      last_error = vine_last_error();
+    }
+    vine_family_solver_destroy(fs);
     return st;
   }
   Graph<Edge_t>* g_;

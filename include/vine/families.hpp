@@ -17,12 +17,14 @@
 //
 // Not in the reference (it never extracts an answer for these families):
 // family_traits<Label_t> names the answer -- the best label of t's Pareto set
-// in a fixed lexicographic order -- the four output fields, and how a
-// constraint is built from the per-query limits (delay_L, hop_L, maxcost_L).
+// in a fixed lexicographic order -- the four output fields, the family id of
+// the C ABI, and how a constraint maps to and from the per-query limits
+// (delay_L, hop_L, maxcost_L).
 // Defaults hop_L = 100 and maxcost_L = 10000 are the reference's constructor
 // defaults (include/constraint.hpp:31, 61).
 #pragma once
 
+#include "vine.h"
 #include "vine/label.hpp"
 
 namespace vine {
@@ -206,6 +208,12 @@ __host__ __device__ inline bool lex_before(const float* a, const float* b, const
 template <>
 struct family_traits<vine::c_label> {  // CSP == 1 and 2-OSP: min (d, delay)
   using constraint = vine::c_constraint<vine::c_label>;
+  static constexpr int family = VINE_FAMILY_CSP;
+  static void limits(const constraint& C, float* lim) {
+    lim[0] = C.delay_L;
+    lim[1] = 0.f;
+    lim[2] = 0.f;
+  }
   __host__ __device__ static void fields(const vine::c_label& l, float* f) {
     f[0] = l.d;
     f[1] = l.delay;
@@ -221,6 +229,12 @@ struct family_traits<vine::c_label> {  // CSP == 1 and 2-OSP: min (d, delay)
 template <>
 struct family_traits<hop::c_label> {  // min (d, delay, hop)
   using constraint = hop::c_constraint<hop::c_label>;
+  static constexpr int family = VINE_FAMILY_HOP;
+  static void limits(const constraint& C, float* lim) {
+    lim[0] = C.delay_L;
+    lim[1] = (float)C.hop_L;
+    lim[2] = 0.f;
+  }
   __host__ __device__ static void fields(const hop::c_label& l, float* f) {
     f[0] = l.d;
     f[1] = l.delay;
@@ -238,6 +252,12 @@ struct family_traits<hop::c_label> {  // min (d, delay, hop)
 template <>
 struct family_traits<hop_maxcost::c_label> {  // min (d, delay, hop, maxcost)
   using constraint = hop_maxcost::c_constraint<hop_maxcost::c_label>;
+  static constexpr int family = VINE_FAMILY_HOP_MAXCOST;
+  static void limits(const constraint& C, float* lim) {
+    lim[0] = C.delay_L;
+    lim[1] = (float)C.hop_L;
+    lim[2] = C.maxcost_L;
+  }
   __host__ __device__ static void fields(const hop_maxcost::c_label& l, float* f) {
     f[0] = l.d;
     f[1] = l.delay;
@@ -257,6 +277,12 @@ struct family_traits<hop_maxcost::c_label> {  // min (d, delay, hop, maxcost)
 template <>
 struct family_traits<social::c_label> {  // max (d, delay)
   using constraint = social::c_constraint<social::c_label>;
+  static constexpr int family = VINE_FAMILY_SOCIAL;
+  static void limits(const constraint& C, float* lim) {
+    lim[0] = C.delay_L;
+    lim[1] = 0.f;
+    lim[2] = 0.f;
+  }
   __host__ __device__ static void fields(const social::c_label& l, float* f) {
     f[0] = l.d;
     f[1] = l.delay;
@@ -272,6 +298,12 @@ struct family_traits<social::c_label> {  // max (d, delay)
 template <>
 struct family_traits<qos::c_label> {  // min d, then max bandwidth
   using constraint = qos::c_constraint<qos::c_label>;
+  static constexpr int family = VINE_FAMILY_QOS;
+  static void limits(const constraint& C, float* lim) {
+    lim[0] = C.delay_L;
+    lim[1] = 0.f;
+    lim[2] = 0.f;
+  }
   __host__ __device__ static void fields(const qos::c_label& l, float* f) {
     f[0] = l.d;
     f[1] = l.delay;

@@ -128,3 +128,32 @@ def test_family_argument_errors(gpu):
             fs.solve(np.array([0], np.int32), np.array([64], np.int32), np.array([10.0], np.float32))
         with pytest.raises(S.VineError, match="capacity"):
             fs.solve(np.array([0], np.int32), np.array([63], np.int32), np.array([1e6], np.float32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,fam,limits", [
+    ("hop", S.FAMILY_HOP, [(100, 6, 0), (100, 5, 0), (4, 100, 0), (0, 100, 0)]),
+    ("qos", S.FAMILY_QOS, [(1, 0, 0), (2, 0, 0), (3, 0, 0), (1, 0, 0)]),
+])
+def test_cpp_facade_families(gpu, mode, fam, limits):
+    """cspp<hop::c_label> and cspp<qos::c_label> of the C++ facade (the reference's
+    solver API with another label family) on the grid of tests/cpp/facade_main.cpp."""
+    import os
+    import subprocess
+
+    from helpers import facade_grid
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    print("facade, family %s" % mode, flush=True)
+    r = subprocess.run([os.path.join(root, "vine_amd", "vine_facade_check"), mode], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = [ln.split() for ln in r.stdout.strip().splitlines()]
+    g = facade_grid()
+    tasks = [(0, 15), (0, 15), (5, 10), (3, 3)]
+    for (s, t), lim, out in zip(tasks, limits, rows):
+        ref = O.family(g, fam, s, t, lim)
+        assert bool(int(out[0])) == ref.feasible, (mode, s, t, out)
+        if ref.feasible:
+            assert [float(x) for x in out[1:4]] == [float(x) for x in ref.best[:3]], (mode, s, t, out, ref.best)
+            label = np.array([float(x) for x in out[1:4]] + [0.0], F32)
+            _check_path(g, fam, np.array([int(x) for x in out[4:]]), s, t, label, np.array(lim, F32))

@@ -155,30 +155,8 @@ def test_cpp_facade_matches_oracle(gpu):
                        timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     rows = [list(map(float, ln.split())) for ln in r.stdout.strip().splitlines()]
-    # the same grid as facade_main.cpp
-    import numpy as np
-    arcs = []
-    for rr in range(4):
-        for cc in range(4):
-            i = rr * 4 + cc
-            if cc + 1 < 4:
-                w = (1.0 + (i * 7) % 5, 1.0 + (i * 3) % 4)
-                arcs += [(i, i + 1, w), (i + 1, i, w)]
-            if rr + 1 < 4:
-                w = (2.0 + (i * 5) % 3, 1.0 + (i * 11) % 3)
-                arcs += [(i, i + 4, w), (i + 4, i, w)]
-    adj = [[] for _ in range(16)]
-    for u, v, w in arcs:
-        adj[u].append((v, w))
-    row = np.zeros(17, np.int32)
-    dst, ln, dl = [], [], []
-    for u in range(16):
-        for v, w in adj[u]:
-            dst.append(v)
-            ln.append(w[0])
-            dl.append(w[1])
-        row[u + 1] = len(dst)
-    g = W.Graph(row, np.array(dst, np.int32), np.array(ln, np.float32), np.array(dl, np.float32))
+    from helpers import facade_grid
+    g = facade_grid()
     tasks = [(0, 15, 100.0), (0, 15, 7.0), (5, 10, 2.0), (3, 3, 0.0)]
     for (s, t, b), out in zip(tasks, rows):
         ref = O.cspp(g, s, t, b)
