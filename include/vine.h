@@ -260,9 +260,13 @@ typedef enum {
 typedef struct vine_family_solver vine_family_solver;
 
 /* Device memory for max_batch queries per batch and label_capacity labels
- * (0 = 2^25); about 37 B per label plus 8 B per (query slot, vertex). */
+ * (0 = 2^25); about 37 B per label plus 8 B per (query slot, vertex).
+ * prune = 0: the reference's one-to-all search (every vertex's Pareto set is
+ * complete, vine_family_front returns it); prune = 1: labels that cannot beat
+ * the best answer found at t so far are dropped (the answer is the same,
+ * Pareto sets other than the answer's may be partial). */
 vine_status vine_family_solver_create(const vine_graph* g, int32_t family, int32_t max_batch,
-                                      int64_t label_capacity, vine_family_solver** out);
+                                      int64_t label_capacity, int32_t prune, vine_family_solver** out);
 vine_status vine_family_solver_destroy(vine_family_solver* s);
 
 /* Host buffers; queries run in batches of max_batch.  Paths as in vine_solve

@@ -60,13 +60,14 @@ def _check_path(g, fam, path, s, t, label, lim):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("prune", [0, 1])
 @pytest.mark.parametrize("fam", [S.FAMILY_CSP, S.FAMILY_HOP, S.FAMILY_HOP_MAXCOST, S.FAMILY_2OSP, S.FAMILY_SOCIAL,
                                  S.FAMILY_QOS])
-def test_family_parity(gpu, fam):
+def test_family_parity(gpu, fam, prune):
     g = _social_graph(3) if fam == S.FAMILY_SOCIAL else W.grid(24, 24, seed=fam)
     src, dst, lim = _queries(g, fam, 48, seed=fam)
-    print("family %d: n=%d, %d queries in batches of 16" % (fam, g.n, len(src)), flush=True)
-    with S.FamilySolver(g, fam, max_batch=16) as fs:
+    print("family %d, prune %d: n=%d, %d queries in batches of 16" % (fam, prune, g.n, len(src)), flush=True)
+    with S.FamilySolver(g, fam, max_batch=16, prune=prune) as fs:
         a = fs.solve(src, dst, lim[:, 0], lim[:, 1], lim[:, 2])
         refs = [O.family(g, fam, src[i], dst[i], lim[i]) for i in range(len(src))]
         for i, r in enumerate(refs):
@@ -76,8 +77,9 @@ def test_family_parity(gpu, fam):
                 _check_path(g, fam, a.path_of(i), src[i], dst[i], a.label[i], lim[i])
             else:
                 assert np.all(np.isinf(a.label[i]))
-        # whole Pareto sets at t for the last batch (queries 32..47)
-        for k in range(16):
+        # whole Pareto sets at t for the last batch (queries 32..47); target
+        # pruning keeps only the answer exact
+        for k in range(16 if prune == 0 else 0):
             i = 32 + k
             got = fs.front(k, dst[i])
             order = np.lexsort(tuple(got[:, j] for j in range(3, -1, -1)))

@@ -68,6 +68,11 @@ struct FamCtx {
   long long* touched; // [cap] (slot * n + vertex)
   const float* lim;   // [B*3] per-slot limits
   FamCounters* ct;
+  // target pruning (prune = 1): the best label at t so far, per slot
+  int32_t prune;
+  const int32_t* qdst;  // [B] target (internal id)
+  void* best;           // Label_t[B]
+  int32_t* has_best;    // [B]
 };
 
 // Sums a and b over the workgroup and adds them to *da / *db with one atomic
@@ -119,6 +124,9 @@ __global__ void k_fam_init(FamCtx c, int nb, const int32_t* src) {
   c.dead[b] = 0;
   c.head[(long long)b * c.n + src[b]] = b;
   c.front[0][b] = b;
+  // s == t: the source is the answer already
+  c.has_best[b] = src[b] == c.qdst[b];
+  if (src[b] == c.qdst[b]) ((Label_t*)c.best)[b] = lab[b];
 }
 
 template <typename Label_t>
@@ -140,6 +148,10 @@ __global__ __launch_bounds__(kBlock) void k_fam_expand(FamCtx c, int cur) {
       Label_t son;
       L.expand(e.x, vine::c_edge(__int_as_float(e.y), __int_as_float(e.z)), &son);
       if (!C.pass_constraints_check(son)) continue;  // the reference breaks here (algorithm/bucket.cuh:76-78)
+      // target pruning: extending never makes a label better in the answer
+      // order (every family's expand is monotone), so a son that is not
+      // better than t's best so far cannot lead to the answer
+      if (c.prune && c.has_best[q] && !Traits::better(son, ((const Label_t*)c.best)[q])) continue;
       const long long key = (long long)q * c.n + son.vid;
       bool dominated = false;
       for (int x = c.head[key]; x >= 0; x = c.nxt[x])
@@ -168,6 +180,7 @@ __global__ __launch_bounds__(kBlock) void k_fam_expand(FamCtx c, int cur) {
 
 template <typename Label_t>
 __global__ __launch_bounds__(kBlock) void k_fam_merge(FamCtx c, int cur) {
+  using Traits = vine::family_traits<Label_t>;
   Label_t* lab = (Label_t*)c.lab;
   const int T = c.ct->touched_n;
   const int nx = cur ^ 1;
@@ -211,6 +224,18 @@ __global__ __launch_bounds__(kBlock) void k_fam_merge(FamCtx c, int cur) {
         n_ins++;
       }
       cand = next_cand;
+    }
+    if (c.prune) {
+      const int q = (int)(key / c.n);
+      if (key - (long long)q * c.n == c.qdst[q]) {
+        // only this thread writes slot q's best in this kernel
+        Label_t* best = (Label_t*)c.best;
+        for (int x = c.head[key]; x >= 0; x = c.nxt[x])
+          if (!c.has_best[q] || Traits::better(lab[x], best[q])) {
+            best[q] = lab[x];
+            c.has_best[q] = 1;
+          }
+      }
     }
   }
   block_add2(n_ins, 0, &c.ct->inserted, nullptr);
@@ -298,7 +323,8 @@ struct vine_family_solver {
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   FamCtx c{};
-  int32_t *d_src = nullptr, *d_dst = nullptr, *d_isrc = nullptr, *d_idst = nullptr;
+  int32_t *d_src = nullptr, *d_dst = nullptr, *d_isrc = nullptr, *d_idst = nullptr, *d_has_best = nullptr;
+  void* d_bestlab = nullptr;
   int32_t *d_feas = nullptr, *d_best = nullptr, *d_hops = nullptr, *d_count = nullptr;
   float *d_out = nullptr, *d_lim = nullptr, *d_front = nullptr;
   long long* d_off = nullptr;
@@ -505,8 +531,9 @@ int64_t front_family(const vine_family_solver* s, int32_t slot, int32_t v, float
 extern "C" {
 
 vine_status vine_family_solver_create(const vine_graph* g, int32_t family, int32_t max_batch, int64_t label_capacity,
-                                      vine_family_solver** out) {
-  if (!g || !out || family < VINE_FAMILY_CSP || family > VINE_FAMILY_QOS || max_batch < 1) {
+                                      int32_t prune, vine_family_solver** out) {
+  if (!g || !out || family < VINE_FAMILY_CSP || family > VINE_FAMILY_QOS || max_batch < 1 || prune < 0 ||
+      prune > 1) {
     set_error("vine_family_solver_create: invalid arguments");
     return VINE_E_ARG;
   }
@@ -541,12 +568,18 @@ vine_status vine_family_solver_create(const vine_graph* g, int32_t family, int32
                     (void**)&c.head, (void**)&c.chead, (void**)&c.front[0], (void**)&c.front[1], (void**)&c.touched,
                     (void**)&s->d_lim, (void**)&c.ct, (void**)&s->d_src, (void**)&s->d_dst, (void**)&s->d_isrc,
                     (void**)&s->d_idst, (void**)&s->d_feas, (void**)&s->d_best, (void**)&s->d_hops,
-                    (void**)&s->d_count, (void**)&s->d_out, (void**)&s->d_front, (void**)&s->d_off};
+                    (void**)&s->d_count, (void**)&s->d_out, (void**)&s->d_front, (void**)&s->d_off,
+                    (void**)&s->d_bestlab, (void**)&s->d_has_best};
   const size_t bytes[] = {cap * label_size(family), cap * 4, cap * 4, cap * 4, cap * 4, cap, cells * 4, cells * 4,
                           cap * 4, cap * 4, cap * 8, B * 12, sizeof(FamCounters), B * 4, B * 4, B * 4,
-                          B * 4, B * 4, B * 4, B * 4, 4, B * 16, 4096 * 16, (B + 1) * 8};
+                          B * 4, B * 4, B * 4, B * 4, 4, B * 16, 4096 * 16, (B + 1) * 8, B * label_size(family),
+                          B * 4};
   for (size_t k = 0; k < sizeof(bytes) / sizeof(bytes[0]) && r == VINE_OK; k++) r = fam_alloc(s, slots[k], bytes[k]);
   c.lim = s->d_lim;
+  c.prune = prune;
+  c.qdst = s->d_idst;
+  c.best = s->d_bestlab;
+  c.has_best = s->d_has_best;
   if (r != VINE_OK) {
     vine_family_solver_destroy(s);
     return r;
