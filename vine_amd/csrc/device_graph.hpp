@@ -15,6 +15,7 @@ struct vine_graph {
   int32_t max_deg = 0;
   float max_len = 0.f;
   float max_delay = 0.f;
+  bool integer_weights = true;  // every len and delay an integer below 2^24 (path sums exact)
   int device = 0;
   // vertices are renumbered for locality (layout.cpp): perm[caller id] =
   // internal id, d_inv the inverse; every array below uses internal ids

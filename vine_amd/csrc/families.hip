@@ -38,6 +38,45 @@
 
 using vine::set_error;
 
+namespace vine {
+namespace bounds {
+// Distance to t for the bound searches of the pruned search: the generic
+// search itself, run from t on the transposed graph with a one-criterion
+// label (W = 0 length, 1 delay, 2 arcs), leaves at every vertex the single
+// label of its shortest distance.
+template <int W>
+struct dist_label {
+  int vid;
+  float d;
+  __host__ __device__ dist_label() : vid(-1), d(0.f) {}
+  __host__ __device__ explicit dist_label(int v) : vid(v), d(0.f) {}
+  template <typename Edge_t>
+  __host__ __device__ void expand(int next_v, const Edge_t& e, dist_label* son) const {
+    son->vid = next_v;
+    son->d = d + (W == 0 ? e.len : (W == 1 ? e.delay : 1.f));
+  }
+  __host__ __device__ static bool dominate(const dist_label& a, const dist_label& b) { return a.d <= b.d; }
+};
+struct no_constraint {
+  template <typename L>
+  __host__ __device__ bool pass_constraints_check(const L&) const { return true; }
+};
+}  // namespace bounds
+
+template <int W>
+struct family_traits<bounds::dist_label<W>> {
+  using constraint = bounds::no_constraint;
+  __host__ __device__ static void fields(const bounds::dist_label<W>& l, float* f) {
+    f[0] = l.d;
+    f[1] = f[2] = f[3] = 0.f;
+  }
+  __host__ __device__ static bool better(const bounds::dist_label<W>& a, const bounds::dist_label<W>& b) {
+    return a.d < b.d;
+  }
+  __host__ __device__ static constraint make(const float*) { return constraint(); }
+};
+}  // namespace vine
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -73,6 +112,11 @@ struct FamCtx {
   const int32_t* qdst;  // [B] target (internal id)
   void* best;           // Label_t[B]
   int32_t* has_best;    // [B]
+  // distance bounds to t per (slot, vertex), prune = 1 only (null: unused):
+  // length (answer orders led by min d), delay (additive delay), arcs (hop limit)
+  const float* blen;
+  const float* bdelay;
+  const float* bhop;
 };
 
 // Sums a and b over the workgroup and adds them to *da / *db with one atomic
@@ -153,6 +197,20 @@ __global__ __launch_bounds__(kBlock) void k_fam_expand(FamCtx c, int cur) {
       // better than t's best so far cannot lead to the answer
       if (c.prune && c.has_best[q] && !Traits::better(son, ((const Label_t*)c.best)[q])) continue;
       const long long key = (long long)q * c.n + son.vid;
+      if (c.prune && (c.blen || c.bdelay || c.bhop)) {
+        // no completion through son.vid can be feasible (delay, arcs) or beat
+        // t's best length; integer weights keep these sums exact
+        float f[4];
+        Traits::fields(son, f);
+        const float* lv = c.lim + 3 * q;
+        if (c.bdelay && f[1] + c.bdelay[key] > lv[0]) continue;
+        if (c.bhop && f[2] + c.bhop[key] > (float)(int)lv[1]) continue;
+        if (c.blen && c.has_best[q]) {
+          float bf[4];
+          Traits::fields(((const Label_t*)c.best)[q], bf);
+          if (f[0] + c.blen[key] > bf[0]) continue;
+        }
+      }
       bool dominated = false;
       for (int x = c.head[key]; x >= 0; x = c.nxt[x])
         if (Label_t::dominate(lab[x], son)) {
@@ -298,6 +356,16 @@ __global__ void k_fam_front(FamCtx c, long long key, float* out, int cap, int32_
   *count = k;
 }
 
+// bound[slot * n + v] = the distance label's value at v (+inf where none)
+template <typename Label_t>
+__global__ void k_fam_dist(FamCtx c, long long cells, float* bound) {
+  const Label_t* lab = (const Label_t*)c.lab;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += (long long)gridDim.x * blockDim.x) {
+    const int x = c.head[i];
+    bound[i] = x >= 0 ? lab[x].d : INFINITY;
+  }
+}
+
 __global__ void k_map(int nb, const int32_t* perm, const int32_t* in, int32_t* out) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nb) out[b] = perm[in[b]];
@@ -325,6 +393,7 @@ struct vine_family_solver {
   FamCtx c{};
   int32_t *d_src = nullptr, *d_dst = nullptr, *d_isrc = nullptr, *d_idst = nullptr, *d_has_best = nullptr;
   void* d_bestlab = nullptr;
+  float *d_blen = nullptr, *d_bdelay = nullptr, *d_bhop = nullptr;
   int32_t *d_feas = nullptr, *d_best = nullptr, *d_hops = nullptr, *d_count = nullptr;
   float *d_out = nullptr, *d_lim = nullptr, *d_front = nullptr;
   long long* d_off = nullptr;
@@ -359,17 +428,16 @@ vine_status fam_alloc(vine_family_solver* s, void** p, size_t bytes) {
 int grid_for(long long work) { return (int)std::max<long long>(1, std::min<long long>((work + kBlock - 1) / kBlock, 4096)); }
 
 // One batch: nb queries already in d_src/d_dst (caller ids) and d_lim.
+// Rounds of the generic search for nb slots from the sources `from`
+// (internal ids), on the graph c.row / c.edges, until the frontier is empty.
 template <typename Label_t>
-vine_status run_batch(vine_family_solver* s, int nb, int64_t* counters) {
-  FamCtx& c = s->c;
+vine_status run_search(vine_family_solver* s, FamCtx& c, int nb, const int32_t* from) {
   const int n = s->g->n;
   hipStream_t st = s->stream;
-  hipLaunchKernelGGL(k_map, dim3(grid_for(nb)), dim3(kBlock), 0, st, nb, s->g->d_perm, s->d_src, s->d_isrc);
-  hipLaunchKernelGGL(k_map, dim3(grid_for(nb)), dim3(kBlock), 0, st, nb, s->g->d_perm, s->d_dst, s->d_idst);
   const long long cells = (long long)nb * n;
   hipLaunchKernelGGL(k_fill, dim3(grid_for(cells)), dim3(kBlock), 0, st, c.head, cells, -1);
   hipLaunchKernelGGL(k_fill, dim3(grid_for(cells)), dim3(kBlock), 0, st, c.chead, cells, -1);
-  hipLaunchKernelGGL(k_fam_init<Label_t>, dim3(grid_for(nb)), dim3(kBlock), 0, st, c, nb, s->d_isrc);
+  hipLaunchKernelGGL(k_fam_init<Label_t>, dim3(grid_for(nb)), dim3(kBlock), 0, st, c, nb, from);
   FAM_TRY(hipGetLastError());
   // rounds until the frontier is empty; the host looks every `sync` rounds
   int cur = 0;
@@ -399,6 +467,48 @@ vine_status run_batch(vine_family_solver* s, int nb, int64_t* counters) {
       if (h.fn[cur] == 0) break;
     }
   }
+  return VINE_OK;
+}
+
+// Distances to t for the pruned search: the generic search from each slot's
+// target on the transposed graph, then one value per (slot, vertex).
+template <int W>
+vine_status run_bound(vine_family_solver* s, int nb, float* out) {
+  using L = vine::bounds::dist_label<W>;
+  FamCtx c = s->c;
+  c.row = s->g->d_rrow;
+  c.edges = s->g->d_redges;
+  c.prune = 0;
+  c.blen = c.bdelay = c.bhop = nullptr;
+  vine_status r = run_search<L>(s, c, nb, s->d_idst);
+  if (r != VINE_OK) return r;
+  const long long cells = (long long)nb * s->g->n;
+  hipLaunchKernelGGL(k_fam_dist<L>, dim3(grid_for(cells)), dim3(kBlock), 0, s->stream, c, cells, out);
+  FAM_TRY(hipGetLastError());
+  return VINE_OK;
+}
+
+template <typename Label_t>
+vine_status run_batch(vine_family_solver* s, int nb, int64_t* counters) {
+  FamCtx& c = s->c;
+  hipStream_t st = s->stream;
+  hipLaunchKernelGGL(k_map, dim3(grid_for(nb)), dim3(kBlock), 0, st, nb, s->g->d_perm, s->d_src, s->d_isrc);
+  hipLaunchKernelGGL(k_map, dim3(grid_for(nb)), dim3(kBlock), 0, st, nb, s->g->d_perm, s->d_dst, s->d_idst);
+  FAM_TRY(hipGetLastError());
+  if (c.blen) {
+    vine_status r = run_bound<0>(s, nb, s->d_blen);
+    if (r != VINE_OK) return r;
+  }
+  if (c.bdelay) {
+    vine_status r = run_bound<1>(s, nb, s->d_bdelay);
+    if (r != VINE_OK) return r;
+  }
+  if (c.bhop) {
+    vine_status r = run_bound<2>(s, nb, s->d_bhop);
+    if (r != VINE_OK) return r;
+  }
+  vine_status r = run_search<Label_t>(s, c, nb, s->d_isrc);
+  if (r != VINE_OK) return r;
   hipLaunchKernelGGL(k_fam_answer<Label_t>, dim3(grid_for(nb)), dim3(kBlock), 0, st, c, nb, s->d_idst, s->d_out,
                      s->d_feas, s->d_best, s->d_hops);
   FAM_TRY(hipGetLastError());
@@ -580,6 +690,18 @@ vine_status vine_family_solver_create(const vine_graph* g, int32_t family, int32
   c.qdst = s->d_idst;
   c.best = s->d_bestlab;
   c.has_best = s->d_has_best;
+  // distance bounds for the pruned search, where the family's order and
+  // constraints are additive in them and sums of integer weights are exact
+  if (r == VINE_OK && prune && g->integer_weights && family != VINE_FAMILY_SOCIAL) {
+    const bool additive_delay = family != VINE_FAMILY_QOS;
+    const bool hops = family == VINE_FAMILY_HOP || family == VINE_FAMILY_HOP_MAXCOST;
+    r = fam_alloc(s, (void**)&s->d_blen, cells * 4);
+    if (r == VINE_OK && additive_delay) r = fam_alloc(s, (void**)&s->d_bdelay, cells * 4);
+    if (r == VINE_OK && hops) r = fam_alloc(s, (void**)&s->d_bhop, cells * 4);
+    c.blen = s->d_blen;
+    c.bdelay = s->d_bdelay;
+    c.bhop = s->d_bhop;
+  }
   if (r != VINE_OK) {
     vine_family_solver_destroy(s);
     return r;

@@ -285,6 +285,9 @@ vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, cons
   for (int64_t p = 0; p < m; p++) {
     g->max_len = std::max(g->max_len, len[p]);
     g->max_delay = std::max(g->max_delay, delay[p]);
+    if (len[p] != std::floor(len[p]) || delay[p] != std::floor(delay[p]) || len[p] >= 16777216.f ||
+        delay[p] >= 16777216.f)
+      g->integer_weights = false;
   }
   hipError_t err = hipGetDevice(&g->device);
   size_t eb = sizeof(int4) * std::max<int64_t>(m, 1);
