@@ -263,10 +263,16 @@ typedef struct vine_family_solver vine_family_solver;
  * (0 = 2^25); about 37 B per label plus 8 B per (query slot, vertex).
  * prune = 0: the reference's one-to-all search (every vertex's Pareto set is
  * complete, vine_family_front returns it); prune = 1: labels that cannot beat
- * the best answer found at t so far are dropped (the answer is the same,
- * Pareto sets other than the answer's may be partial). */
+ * the best answer found at t so far, or (integer weights) cannot complete
+ * within the limits by their shortest delay / arc count to t, are dropped
+ * (the answer is the same, Pareto sets other than the answer's may be
+ * partial).  window > 0 (families other than social): each round expands only
+ * the labels whose length plus shortest length to t is within `window` of
+ * the query's smallest, as the batched solver's fixed window does; 0 expands
+ * every frontier label each round. */
 vine_status vine_family_solver_create(const vine_graph* g, int32_t family, int32_t max_batch,
-                                      int64_t label_capacity, int32_t prune, vine_family_solver** out);
+                                      int64_t label_capacity, int32_t prune, float window,
+                                      vine_family_solver** out);
 vine_status vine_family_solver_destroy(vine_family_solver* s);
 
 /* Host buffers; queries run in batches of max_batch.  Paths as in vine_solve

@@ -169,8 +169,10 @@ class cspp {
     const int batch = options.max_batch > 1 ? options.max_batch : (ntask < 64 ? (ntask > 0 ? ntask : 1) : 64);
     // only the answers are returned: target pruning unless the options ask
     // for the reference's one-to-all search (VINE_PRUNE_NONE)
+    // (a fixed window, VINE_DELTA_FIXED with delta_fixed > 0, orders the rounds)
+    const float window = options.delta_policy == VINE_DELTA_FIXED ? options.delta_fixed : 0.f;
     vine_status st = vine_family_solver_create(dg, Traits::family, batch, 0, options.prune != VINE_PRUNE_NONE ? 1 : 0,
-                                               &fs);
+                                               window, &fs);
     if (st != VINE_OK) return fail(st);
     std::vector<int32_t> src(ntask), snk(sink_l.begin(), sink_l.end());
     std::vector<float> lim(3 * (size_t)ntask);

@@ -60,14 +60,15 @@ def _check_path(g, fam, path, s, t, label, lim):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prune", [0, 1])
+@pytest.mark.parametrize("prune,window", [(0, 0.0), (1, 0.0), (1, 400.0)])
 @pytest.mark.parametrize("fam", [S.FAMILY_CSP, S.FAMILY_HOP, S.FAMILY_HOP_MAXCOST, S.FAMILY_2OSP, S.FAMILY_SOCIAL,
                                  S.FAMILY_QOS])
-def test_family_parity(gpu, fam, prune):
+def test_family_parity(gpu, fam, prune, window):
     g = _social_graph(3) if fam == S.FAMILY_SOCIAL else W.grid(24, 24, seed=fam)
     src, dst, lim = _queries(g, fam, 48, seed=fam)
-    print("family %d, prune %d: n=%d, %d queries in batches of 16" % (fam, prune, g.n, len(src)), flush=True)
-    with S.FamilySolver(g, fam, max_batch=16, prune=prune) as fs:
+    print("family %d, prune %d, window %g: n=%d, %d queries in batches of 16" % (fam, prune, window, g.n, len(src)),
+          flush=True)
+    with S.FamilySolver(g, fam, max_batch=16, prune=prune, window=window) as fs:
         a = fs.solve(src, dst, lim[:, 0], lim[:, 1], lim[:, 2])
         refs = [O.family(g, fam, src[i], dst[i], lim[i]) for i in range(len(src))]
         for i, r in enumerate(refs):

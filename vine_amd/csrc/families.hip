@@ -92,6 +92,7 @@ struct FamCounters {
 
 struct FamCtx {
   int32_t n;
+  int32_t B;    // query slots
   int32_t cap;  // label capacity
   const int32_t* row;
   const int4* edges;  // {dst, len bits, delay bits, 0}
@@ -117,7 +118,20 @@ struct FamCtx {
   const float* blen;
   const float* bdelay;
   const float* bhop;
+  // key-ordered windows (window > 0, families ordered by min d first): a
+  // round expands the labels whose key d + blen is within `window` of the
+  // query's smallest key and carries the rest to the next round
+  float window;
+  uint32_t* qkey;  // [2][B] smallest key of the query's frontier (float bits), per round parity
 };
+
+// key of label L of slot q: length plus the length bound to t (if computed)
+template <typename Label_t>
+__device__ __forceinline__ float fam_key(const FamCtx& c, const Label_t& L, int q) {
+  float f[4];
+  vine::family_traits<Label_t>::fields(L, f);
+  return f[0] + (c.blen ? c.blen[(long long)q * c.n + L.vid] : 0.f);
+}
 
 // Sums a and b over the workgroup and adds them to *da / *db with one atomic
 // each; every thread of the workgroup must call it.
@@ -171,6 +185,10 @@ __global__ void k_fam_init(FamCtx c, int nb, const int32_t* src) {
   // s == t: the source is the answer already
   c.has_best[b] = src[b] == c.qdst[b];
   if (src[b] == c.qdst[b]) ((Label_t*)c.best)[b] = lab[b];
+  if (c.window > 0.f) {
+    c.qkey[b] = __float_as_uint(fam_key(c, lab[b], b));
+    c.qkey[c.B + b] = __float_as_uint(INFINITY);
+  }
 }
 
 template <typename Label_t>
@@ -184,6 +202,21 @@ __global__ __launch_bounds__(kBlock) void k_fam_expand(FamCtx c, int cur) {
     if (c.dead[i]) continue;
     const Label_t L = lab[i];
     const int q = c.lq[i];
+    if (c.window > 0.f) {
+      const float k = fam_key(c, L, q);
+      if (!(k < INFINITY)) continue;  // t is unreachable from here (length bound +inf)
+      if (c.prune && c.has_best[q]) {
+        float bf[4];
+        Traits::fields(((const Label_t*)c.best)[q], bf);
+        if (k > bf[0]) continue;  // no completion can beat t's best length
+      }
+      if (k > __uint_as_float(c.qkey[cur * c.B + q]) + c.window) {
+        // outside this round's window: carried to the next frontier
+        c.front[cur ^ 1][atomicAdd(&c.ct->fn[cur ^ 1], 1)] = i;
+        atomicMin(&c.qkey[(cur ^ 1) * c.B + q], __float_as_uint(k));
+        continue;
+      }
+    }
     const typename Traits::constraint C = Traits::make(c.lim + 3 * q);
     n_exp++;
     for (int p = c.row[L.vid]; p < c.row[L.vid + 1]; p++) {
@@ -203,6 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_fam_expand(FamCtx c, int cur) {
         float f[4];
         Traits::fields(son, f);
         const float* lv = c.lim + 3 * q;
+        if (c.blen && !(c.blen[key] < INFINITY)) continue;  // t is unreachable from son.vid
         if (c.bdelay && f[1] + c.bdelay[key] > lv[0]) continue;
         if (c.bhop && f[2] + c.bhop[key] > (float)(int)lv[1]) continue;
         if (c.blen && c.has_best[q]) {
@@ -243,6 +277,9 @@ __global__ __launch_bounds__(kBlock) void k_fam_merge(FamCtx c, int cur) {
   const int T = c.ct->touched_n;
   const int nx = cur ^ 1;
   long long n_ins = 0;
+  if (c.window > 0.f)  // this round's keys are consumed: start the round after next from +inf
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < c.B; b += gridDim.x * blockDim.x)
+      c.qkey[cur * c.B + b] = __float_as_uint(INFINITY);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
     const long long key = c.touched[t];
     int cand = c.chead[key];
@@ -279,6 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_fam_merge(FamCtx c, int cur) {
         c.nxt[cand] = c.head[key];
         c.head[key] = cand;
         c.front[nx][atomicAdd(&c.ct->fn[nx], 1)] = cand;
+        if (c.window > 0.f) atomicMin(&c.qkey[nx * c.B + c.lq[cand]], __float_as_uint(fam_key(c, L, c.lq[cand])));
         n_ins++;
       }
       cand = next_cand;
@@ -394,6 +432,7 @@ struct vine_family_solver {
   int32_t *d_src = nullptr, *d_dst = nullptr, *d_isrc = nullptr, *d_idst = nullptr, *d_has_best = nullptr;
   void* d_bestlab = nullptr;
   float *d_blen = nullptr, *d_bdelay = nullptr, *d_bhop = nullptr;
+  uint32_t* d_qkey = nullptr;
   int32_t *d_feas = nullptr, *d_best = nullptr, *d_hops = nullptr, *d_count = nullptr;
   float *d_out = nullptr, *d_lim = nullptr, *d_front = nullptr;
   long long* d_off = nullptr;
@@ -479,6 +518,7 @@ vine_status run_bound(vine_family_solver* s, int nb, float* out) {
   c.row = s->g->d_rrow;
   c.edges = s->g->d_redges;
   c.prune = 0;
+  c.window = 0.f;
   c.blen = c.bdelay = c.bhop = nullptr;
   vine_status r = run_search<L>(s, c, nb, s->d_idst);
   if (r != VINE_OK) return r;
@@ -641,9 +681,9 @@ int64_t front_family(const vine_family_solver* s, int32_t slot, int32_t v, float
 extern "C" {
 
 vine_status vine_family_solver_create(const vine_graph* g, int32_t family, int32_t max_batch, int64_t label_capacity,
-                                      int32_t prune, vine_family_solver** out) {
+                                      int32_t prune, float window, vine_family_solver** out) {
   if (!g || !out || family < VINE_FAMILY_CSP || family > VINE_FAMILY_QOS || max_batch < 1 || prune < 0 ||
-      prune > 1) {
+      prune > 1 || !(window >= 0.f)) {
     set_error("vine_family_solver_create: invalid arguments");
     return VINE_E_ARG;
   }
@@ -670,6 +710,7 @@ vine_status vine_family_solver_create(const vine_graph* g, int32_t family, int32
   }
   FamCtx& c = s->c;
   c.n = g->n;
+  c.B = s->B;
   c.cap = s->cap;
   c.row = g->d_row;
   c.edges = g->d_edges;
@@ -679,17 +720,20 @@ vine_status vine_family_solver_create(const vine_graph* g, int32_t family, int32
                     (void**)&s->d_lim, (void**)&c.ct, (void**)&s->d_src, (void**)&s->d_dst, (void**)&s->d_isrc,
                     (void**)&s->d_idst, (void**)&s->d_feas, (void**)&s->d_best, (void**)&s->d_hops,
                     (void**)&s->d_count, (void**)&s->d_out, (void**)&s->d_front, (void**)&s->d_off,
-                    (void**)&s->d_bestlab, (void**)&s->d_has_best};
+                    (void**)&s->d_bestlab, (void**)&s->d_has_best, (void**)&s->d_qkey};
   const size_t bytes[] = {cap * label_size(family), cap * 4, cap * 4, cap * 4, cap * 4, cap, cells * 4, cells * 4,
                           cap * 4, cap * 4, cap * 8, B * 12, sizeof(FamCounters), B * 4, B * 4, B * 4,
                           B * 4, B * 4, B * 4, B * 4, 4, B * 16, 4096 * 16, (B + 1) * 8, B * label_size(family),
-                          B * 4};
+                          B * 4, B * 8};
   for (size_t k = 0; k < sizeof(bytes) / sizeof(bytes[0]) && r == VINE_OK; k++) r = fam_alloc(s, slots[k], bytes[k]);
   c.lim = s->d_lim;
   c.prune = prune;
   c.qdst = s->d_idst;
   c.best = s->d_bestlab;
   c.has_best = s->d_has_best;
+  c.qkey = s->d_qkey;
+  // windows need an order led by min d (all families but social)
+  c.window = family == VINE_FAMILY_SOCIAL ? 0.f : window;
   // distance bounds for the pruned search, where the family's order and
   // constraints are additive in them and sums of integer weights are exact
   if (r == VINE_OK && prune && g->integer_weights && family != VINE_FAMILY_SOCIAL) {

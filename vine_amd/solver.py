@@ -122,7 +122,7 @@ def _L():
             "vine_solver_query_stats": (i32, [vp, i32, vp, vp]),
             "vine_solver_set_mlp": (i32, [vp, vp, i64]),
             "vine_solver_pareto": (i64, [vp, i32, i32, vp, i64]),
-            "vine_family_solver_create": (i32, [vp, i32, i32, i64, i32, P(vp)]),
+            "vine_family_solver_create": (i32, [vp, i32, i32, i64, i32, ctypes.c_float, P(vp)]),
             "vine_family_solver_destroy": (i32, [vp]),
             "vine_family_solve": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, i64, vp]),
             "vine_family_front": (i64, [vp, i32, i32, vp, i64]),
@@ -306,13 +306,15 @@ class FamilyAnswer:
 class FamilySolver:
     """Generic label-correcting search for one label family (vine_family_solve)."""
 
-    def __init__(self, graph, family, max_batch=64, label_capacity=0, prune=0):
+    def __init__(self, graph, family, max_batch=64, label_capacity=0, prune=0, window=0.0):
         """prune = 0: the reference's one-to-all search (complete Pareto sets);
-        1: drop labels that cannot beat t's best answer (same answers)."""
+        1: drop labels that cannot beat t's best answer (same answers).
+        window > 0: key-ordered rounds (length + bound to t within window of the
+        query's smallest); 0: every frontier label every round."""
         self.graph = graph if isinstance(graph, DeviceGraph) else DeviceGraph(graph)
         h = ctypes.c_void_p()
         _check(_L().vine_family_solver_create(self.graph.h, int(family), int(max_batch), int(label_capacity),
-                                              int(prune), ctypes.byref(h)))
+                                              int(prune), float(window), ctypes.byref(h)))
         self.h = h
         self.family = family
 
