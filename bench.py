@@ -258,7 +258,8 @@ def main():
                    "parallelism": "single GPU"},
         "latency_us": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "work_per_step": {k: agg[k] / args.steps for k in ("rounds", "sssp_rounds", "scanned", "expanded", "relaxed",
-                                                           "candidates", "inserted", "sssp_relaxed", "far_pulled")},
+                                                           "candidates", "cand_chunks", "inserted", "sssp_relaxed",
+                                                           "far_pulled")},
         "roofline": roofline,
         "plain": plain,
         "vs_plain": (qps / plain["1.1x"]["qps"]) if plain else None,

@@ -138,6 +138,7 @@ typedef struct {
   int64_t scanned;          /* frontier entries read by the expand kernels */
   int64_t timed_rounds;     /* rounds whose kernels the ms_* figures cover (time_kernels) */
   int64_t far_units;        /* far-tier units (16 entries each) allocated, summed over batches */
+  int64_t cand_chunks;      /* candidate-list chunks folded by the merge kernel (records / chunks = fill) */
 } vine_stats;
 
 void vine_options_default(vine_options* opt);

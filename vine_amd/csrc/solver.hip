@@ -932,6 +932,7 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
   st->near_carried += (int64_t)s->h_ct->near_carried;
   st->far_appended += (int64_t)s->h_ct->far_appended;
   st->far_units += (int64_t)s->h_ct->far_chunks;
+  st->cand_chunks += (int64_t)s->h_ct->cand_chunks;
   st->scanned += (int64_t)s->h_ct->scanned;
   st->batches++;
   return VINE_OK;
@@ -1111,6 +1112,7 @@ void merge_stats(vine_stats& a, const vine_stats& b) {
   a.scanned += b.scanned;
   a.timed_rounds += b.timed_rounds;
   a.far_units += b.far_units;
+  a.cand_chunks += b.cand_chunks;
 }
 
 // Per-query records of the parts, in the caller's query order.

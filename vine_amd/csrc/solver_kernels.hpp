@@ -66,6 +66,7 @@ struct Counters {
   unsigned long long expanded, relaxed, cands, inserted, sssp_relaxed;
   unsigned long long touched_total;  // (query, vertex) merges
   unsigned long long merge_scanned;  // Pareto entries compared in merge
+  unsigned long long cand_chunks;    // candidate chunks folded by merge
   int merge_chain_max;               // longest candidate list merged by one lane
   int far_chunks;                    // far-tier units handed out (bump)
   int drain_n;                       // drain-list segments queued for the next expand
@@ -224,6 +225,43 @@ __device__ __forceinline__ void block_add(unsigned long long* const (&ctr)[N], c
     for (int w = 0; w < (int)(blockDim.x + 63) / 64; w++) sum += red[threadIdx.x][w];
     if (sum) atomicAdd(ctr[threadIdx.x], (unsigned long long)sum);
   }
+}
+
+// Orders the calling wave's LDS accesses around it (a wave executes its LDS
+// instructions in order; this keeps the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Groups the wave's lanes with `act` set by a 64-bit key: a lane that joins
+// gets its group's lanes in `gm` and one of them as `lead`.  Matching goes
+// through a table of GROUP_SLOTS entries in LDS owned by the calling wave
+// (slot = hash of the key); a lane whose slot a different key took stays
+// alone (lead = lane, gm = its own bit, as on entry).  Called by every lane
+// of the wave; the mask table holds zeros on entry and again on return.
+constexpr int GROUP_SLOTS = 128;
+
+__device__ __forceinline__ void wave_group_by_key(int* s_lead, unsigned long long* s_mask, bool act,
+                                                  long long key, int lane, int& lead, unsigned long long& gm) {
+  const unsigned long long many = __ballot(act);
+  if ((many & (many - 1ull)) == 0ull) return;  // wave-uniform: at most one lane
+  const int hs = (int)(((unsigned long long)key * 0x9E3779B97F4A7C15ull) >> 57);
+  if (act) s_lead[hs] = lane;
+  wave_lds_order();
+  const int l = act ? s_lead[hs] : lane;
+  const int klo = __shfl((int)(key & 0xffffffffll), l, WAVE);
+  const int khi = __shfl((int)(key >> 32), l, WAVE);
+  const bool join = act && klo == (int)(key & 0xffffffffll) && khi == (int)(key >> 32);
+  if (join) atomicOr(&s_mask[hs], 1ull << lane);
+  wave_lds_order();
+  if (join) {
+    gm = s_mask[hs];
+    lead = l;
+  }
+  wave_lds_order();
+  if (join && lane == l) s_mask[hs] = 0ull;
 }
 
 __device__ __forceinline__ void wave_add(unsigned long long* ctr, int v) {
@@ -752,6 +790,12 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, int lpw) {
     c.ts[r & (TS_CAP - 1)] = wall_clock64();  // start of round r
   }
   const int lane = lane_id();
+  // per-wave tables grouping a step's candidates by (query, vertex)
+  __shared__ int s_lead[4][GROUP_SLOTS];
+  __shared__ unsigned long long s_mask[4][GROUP_SLOTS];
+  for (int j = threadIdx.x; j < 4 * GROUP_SLOTS; j += blockDim.x) (&s_mask[0][0])[j] = 0ull;
+  __syncthreads();
+  const int wid = threadIdx.x / WAVE;
   const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
   int n_exp = 0, n_rel = 0, n_cand = 0, n_far = 0, n_near = 0, n_scan = 0;
   const uint32_t* fmin_cur = c.q_fmin + (size_t)cur * c.B;
@@ -900,41 +944,60 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, int lpw) {
         // other in merge
         if (ok && front_dominates<Label_t>(c.pool_cd + (uint32_t)ps.x, pset_cnt(ps), son.vid, son)) ok = false;
       }
-      // lock-free append to the (query, vertex) chunk list: claim a slot in
-      // the head chunk, or publish a new head chunk holding the record (a
-      // head read before other lanes published theirs is still a chunk of
-      // this round's list; the CAS below links behind whatever is current)
-      bool first = false;
+      // Lock-free append to the (query, vertex) chunk list.  The wave's
+      // candidates for one (query, vertex) append as a group (several labels
+      // of one vertex reach the same neighbours in the same step): its leader
+      // claims slots in the head chunk for all of them, then publishes new
+      // full chunks for the rest in one run.  A head read before other lanes
+      // published theirs is still a chunk of this round's list; the CAS links
+      // behind whatever is current.
+      int lead = lane;
+      unsigned long long gm = 1ull << lane;
+      wave_group_by_key(s_lead[wid], s_mask[wid], ok, qv, lane, lead, gm);
+      const int gn = __popcll(gm);
+      const int rank = __popcll(gm & ((1ull << lane) - 1ull));
+      h = __shfl(h, lead, WAVE);
+      int4 rec = make_int4(0, 0, -1, 0);
       if (ok) {
-        n_cand++;
-        float2 pk = LT::pack(son);
-        int4 rec = make_int4(__float_as_int(pk.x), __float_as_int(pk.y), oid, 0);
-        bool done = false;
-        if (h >= 0) {
-          int slot = atomicAdd(&c.chd[h].x, 1);
-          if (slot < CCHUNK) {
-            c.crec[(long long)h * CCHUNK + slot] = rec;
-            done = true;
+        const float2 pk = LT::pack(son);
+        rec = make_int4(__float_as_int(pk.x), __float_as_int(pk.y), oid, 0);
+      }
+      int s0 = CCHUNK;  // first slot the group claimed in the head chunk
+      if (ok && lane == lead && h >= 0) s0 = atomicAdd(&c.chd[h].x, gn);
+      s0 = __shfl(s0, lead, WAVE);
+      const int nfit = min(max(CCHUNK - s0, 0), gn);
+      if (ok && rank < nfit) c.crec[(long long)h * CCHUNK + s0 + rank] = rec;
+      const int rest = gn - nfit;  // records for new chunks
+      bool first = false;
+      int nc = -1;
+      if (ok && lane == lead && rest > 0) {
+        const int k = (rest + CCHUNK - 1) / CCHUNK;
+        nc = atomicAdd(&ct->cand_ctr[cur], k);
+        if (nc < 0 || nc > c.chunk_cap - k) {
+          atomicOr(&ct->overflow, OVF_CAND);
+          nc = -1;
+        } else {
+          // chunks nc .. nc+k-1, full but the last; the last is the new head
+          // and links back to nc, which links to the list as it was
+          for (int j = 1; j < k; j++)
+            c.chd[nc + j] = make_int2(j < k - 1 ? CCHUNK : rest - (k - 1) * CCHUNK, nc + j - 1);
+          int expected = h;
+          for (;;) {
+            c.chd[nc] = make_int2(k == 1 ? rest : CCHUNK, expected);
+            __threadfence();
+            const int prev = atomicCAS(&c.cand_head[qv], expected, nc + k - 1);
+            if (prev == expected) break;
+            expected = prev;  // another lane published first: link behind it
           }
-        }
-        if (!done) {
-          int nc = atomicAdd(&ct->cand_ctr[cur], 1);
-          if (nc >= c.chunk_cap) {
-            atomicOr(&ct->overflow, OVF_CAND);
-          } else {
-            c.crec[(long long)nc * CCHUNK] = rec;
-            int expected = h;
-            for (;;) {
-              c.chd[nc] = make_int2(1, expected);
-              __threadfence();
-              int prev = atomicCAS(&c.cand_head[qv], expected, nc);
-              if (prev == expected) break;
-              expected = prev;  // another lane published first: link behind it
-            }
-            first = (expected < 0);
-          }
+          first = (expected < 0);
         }
       }
+      nc = __shfl(nc, lead, WAVE);
+      if (ok && rank >= nfit && nc >= 0) {
+        const int j = rank - nfit;
+        c.crec[(long long)(nc + j / CCHUNK) * CCHUNK + j % CCHUNK] = rec;
+      }
+      n_cand += ok ? 1 : 0;
       int tix = wave_alloc32(&ct->touched_n[cur], first);
       if (first) {
         if (tix < c.chunk_cap) c.touched[tix] = qv;
@@ -989,7 +1052,7 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
   const unsigned long long gmask = ((1ull << MG) - 1ull) << gbase;
   constexpr int GPW = WAVE / MG;  // groups per wave
   const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
-  int n_ins = 0, scanned = 0, steps_max = 0, n_touch = 0;
+  int n_ins = 0, scanned = 0, steps_max = 0, n_touch = 0, n_chunks = 0;
 
   for (long long base = ((long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * GPW;
        base < tn; base += nwaves * GPW) {
@@ -1018,6 +1081,17 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
     uint32_t gkey = __float_as_uint(FINF);
     int gins = 0;
     int steps = 0;
+    // The list is walked one chunk ahead: the next chunk's header and records
+    // are in flight while this chunk's label ids are allocated and its
+    // candidates folded.  A chunk's records are read with its header, not
+    // after it (lanes past the count read slots that are ignored).  Chunk
+    // links always name a chunk of this round or -1.
+    int2 hd = make_int2(0, -1);
+    int4 hrec = make_int4(0, 0, -1, 0);
+    if (h >= 0) {
+      hd = c.chd[h];
+      if (gl < CCHUNK) hrec = c.crec[(long long)h * CCHUNK + gl];
+    }
     while (__ballot(h >= 0)) {
       if (h >= 0 && ++steps > max_steps) {
         atomicOr(&ct->overflow, OVF_CAND);
@@ -1026,12 +1100,10 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
       int n8 = 0, nexth = -1;
       int4 myrec = make_int4(0, 0, -1, 0);
       if (h >= 0) {
-        // the chunk's records are read with its header, not after it (lanes
-        // past the count read slots that are ignored)
-        const int2 hd = c.chd[h];
-        if (gl < CCHUNK) myrec = c.crec[(long long)h * CCHUNK + gl];
+        myrec = hrec;
         n8 = min(hd.x, CCHUNK);
         nexth = hd.y;
+        n_chunks += gl == 0 ? 1 : 0;
       }
       int kmax = n8;
 #pragma unroll
@@ -1039,6 +1111,12 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
       // label ids for the whole chunk in one wave-wide allocation; the ids
       // of candidates that do not survive are marked dead below
       long long rbase = wave_alloc64_n(&ct->label_ctr, gl == 0 ? n8 : 0);
+      hd = make_int2(0, -1);
+      hrec = make_int4(0, 0, -1, 0);
+      if (nexth >= 0) {
+        hd = c.chd[nexth];
+        if (gl < CCHUNK) hrec = c.crec[(long long)nexth * CCHUNK + gl];
+      }
       rbase = __shfl(rbase, gbase, WAVE);
       int used = 0;
       for (int k = 0; k < kmax; k++) {
@@ -1199,9 +1277,9 @@ __global__ void __launch_bounds__(256) k_merge(Ctx c) {
   for (int o = 32; o > 0; o >>= 1) steps_max = max(steps_max, __shfl_xor(steps_max, o, WAVE));
   if (lane == 0 && steps_max > 0 && steps_max > load_agent_u32((const uint32_t*)&ct->merge_chain_max))
     atomicMax(&ct->merge_chain_max, steps_max);
-  unsigned long long* const ctrs[3] = {&ct->touched_total, &ct->merge_scanned, &ct->inserted};
-  const long long vals[3] = {n_touch, scanned, n_ins};
-  block_add<3>(ctrs, vals);
+  unsigned long long* const ctrs[4] = {&ct->touched_total, &ct->merge_scanned, &ct->inserted, &ct->cand_chunks};
+  const long long vals[4] = {n_touch, scanned, n_ins, n_chunks};
+  block_add<4>(ctrs, vals);
 }
 
 // ---------------------------------------------------------------------------

@@ -84,6 +84,7 @@ class Stats(ctypes.Structure):
         ("scanned", ctypes.c_int64),
         ("timed_rounds", ctypes.c_int64),
         ("far_units", ctypes.c_int64),
+        ("cand_chunks", ctypes.c_int64),
     ]
 
     def as_dict(self):
