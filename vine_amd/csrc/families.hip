@@ -481,7 +481,10 @@ vine_status run_search(vine_family_solver* s, FamCtx& c, int nb, const int32_t* 
   // rounds until the frontier is empty; the host looks every `sync` rounds
   int cur = 0;
   const int sync = 8;
-  const int grid = 2048;
+  // workgroups per round kernel: sized to the frontier the host last read
+  // (up to 16x growth until the next read; the kernels loop over any size),
+  // so that the many small rounds of a windowed search launch small grids
+  int grid = 2048;
   // every family here converges (dominance discards cycles); the cap only
   // turns a runaway search into an error
   const long long max_rounds = 64LL * n + 1024;
@@ -504,10 +507,13 @@ vine_status run_search(vine_family_solver* s, FamCtx& c, int nb, const int32_t* 
         return VINE_E_OVERFLOW;
       }
       if (h.fn[cur] == 0) break;
+      grid = (int)std::max<long long>(16, std::min<long long>(2048, (16LL * h.fn[cur] + kBlock - 1) / kBlock));
     }
   }
   return VINE_OK;
 }
+
+constexpr float BOUND_WINDOW = 2.f;
 
 // Distances to t for the pruned search: the generic search from each slot's
 // target on the transposed graph, then one value per (slot, vertex).
@@ -518,7 +524,10 @@ vine_status run_bound(vine_family_solver* s, int nb, float* out) {
   c.row = s->g->d_rrow;
   c.edges = s->g->d_redges;
   c.prune = 0;
-  c.window = 0.f;
+  // length and delay: windows of BOUND_WINDOW mean arcs order the search like
+  // delta-stepping (fewer re-relaxations; every label is still expanded, so
+  // the distances are exact).  Arc counts settle in BFS order without one.
+  c.window = W == 0 ? BOUND_WINDOW * s->g->mean_len : (W == 1 ? BOUND_WINDOW * s->g->mean_delay : 0.f);
   c.blen = c.bdelay = c.bhop = nullptr;
   vine_status r = run_search<L>(s, c, nb, s->d_idst);
   if (r != VINE_OK) return r;
