@@ -42,18 +42,23 @@ namespace vine {
 namespace bounds {
 // Distance to t for the bound searches of the pruned search: the generic
 // search itself, run from t on the transposed graph with a one-criterion
-// label (W = 0 length, 1 delay, 2 arcs), leaves at every vertex the single
-// label of its shortest distance.
+// label, leaves at every vertex the single label of its shortest distance.
+//   W = 0 length, 1 delay, 2 arcs;
+//   W = 3 length over the arcs whose delay is at least the query's limit
+//         (QoS: a feasible path uses only those arcs, so this is the length
+//         still needed, not just a lower bound of it)
 template <int W>
 struct dist_label {
   int vid;
   float d;
-  __host__ __device__ dist_label() : vid(-1), d(0.f) {}
-  __host__ __device__ explicit dist_label(int v) : vid(v), d(0.f) {}
+  float gate;  // W = 3: delay of the arc that reached vid, tested by the constraint
+  __host__ __device__ dist_label() : vid(-1), d(0.f), gate(INFINITY) {}
+  __host__ __device__ explicit dist_label(int v) : vid(v), d(0.f), gate(INFINITY) {}
   template <typename Edge_t>
   __host__ __device__ void expand(int next_v, const Edge_t& e, dist_label* son) const {
     son->vid = next_v;
-    son->d = d + (W == 0 ? e.len : (W == 1 ? e.delay : 1.f));
+    son->d = d + (W == 0 || W == 3 ? e.len : (W == 1 ? e.delay : 1.f));
+    son->gate = e.delay;
   }
   __host__ __device__ static bool dominate(const dist_label& a, const dist_label& b) { return a.d <= b.d; }
 };
@@ -61,11 +66,27 @@ struct no_constraint {
   template <typename L>
   __host__ __device__ bool pass_constraints_check(const L&) const { return true; }
 };
+// the QoS arc filter: an arc whose delay is below the limit is never used
+struct min_gate {
+  float L;
+  template <typename Lab>
+  __host__ __device__ bool pass_constraints_check(const Lab& l) const { return !(l.gate < L); }
+};
+template <int W>
+struct bound_constraint {
+  using type = no_constraint;
+  __host__ __device__ static type make(const float*) { return type(); }
+};
+template <>
+struct bound_constraint<3> {
+  using type = min_gate;
+  __host__ __device__ static type make(const float* lim) { return type{lim[0]}; }
+};
 }  // namespace bounds
 
 template <int W>
 struct family_traits<bounds::dist_label<W>> {
-  using constraint = bounds::no_constraint;
+  using constraint = typename bounds::bound_constraint<W>::type;
   __host__ __device__ static void fields(const bounds::dist_label<W>& l, float* f) {
     f[0] = l.d;
     f[1] = f[2] = f[3] = 0.f;
@@ -73,7 +94,7 @@ struct family_traits<bounds::dist_label<W>> {
   __host__ __device__ static bool better(const bounds::dist_label<W>& a, const bounds::dist_label<W>& b) {
     return a.d < b.d;
   }
-  __host__ __device__ static constraint make(const float*) { return constraint(); }
+  __host__ __device__ static constraint make(const float* lim) { return bounds::bound_constraint<W>::make(lim); }
 };
 }  // namespace vine
 
@@ -409,6 +430,13 @@ __global__ void k_map(int nb, const int32_t* perm, const int32_t* in, int32_t* o
   if (b < nb) out[b] = perm[in[b]];
 }
 
+// the bound searches run in the family's label arena
+static_assert(sizeof(vine::bounds::dist_label<3>) <= sizeof(vine::c_label) &&
+                  sizeof(vine::bounds::dist_label<3>) <= sizeof(vine::qos::c_label) &&
+                  sizeof(vine::bounds::dist_label<3>) <= sizeof(vine::hop::c_label) &&
+                  sizeof(vine::bounds::dist_label<3>) <= sizeof(vine::hop_maxcost::c_label),
+              "bound labels must fit the family's label slots");
+
 size_t label_size(int family) {
   switch (family) {
     case VINE_FAMILY_HOP: return sizeof(vine::hop::c_label);
@@ -527,7 +555,7 @@ vine_status run_bound(vine_family_solver* s, int nb, float* out) {
   // length and delay: windows of BOUND_WINDOW mean arcs order the search like
   // delta-stepping (fewer re-relaxations; every label is still expanded, so
   // the distances are exact).  Arc counts settle in BFS order without one.
-  c.window = W == 0 ? BOUND_WINDOW * s->g->mean_len : (W == 1 ? BOUND_WINDOW * s->g->mean_delay : 0.f);
+  c.window = (W == 0 || W == 3) ? BOUND_WINDOW * s->g->mean_len : (W == 1 ? BOUND_WINDOW * s->g->mean_delay : 0.f);
   c.blen = c.bdelay = c.bhop = nullptr;
   vine_status r = run_search<L>(s, c, nb, s->d_idst);
   if (r != VINE_OK) return r;
@@ -545,7 +573,8 @@ vine_status run_batch(vine_family_solver* s, int nb, int64_t* counters) {
   hipLaunchKernelGGL(k_map, dim3(grid_for(nb)), dim3(kBlock), 0, st, nb, s->g->d_perm, s->d_dst, s->d_idst);
   FAM_TRY(hipGetLastError());
   if (c.blen) {
-    vine_status r = run_bound<0>(s, nb, s->d_blen);
+    // QoS: length over the arcs the query may use
+    vine_status r = s->family == VINE_FAMILY_QOS ? run_bound<3>(s, nb, s->d_blen) : run_bound<0>(s, nb, s->d_blen);
     if (r != VINE_OK) return r;
   }
   if (c.bdelay) {
