@@ -117,15 +117,6 @@ def test_shipped_tech_routers_rf():
     assert fwd == rev
 
 
-def test_shipped_tech_internet_as_when_present():
-    path = "/root/reference/inputs/tech-internet-as.mtx.txt"
-    if not os.path.exists(path):
-        pytest.skip("reference inputs not mounted")
-    g = W.read_graph(path, W.FILE_SNAP)
-    assert g.n == 40164 and g.m == 2 * 85123
-    assert g.len.min() >= 1 and g.len.max() <= 25 and g.delay.min() >= 1 and g.delay.max() <= 10
-
-
 def test_shipped_rquery_files():
     total = 0
     for name in sorted(os.listdir(INPUTS)):
