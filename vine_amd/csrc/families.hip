@@ -77,6 +77,17 @@ struct bound_constraint {
   using type = no_constraint;
   __host__ __device__ static type make(const float*) { return type(); }
 };
+// delay to t: vertices beyond the query's delay limit are pruned anyway
+struct max_d {
+  float L;
+  template <typename Lab>
+  __host__ __device__ bool pass_constraints_check(const Lab& l) const { return !(l.d > L); }
+};
+template <>
+struct bound_constraint<1> {
+  using type = max_d;
+  __host__ __device__ static type make(const float* lim) { return type{lim[0]}; }
+};
 template <>
 struct bound_constraint<3> {
   using type = min_gate;
@@ -144,6 +155,9 @@ struct FamCtx {
   // query's smallest key and carries the rest to the next round
   float window;
   uint32_t* qkey;  // [2][B] smallest key of the query's frontier (float bits), per round parity
+  // bound searches only: per (slot, vertex) delay to t; a vertex farther than
+  // the query's delay limit is not entered (null: no gate)
+  const float* vgate;
 };
 
 // key of label L of slot q: length plus the length bound to t (if computed)
@@ -246,6 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_fam_expand(FamCtx c, int cur) {
       Label_t son;
       L.expand(e.x, vine::c_edge(__int_as_float(e.y), __int_as_float(e.z)), &son);
       if (!C.pass_constraints_check(son)) continue;  // the reference breaks here (algorithm/bucket.cuh:76-78)
+      if (c.vgate && !(c.vgate[(long long)q * c.n + son.vid] <= c.lim[3 * q])) continue;
       // target pruning: extending never makes a label better in the answer
       // order (every family's expand is monotone), so a son that is not
       // better than t's best so far cannot lead to the answer
@@ -557,6 +572,10 @@ vine_status run_bound(vine_family_solver* s, int nb, float* out) {
   // the distances are exact).  Arc counts settle in BFS order without one.
   c.window = (W == 0 || W == 3) ? BOUND_WINDOW * s->g->mean_len : (W == 1 ? BOUND_WINDOW * s->g->mean_delay : 0.f);
   c.blen = c.bdelay = c.bhop = nullptr;
+  // length and arc counts over the vertices within the query's delay limit of
+  // t (the delay search runs first): every feasible completion stays there,
+  // so the restricted distances are still lower bounds, and tighter ones
+  c.vgate = ((W == 0 || W == 2) && s->c.bdelay) ? s->d_bdelay : nullptr;
   vine_status r = run_search<L>(s, c, nb, s->d_idst);
   if (r != VINE_OK) return r;
   const long long cells = (long long)nb * s->g->n;
@@ -572,13 +591,13 @@ vine_status run_batch(vine_family_solver* s, int nb, int64_t* counters) {
   hipLaunchKernelGGL(k_map, dim3(grid_for(nb)), dim3(kBlock), 0, st, nb, s->g->d_perm, s->d_src, s->d_isrc);
   hipLaunchKernelGGL(k_map, dim3(grid_for(nb)), dim3(kBlock), 0, st, nb, s->g->d_perm, s->d_dst, s->d_idst);
   FAM_TRY(hipGetLastError());
+  if (c.bdelay) {  // first: it gates the length and arc-count searches
+    vine_status r = run_bound<1>(s, nb, s->d_bdelay);
+    if (r != VINE_OK) return r;
+  }
   if (c.blen) {
     // QoS: length over the arcs the query may use
     vine_status r = s->family == VINE_FAMILY_QOS ? run_bound<3>(s, nb, s->d_blen) : run_bound<0>(s, nb, s->d_blen);
-    if (r != VINE_OK) return r;
-  }
-  if (c.bdelay) {
-    vine_status r = run_bound<1>(s, nb, s->d_bdelay);
     if (r != VINE_OK) return r;
   }
   if (c.bhop) {
@@ -770,6 +789,7 @@ vine_status vine_family_solver_create(const vine_graph* g, int32_t family, int32
   c.best = s->d_bestlab;
   c.has_best = s->d_has_best;
   c.qkey = s->d_qkey;
+  c.vgate = nullptr;
   // windows need an order led by min d (all families but social)
   c.window = family == VINE_FAMILY_SOCIAL ? 0.f : window;
   // distance bounds for the pruned search, where the family's order and
