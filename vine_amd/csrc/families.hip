@@ -116,7 +116,7 @@ constexpr int kBlock = 256;
 struct FamCounters {
   int32_t label_ctr;  // next free label id
   int32_t fn[2];      // frontier sizes (current, next)
-  int32_t touched_n;  // (query, vertex) pairs with candidates this round
+  int32_t touched_n[2];  // (query, vertex) pairs with candidates, per round parity
   int32_t overflow;   // 1 = label capacity exceeded, 2 = path walk too long
   int32_t pad;
   long long expanded, relaxed, inserted, rounds;
@@ -204,7 +204,7 @@ __global__ void k_fam_init(FamCtx c, int nb, const int32_t* src) {
     c.ct->label_ctr = nb;
     c.ct->fn[0] = nb;
     c.ct->fn[1] = 0;
-    c.ct->touched_n = 0;
+    c.ct->touched_n[0] = c.ct->touched_n[1] = 0;
     c.ct->overflow = 0;
     c.ct->expanded = c.ct->relaxed = c.ct->inserted = c.ct->rounds = 0;
   }
@@ -227,10 +227,12 @@ __global__ void k_fam_init(FamCtx c, int nb, const int32_t* src) {
 }
 
 template <typename Label_t>
-__global__ __launch_bounds__(kBlock) void k_fam_expand(FamCtx c, int cur) {
+__global__ __launch_bounds__(kBlock) void k_fam_expand(FamCtx c, int cur, int first) {
   using Traits = vine::family_traits<Label_t>;
   Label_t* lab = (Label_t*)c.lab;
   const int F = c.ct->fn[cur];
+  // rounds counts the rounds that left a frontier behind
+  if (!first && F > 0 && blockIdx.x == 0 && threadIdx.x == 0) c.ct->rounds++;
   long long n_exp = 0, n_rel = 0;
   for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < F; f += gridDim.x * blockDim.x) {
     const int i = c.front[cur][f];
@@ -300,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_fam_expand(FamCtx c, int cur) {
       c.dead[id] = 0;
       const int old = atomicExch(&c.chead[key], id);
       c.cnext[id] = old;
-      if (old < 0) c.touched[atomicAdd(&c.ct->touched_n, 1)] = key;
+      if (old < 0) c.touched[atomicAdd(&c.ct->touched_n[cur], 1)] = key;
     }
   }
   block_add2(n_exp, n_rel, &c.ct->expanded, &c.ct->relaxed);
@@ -310,9 +312,15 @@ template <typename Label_t>
 __global__ __launch_bounds__(kBlock) void k_fam_merge(FamCtx c, int cur) {
   using Traits = vine::family_traits<Label_t>;
   Label_t* lab = (Label_t*)c.lab;
-  const int T = c.ct->touched_n;
+  const int T = c.ct->touched_n[cur];
   const int nx = cur ^ 1;
   long long n_ins = 0;
+  // the frontier expand consumed and the pair count of the round after this
+  // one: read by no kernel until the next expand
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c.ct->fn[cur] = 0;
+    c.ct->touched_n[nx] = 0;
+  }
   if (c.window > 0.f)  // this round's keys are consumed: start the round after next from +inf
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < c.B; b += gridDim.x * blockDim.x)
       c.qkey[cur * c.B + b] = __float_as_uint(INFINITY);
@@ -371,14 +379,6 @@ __global__ __launch_bounds__(kBlock) void k_fam_merge(FamCtx c, int cur) {
     }
   }
   block_add2(n_ins, 0, &c.ct->inserted, nullptr);
-}
-
-__global__ void k_fam_swap(FamCtx c, int cur) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    if (c.ct->fn[cur ^ 1] > 0) c.ct->rounds++;
-    c.ct->fn[cur] = 0;
-    c.ct->touched_n = 0;
-  }
 }
 
 // Best label of t's list per slot; path length by the parent chain.
@@ -536,9 +536,8 @@ vine_status run_search(vine_family_solver* s, FamCtx& c, int nb, const int32_t* 
       set_error("vine_family_solve: no convergence after %lld rounds", r);
       return VINE_E_HIP;
     }
-    hipLaunchKernelGGL(k_fam_expand<Label_t>, dim3(grid), dim3(kBlock), 0, st, c, cur);
+    hipLaunchKernelGGL(k_fam_expand<Label_t>, dim3(grid), dim3(kBlock), 0, st, c, cur, r == 0 ? 1 : 0);
     hipLaunchKernelGGL(k_fam_merge<Label_t>, dim3(grid), dim3(kBlock), 0, st, c, cur);
-    hipLaunchKernelGGL(k_fam_swap, dim3(1), dim3(64), 0, st, c, cur);
     cur ^= 1;
     if ((r + 1) % sync == 0) {
       FAM_TRY(hipGetLastError());
