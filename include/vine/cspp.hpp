@@ -119,7 +119,7 @@ class cspp {
     delay_l.assign(ntask, 0.f);
     std::vector<int32_t> feas(ntask, 0);
     std::vector<int64_t> off(ntask + 1, 0);
-    std::vector<int32_t> path((size_t)ntask * 64 + 1);
+    std::vector<int32_t> path((size_t)ntask * 256 + 1);  // retried at the reported size if short
     st = vine_solve(sv, ntask, src.data(), snk.data(), bound.data(), cost_l.data(), delay_l.data(), feas.data(),
                     off.data(), path.data(), (int64_t)path.size(), &stats);
     if (st == VINE_E_OVERFLOW && off[ntask] > (int64_t)path.size()) {
@@ -183,7 +183,7 @@ class cspp {
     label_l.assign(4 * (size_t)ntask, 0.f);
     std::vector<int32_t> feas(ntask, 0);
     std::vector<int64_t> off(ntask + 1, 0);
-    std::vector<int32_t> path((size_t)ntask * 64 + 1);
+    std::vector<int32_t> path((size_t)ntask * 256 + 1);  // retried at the reported size if short
     st = vine_family_solve(fs, ntask, src.data(), snk.data(), lim.data(), label_l.data(), feas.data(), off.data(),
                            path.data(), (int64_t)path.size(), nullptr);
     if (st == VINE_E_OVERFLOW && off[ntask] > (int64_t)path.size()) {

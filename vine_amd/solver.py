@@ -197,6 +197,9 @@ class Solver:
     def __init__(self, graph, **options):
         self.graph = graph if isinstance(graph, DeviceGraph) else DeviceGraph(graph)
         self.options = default_options(**options)
+        # path vertices per query seen so far: sizes the next path buffer so
+        # that a solve is not repeated for want of room (VINE_E_OVERFLOW)
+        self._path_per_query = 0.0
         h = ctypes.c_void_p()
         _check(_L().vine_solver_create(self.graph.h, ctypes.byref(self.options), ctypes.byref(h)))
         self.h = h
@@ -230,7 +233,7 @@ class Solver:
         feas = np.zeros(nq, np.int32)
         off = np.zeros(nq + 1, np.int64)
         if path_cap is None:
-            path_cap = max(1, nq * 64)
+            path_cap = max(1, nq * max(256, int(1.25 * self._path_per_query) + 1))
         while True:
             path = np.zeros(path_cap if paths else 1, np.int32)
             st = Stats()
@@ -243,6 +246,8 @@ class Solver:
                 continue
             _check(rc)
             break
+        if paths and nq:
+            self._path_per_query = max(self._path_per_query, float(off[-1]) / nq)
         return Answer(cost, delay, feas.astype(bool), off, path[:off[-1]] if paths else None, st.as_dict())
 
     def solve_device(self, d_src, d_dst, d_bound, d_cost, d_delay, d_feas, nq):
