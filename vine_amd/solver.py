@@ -353,7 +353,7 @@ class FamilySolver:
         feas = np.zeros(nq, np.int32)
         off = np.zeros(nq + 1, np.int64)
         cnt = np.zeros(4, np.int64)
-        path_cap = max(1, nq * 64)
+        path_cap = max(1, nq * 256)  # retried at the reported size if short
         while True:
             path = np.zeros(path_cap if paths else 1, np.int32)
             rc = _L().vine_family_solve(self.h, nq, src.ctypes.data, dst.ctypes.data, lim.ctypes.data,
