@@ -13,6 +13,16 @@
  *                        => cost = second key of Dijkstra on (delay,len);
  *                        bound < min delay => infeasible), the resource-expanded
  *                        DP below, and brute-force path enumeration on tiny graphs.
+ *   oracle_cspp_astar    a second, independent exact algorithm for the same
+ *                        answer: lexicographic bi-criteria A* (label-setting,
+ *                        the fast-dominance scheme of bi-objective A*, BOA*:
+ *                        Hernandez, Yeoh, Baier, Zhang, Koenig, Salzman,
+ *                        SoCS 2020) with exact reverse-Dijkstra heuristics.  Not
+ *                        a restatement of the reference: it expands each
+ *                        non-dominated label once, so it finishes the long
+ *                        queries the round-synchronous restatement cannot.
+ *                        Pinned by: equality with oracle_cspp on every query
+ *                        both finish, the DP and brute force below.
  *   oracle_dijkstra_lex  textbook Dijkstra on (w1, w2) pairs, lexicographic.
  *   oracle_resource_dp   f[b][v] = min len with delay <= b (integer delays >= 1).
  *   oracle_bruteforce    DFS over all simple paths (tiny graphs).
@@ -449,6 +459,160 @@ out:
   free(rdl);
   free(scratch);
   return rc;
+}
+
+/* ---------------------------------------------------------------------------
+ * Lexicographic bi-criteria A* (label-setting).  A label (v, d, delay) has the
+ * key f = (d + lbl[v], delay + lbd[v]) with lbl / lbd the exact shortest len /
+ * delay from v to t; both are consistent heuristics, so keys never decrease
+ * lexicographically along an arc and labels leave the heap in non-decreasing
+ * key order.  Labels of one vertex therefore leave in non-decreasing (d,
+ * delay) order, and a label is dominated (non-strictly, as c_label::dominate)
+ * by an earlier one at its vertex iff its delay is >= the smallest delay
+ * expanded there (g2min[v]).  A label whose delay plus lbd exceeds the bound
+ * cannot complete (the constraint delay <= bound).  The first label taken at t
+ * is the lexicographic minimum (d, delay) over feasible walks: the answer of
+ * oracle_cspp.  counters[0..2] = labels expanded, arcs relaxed, labels pushed.
+ * Returns 1 feasible, 0 infeasible, negative on error (-4: time limit).
+ */
+typedef struct {
+  double f1, f2;
+  int32_t id;
+} aitem;
+
+static int aless(const aitem* a, const aitem* b) {
+  return a->f1 < b->f1 || (a->f1 == b->f1 && (a->f2 < b->f2 || (a->f2 == b->f2 && a->id < b->id)));
+}
+
+int oracle_cspp_astar(int32_t n, const int32_t* row_ptr, const int32_t* dst, const float* len,
+                      const float* delay, int32_t s, int32_t t, float bound, float* out_cost,
+                      float* out_delay, int32_t* out_path, int32_t path_cap, int32_t* out_path_len,
+                      int64_t* counters, double time_limit_s) {
+  if (n <= 0 || s < 0 || s >= n || t < 0 || t >= n) return -1;
+  int64_t cx = 0, ce = 0, ci = 0;
+  double* lbl = (double*)malloc(sizeof(double) * (size_t)n);
+  double* lbd = (double*)malloc(sizeof(double) * (size_t)n);
+  float* g2min = (float*)malloc(sizeof(float) * (size_t)n);
+  olabel* log = NULL;
+  aitem* h = NULL;
+  int64_t log_n = 0, log_cap = 0, hn = 0, hcap = 0;
+  int rc = 0, best = -1;
+  if (!lbl || !lbd || !g2min || reverse_bounds(n, row_ptr, dst, len, delay, t, lbl, lbd) != 0) {
+    rc = -2;
+    goto done;
+  }
+  for (int32_t v = 0; v < n; v++) g2min[v] = INFINITY;
+#define A_PUSH(L, F1, F2)                                                                 \
+  do {                                                                                    \
+    if (log_n == log_cap) {                                                               \
+      int64_t nc = log_cap ? log_cap * 2 : 4096;                                          \
+      olabel* nl = (olabel*)realloc(log, sizeof(olabel) * (size_t)nc);                    \
+      if (!nl) {                                                                          \
+        rc = -2;                                                                          \
+        goto done;                                                                        \
+      }                                                                                   \
+      log = nl;                                                                           \
+      log_cap = nc;                                                                       \
+    }                                                                                     \
+    log[log_n] = (L);                                                                     \
+    if (hn == hcap) {                                                                     \
+      int64_t nc = hcap ? hcap * 2 : 4096;                                                \
+      aitem* nh = (aitem*)realloc(h, sizeof(aitem) * (size_t)nc);                         \
+      if (!nh) {                                                                          \
+        rc = -2;                                                                          \
+        goto done;                                                                        \
+      }                                                                                   \
+      h = nh;                                                                             \
+      hcap = nc;                                                                          \
+    }                                                                                     \
+    int64_t i_ = hn++;                                                                    \
+    h[i_] = (aitem){(F1), (F2), (int32_t)log_n};                                          \
+    while (i_ > 0) {                                                                      \
+      int64_t p_ = (i_ - 1) / 2;                                                          \
+      if (!aless(&h[i_], &h[p_])) break;                                                  \
+      aitem x_ = h[i_];                                                                   \
+      h[i_] = h[p_];                                                                      \
+      h[p_] = x_;                                                                         \
+      i_ = p_;                                                                            \
+    }                                                                                     \
+    log_n++;                                                                              \
+    ci++;                                                                                 \
+  } while (0)
+
+  if (lbd[s] <= (double)bound) {
+    olabel src = {s, 0.0f, 0.0f, -1};
+    A_PUSH(src, lbl[s], lbd[s]);
+  }
+  const double t_begin = thread_cpu_s();
+  while (hn > 0) {
+    aitem top = h[0];
+    h[0] = h[--hn];
+    for (int64_t i = 0;;) {
+      int64_t l = 2 * i + 1, r = l + 1, b = i;
+      if (l < hn && aless(&h[l], &h[b])) b = l;
+      if (r < hn && aless(&h[r], &h[b])) b = r;
+      if (b == i) break;
+      aitem x = h[i];
+      h[i] = h[b];
+      h[b] = x;
+      i = b;
+    }
+    olabel f = log[top.id];
+    if (f.delay >= g2min[f.vid]) continue; /* dominated by a label expanded earlier at f.vid */
+    g2min[f.vid] = f.delay;
+    if (f.vid == t) {
+      best = top.id;
+      break;
+    }
+    cx++;
+    if (time_limit_s > 0 && (cx & 0xffff) == 0 && thread_cpu_s() - t_begin > time_limit_s) {
+      rc = -4;
+      goto done;
+    }
+    for (int32_t p = row_ptr[f.vid]; p < row_ptr[f.vid + 1]; p++) {
+      ce++;
+      olabel nl;
+      nl.vid = dst[p];
+      nl.d = f.d + len[p];
+      nl.delay = f.delay + delay[p];
+      nl.parent = top.id;
+      if (nl.delay > bound) continue;                           /* !pass_constraints_check */
+      double f2 = (double)nl.delay + lbd[nl.vid];
+      if (f2 > (double)bound) continue;                         /* no feasible completion */
+      if (nl.delay >= g2min[nl.vid]) continue;                  /* dominated at its vertex */
+      A_PUSH(nl, (double)nl.d + lbl[nl.vid], f2);
+    }
+  }
+  if (best < 0) {
+    if (out_cost) *out_cost = INFINITY;
+    if (out_delay) *out_delay = INFINITY;
+    if (out_path_len) *out_path_len = 0;
+    rc = 0;
+  } else {
+    if (out_cost) *out_cost = log[best].d;
+    if (out_delay) *out_delay = log[best].delay;
+    int32_t hops = 0;
+    for (int32_t c = best; c >= 0; c = log[c].parent) hops++;
+    if (out_path_len) *out_path_len = hops;
+    if (out_path && hops <= path_cap) {
+      int32_t k = hops - 1;
+      for (int32_t c = best; c >= 0; c = log[c].parent) out_path[k--] = log[c].vid;
+    }
+    rc = 1;
+  }
+done:
+  if (counters) {
+    counters[0] = cx;
+    counters[1] = ce;
+    counters[2] = ci;
+  }
+  free(lbl);
+  free(lbd);
+  free(g2min);
+  free(log);
+  free(h);
+  return rc;
+#undef A_PUSH
 }
 
 /* ---------------------------------------------------------------------------

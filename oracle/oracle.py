@@ -35,6 +35,8 @@ def lib():
         L.oracle_cspp.restype = i32
         L.oracle_cspp.argtypes = [i32, vp, vp, vp, vp, i32, i32, f32, i32, vp, vp, vp, i32, vp, vp, i32,
                                   vp, vp, vp, i32, f64]
+        L.oracle_cspp_astar.restype = i32
+        L.oracle_cspp_astar.argtypes = [i32, vp, vp, vp, vp, i32, i32, f32, vp, vp, vp, i32, vp, vp, f64]
         L.oracle_dijkstra_lex.restype = i32
         L.oracle_dijkstra_lex.argtypes = [i32, vp, vp, vp, vp, i32, vp, vp]
         L.oracle_resource_dp.restype = i32
@@ -103,6 +105,29 @@ def cspp(g, s, t, bound, prune_target=False, path_cap=None, pareto_cap=4096, tra
     return Result(rc == 1, cost.value, dly.value,
                   path[:plen.value].copy() if plen.value <= path_cap else None,
                   par[:2 * k].reshape(k, 2).copy(), counters, tr)
+
+
+def cspp_astar(g, s, t, bound, path_cap=None, time_limit=0.0):
+    """oracle_cspp_astar: the same answer as cspp() by lexicographic bi-criteria
+    A* (label-setting, exact reverse-Dijkstra heuristics).  Returns Result
+    (pareto empty; counters expanded / relaxed / inserted = labels pushed)."""
+    path_cap = g.n if path_cap is None else path_cap
+    cost = ctypes.c_float()
+    dly = ctypes.c_float()
+    path = np.zeros(max(path_cap, 1), np.int32)
+    plen = ctypes.c_int32()
+    cnt = np.zeros(3, np.int64)
+    rc = lib().oracle_cspp_astar(*_g(g), int(s), int(t), float(bound), ctypes.byref(cost), ctypes.byref(dly),
+                                 path.ctypes.data, path_cap, ctypes.byref(plen), cnt.ctypes.data,
+                                 float(time_limit))
+    if rc == -4:
+        raise TimeLimit("oracle_cspp_astar exceeded %.1f s" % time_limit)
+    if rc < 0:
+        raise RuntimeError("oracle_cspp_astar failed (%d)" % rc)
+    counters = dict(zip(("expanded", "relaxed", "inserted"), (int(x) for x in cnt)))
+    return Result(rc == 1, cost.value, dly.value,
+                  path[:plen.value].copy() if plen.value <= path_cap else None,
+                  np.zeros((0, 2), np.float32), counters)
 
 
 @dataclass

@@ -8,6 +8,10 @@ checked against checks that do not share its code path:
   - invariants (valid paths, sums, mutually non-dominated Pareto sets,
     monotonicity in the bound) and metamorphic relations (scaling lengths,
     relabelling vertices).
+oracle_cspp_astar (label-setting, used for the full-size answer fixtures) is
+checked the same way: against oracle_cspp on every grid query and on a
+geometric-graph sample, the closed forms, brute force on tiny graphs, and the
+committed fixtures' cross-checked queries.
 The AR restatements are checked bit-for-bit against golden vectors produced
 by the reference's own include/AR.cpp (tests/golden/make_golden.py) and, when
 the reference build oracle/_ref exists, against it directly.
@@ -207,3 +211,80 @@ def test_ar_burg_math():
     c = O.ar_burg(x, 1)
     assert abs(c[0] - a) < 0.02
     assert abs(-c[0] - a) > 1.0
+
+
+def test_label_setting_matches_restated(grid):
+    """The label-setting oracle against the restated search on every grid
+    query, and the closed forms for tight and too-small bounds (P2, P3)."""
+    g, q, res = grid
+    for i in range(len(q)):
+        r = res[i]
+        a = O.cspp_astar(g, q.src[i], q.dst[i], q.bound[i])
+        assert (a.feasible, a.cost, a.delay) == (r.feasible, r.cost, r.delay), i
+        if not a.feasible:
+            continue
+        check_path(g, a.path, q.src[i], q.dst[i], a.cost, a.delay, q.bound[i])
+        assert a.counters["expanded"] <= r.counters["expanded"]
+        d1, d2 = O.dijkstra_lex(g, int(q.src[i]), "delay", "len")
+        t = int(q.dst[i])
+        at = O.cspp_astar(g, q.src[i], t, d1[t])
+        assert (at.feasible, at.cost, at.delay) == (True, d2[t], d1[t])
+        assert not O.cspp_astar(g, q.src[i], t, d1[t] - 1).feasible
+
+
+def test_label_setting_bruteforce_tiny():
+    """P5 for the label-setting oracle: parallel arcs, zero lengths, cycles."""
+    rng = np.random.default_rng(12)
+    checked = 0
+    for g in _tiny(4, 200):
+        for _ in range(6):
+            s, t = int(rng.integers(0, g.n)), int(rng.integers(0, g.n))
+            b = float(rng.integers(0, 25))
+            f, c, d = O.bruteforce(g, s, t, b)
+            a = O.cspp_astar(g, s, t, b)
+            assert a.feasible == f
+            if f:
+                assert (a.cost, a.delay) == (c, d)
+                check_path(g, a.path, s, t, a.cost, a.delay, b)
+            checked += f
+    assert checked > 300
+
+
+def test_label_setting_geometric_sample():
+    """A 20k-vertex 3-NN graph, hop-distance queries with tight bounds: the
+    label-setting oracle against the restated search with its bounds."""
+    g = W.knn(20000, seed=9)
+    q = W.hop_queries(g, 40, hop_lo=5, hop_hi=60, f_permille=(1020, 1400), seed=9)
+    for i in range(len(q)):
+        r = O.cspp(g, q.src[i], q.dst[i], q.bound[i], prune_target=2)
+        a = O.cspp_astar(g, q.src[i], q.dst[i], q.bound[i])
+        assert (a.feasible, a.cost, a.delay) == (r.feasible, r.cost, r.delay), i
+        if a.feasible:
+            check_path(g, a.path, q.src[i], q.dst[i], a.cost, a.delay, q.bound[i])
+
+
+def test_full_fixture_ny_batch():
+    """The committed answers of the 4,096-query batched workload: its inputs are
+    what the seeded generators produce, the restated search finished and agreed
+    on most queries when it was written, and a sample of those is re-solved
+    here by both oracles."""
+    from helpers import fixture_answers, graph_crc
+    fx = fixture_answers("ny_batch")
+    g, q = W.preset("ny_batch")
+    assert graph_crc(g) == int(fx["graph_crc"])
+    assert np.array_equal(fx["src"], q.src) and np.array_equal(fx["dst"], q.dst)
+    assert np.array_equal(fx["bound"], q.bound)
+    assert fx["feasible"].all()  # hop queries are drawn reachable, bound >= min delay
+    done = np.nonzero(fx["restated"] == 1)[0]
+    assert done.size > 0.6 * len(q)
+    # the cheapest cross-checked queries by the label-setting work, plus a few random ones
+    pick = np.concatenate([done[np.argsort(fx["X"][done])[:6]],
+                           np.random.default_rng(2).choice(done, 4, replace=False)])
+    for i in pick:
+        a = O.cspp_astar(g, q.src[i], q.dst[i], q.bound[i])
+        assert (a.feasible, np.float32(a.cost), np.float32(a.delay)) == (True, fx["cost"][i], fx["delay"][i])
+        assert (a.counters["expanded"], a.counters["relaxed"]) == (fx["X"][i], fx["E"][i])
+        assert len(a.path) - 1 == fx["hops"][i]
+    for i in pick[:6]:
+        r = O.cspp(g, q.src[i], q.dst[i], q.bound[i], prune_target=2)
+        assert (r.feasible, np.float32(r.cost), np.float32(r.delay)) == (True, fx["cost"][i], fx["delay"][i])
