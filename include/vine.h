@@ -53,7 +53,9 @@ typedef enum {
 enum {
   VINE_PRUNE_NONE = 0,   /* reference one-to-all search: full Pareto sets at every vertex */
   VINE_PRUNE_TARGET = 1, /* drop labels lexicographically no better than t's best */
-  VINE_PRUNE_BOUNDS = 2  /* lower bounds on remaining len/delay to t (reverse SSSP on the GPU) */
+  VINE_PRUNE_BOUNDS = 2  /* lower bounds on remaining len/delay to t (reverse SSSP on the GPU); exact for
+                            integer-valued weights below 2^24, so a graph with other weights runs
+                            VINE_PRUNE_TARGET instead (same answers) */
 };
 
 /* Bucket-width (frontier window) policies.  Scheduling only; never change answers. */
@@ -81,9 +83,15 @@ typedef struct {
   int32_t prune;            /* VINE_PRUNE_* */
   int32_t delta_policy;     /* VINE_DELTA_* */
   float delta_fixed;        /* window for VINE_DELTA_FIXED */
-  int32_t use_graph;        /* reserved, must be 0 (hipGraph capture of rounds is not built) */
+  int32_t use_graph;        /* 1: the rounds_per_sync rounds between two host checks are captured once
+                               into a HIP graph and replayed (one graph launch per check) with the
+                               full grid_blocks grid; with time_kernels the first round of each check
+                               is launched directly and timed, the graph holds the rest.
+                               VINE_DELTA_AR launches directly (it sets the window from the host after
+                               every round).  0: direct launches */
   int32_t rounds_per_sync;  /* rounds between host checks of the frontier size */
-  int32_t block_size;       /* threads per workgroup of the round kernels (64, 128, 192 or 256) */
+  int32_t block_size;       /* threads per workgroup of the round kernels (64, 128, 192 or 256; anything
+                               else is refused with VINE_E_ARG) */
   int32_t grid_blocks;      /* workgroups per round kernel at most (0 = 384) */
   int32_t hash_slots;       /* reserved, must be 0 (no LDS candidate table is built) */
   int64_t label_capacity;   /* label log entries (0 = auto from free memory) */

@@ -179,3 +179,123 @@ def test_workgroup_engine_eligibility(gpu, grid_case):
     with S.Solver(g, max_batch=4, prune=1, delta_policy=S.DELTA_FIXED, delta_fixed=10.0) as s:
         ans = s.solve(q.src[:4], q.dst[:4], q.bound[:4])
     _compare(g, q, ref, ans, idx=range(4))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("block", [0, 32, 96, 320, 512, 1024])
+def test_block_size_refused(gpu, grid_case, block):
+    """The round kernels keep per-wave LDS tables for at most 256 threads (four
+    waves) per workgroup: a larger workgroup, or one that is not whole waves,
+    is refused with VINE_E_ARG when the solver is created, before any launch."""
+    g, q, ref = grid_case
+    print("block_size %d refused" % block, flush=True)
+    with pytest.raises(S.VineError) as e:
+        S.Solver(g, max_batch=4, block_size=block)
+    assert e.value.status == S.VINE_E_ARG
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("block", [64, 192, 256])
+def test_block_sizes_accepted(gpu, grid_case, block):
+    g, q, ref = grid_case
+    print("block_size %d" % block, flush=True)
+    with S.Solver(g, max_batch=25, prune=2, delta_policy=S.DELTA_FIXED, delta_fixed=300.0, block_size=block) as s:
+        ans = s.solve(q.src, q.dst, q.bound)
+    _compare(g, q, ref, ans)
+
+
+def _fractional_knn(n, seed):
+    """k-NN graph with weights that are multiples of 0.1 (not exact in binary):
+    float sums then depend on their order."""
+    g = W.knn(n, seed=seed)
+    rng = np.random.default_rng(seed)
+    ln = (rng.integers(1, 100, g.m) * 0.1).astype(np.float32)
+    dl = (rng.integers(1, 50, g.m) * 0.1).astype(np.float32)
+    return W.from_csr(g.row_ptr, g.dst, ln, dl, "knn-fractional")
+
+
+@pytest.mark.gpu
+def test_fractional_weights(gpu):
+    """With weights that are not integers the bound pruning would add a label's
+    delays in another order than the label (0.1 + (0.5 + 0.1) > (0.1 + 0.1) +
+    0.5 in float32) and could drop the only feasible path; such graphs run
+    target pruning, which compares a label's own sums only.  Answers against the
+    oracle's reference search (one-to-all, no pruning) on the same floats."""
+    # the path 0 -> 1 -> 2 -> 3 whose delay sum equals the bound exactly
+    row = np.array([0, 1, 2, 3, 3], np.int32)
+    dst = np.array([1, 2, 3], np.int32)
+    ln = np.ones(3, np.float32)
+    dl = np.array([0.1, 0.1, 0.5], np.float32)
+    g = W.from_csr(row, dst, ln, dl, "path")
+    bound = np.float32(np.float32(0.1) + np.float32(0.1)) + np.float32(0.5)
+    assert np.float32(0.1) + np.float32(np.float32(0.5) + np.float32(0.1)) > bound
+    ref = O.cspp(g, 0, 3, bound)
+    assert ref.feasible and ref.delay == bound
+    for prune in (S.PRUNE_TARGET, S.PRUNE_BOUNDS):
+        with S.Solver(g, max_batch=1, prune=prune) as s:
+            ans = s.solve(np.array([0], np.int32), np.array([3], np.int32), np.array([bound], np.float32))
+        assert ans.feasible[0] and ans.cost[0] == 3 and ans.delay[0] == bound, prune
+    # random queries on a geometric graph, every pruning mode and a window
+    g = _fractional_knn(1500, 4)
+    rng = np.random.default_rng(4)
+    nq = 24
+    src = rng.integers(0, g.n, nq).astype(np.int32)
+    dst = rng.integers(0, g.n, nq).astype(np.int32)
+    bound = np.zeros(nq, np.float32)
+    for i in range(nq):
+        md = O.dijkstra_lex(g, src[i], "delay", "len")[0][dst[i]]
+        bound[i] = np.float32(1.2 * md) if np.isfinite(md) else np.float32(1.0)
+    print("fractional weights: %d queries on %d vertices" % (nq, g.n), flush=True)
+    ref = [O.cspp(g, src[i], dst[i], bound[i]) for i in range(nq)]
+    q = W.Queries(src, dst, bound, None)
+    for kw in ({"prune": S.PRUNE_BOUNDS}, {"prune": S.PRUNE_TARGET},
+               {"prune": S.PRUNE_BOUNDS, "delta_policy": S.DELTA_FIXED, "delta_fixed": 5.0}):
+        with S.Solver(g, max_batch=8, **kw) as s:
+            ans = s.solve(src, dst, bound)
+        _compare(g, q, ref, ans)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("policy", [S.DELTA_FIXED, S.DELTA_MLP, S.DELTA_INF])
+def test_graph_replay(gpu, grid_case, policy):
+    """use_graph = 1: every host check replays the captured chunk of rounds.
+    Several solves in a row on one handle (the graph is captured again when
+    the batch size changes), each against the oracle and against direct
+    launches, with the first round of every chunk timed through event nodes."""
+    from vine_amd import learn as L
+    g, q, ref = grid_case
+    delta = {S.DELTA_FIXED: 300.0, S.DELTA_MLP: 4.0, S.DELTA_INF: 0.0}[policy]
+    kw = dict(max_batch=25, prune=2, delta_policy=policy, delta_fixed=delta, rounds_per_sync=4, time_kernels=1)
+    print("graph replay policy=%d" % policy, flush=True)
+    with S.Solver(g, **kw) as d:
+        if policy == S.DELTA_MLP:
+            d.set_mlp(np.load(L.DEFAULT_MODEL))
+        direct = d.solve(q.src, q.dst, q.bound)
+    _compare(g, q, ref, direct)
+    with S.Solver(g, use_graph=1, **kw) as s:
+        if policy == S.DELTA_MLP:
+            s.set_mlp(np.load(L.DEFAULT_MODEL))
+        for rep in range(3):
+            a = s.solve(q.src, q.dst, q.bound)
+            _compare(g, q, ref, a)
+            assert np.array_equal(a.cost, direct.cost) and np.array_equal(a.delay, direct.delay), rep
+            assert a.stats["timed_rounds"] > 0 and a.stats["ms_expand"] > 0, rep
+        a = s.solve(q.src[:10], q.dst[:10], q.bound[:10])
+        _compare(g, q, ref, a, idx=range(10))
+        a = s.solve(q.src, q.dst, q.bound)
+        _compare(g, q, ref, a)
+
+
+@pytest.mark.gpu
+def test_graph_replay_streams(gpu):
+    """use_graph with three internal solvers (each captures on its own stream
+    from its own host thread), on the geometric sample, against the oracle."""
+    g = W.knn(20000, seed=5)
+    q = W.hop_queries(g, 30, hop_lo=5, hop_hi=80, f_permille=(1050, 1500), seed=5)
+    ref = [O.cspp(g, q.src[i], q.dst[i], q.bound[i], prune_target=2) for i in range(len(q))]
+    print("graph replay, 3 streams", flush=True)
+    with S.Solver(g, max_batch=30, prune=2, delta_policy=S.DELTA_FIXED, delta_fixed=500.0, rounds_per_sync=8,
+                  use_graph=1, concurrency=3) as s:
+        for rep in range(2):
+            ans = s.solve(q.src, q.dst, q.bound)
+            _compare(g, q, ref, ans)

@@ -121,6 +121,11 @@ struct vine_solver {
   int64_t mlp_nparams = 0;
 
   Ctx ctx{};
+  // use_graph: the rounds between two host checks captured once as a HIP
+  // graph and replayed; re-captured when the batch size or chunk changes
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_nb = -1, graph_k = 0, graph_gsz = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // HIP-event brackets of timed launches, resolved at the end of each batch
   struct EvRec {
@@ -331,6 +336,8 @@ vine_status vine_solver_destroy(vine_solver* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (void* p : s->allocs) (void)hipFree(p);
   if (s->h_ct) (void)hipHostFree(s->h_ct);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : s->evpool) (void)hipEventDestroy(e);
@@ -348,11 +355,24 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
   if (opt) o = *opt;
   else vine_options_default(&o);
   if (o.max_batch < 1 || o.prune < 0 || o.prune > 2 || o.delta_policy < 0 || o.delta_policy > 3 ||
-      o.block_size < 64 || o.block_size > 256 || o.block_size % 64 || o.rounds_per_sync < 1 ||
-      o.grid_blocks < 0 || o.hash_slots < 0 || !(o.delta_fixed >= 0.f) || o.bound_ball < 0 || o.bound_ball > 1) {
+      o.rounds_per_sync < 1 || o.grid_blocks < 0 || o.hash_slots < 0 || !(o.delta_fixed >= 0.f) ||
+      o.bound_ball < 0 || o.bound_ball > 1) {
     set_error("vine_solver_create: invalid options");
     return VINE_E_ARG;
   }
+  // the round kernels' per-wave LDS tables hold dev::MAX_WAVES waves: a larger
+  // workgroup would index past them, so it is refused before any launch
+  if (o.block_size < dev::WAVE || o.block_size > dev::MAX_BLOCK || o.block_size % dev::WAVE) {
+    set_error("vine_solver_create: block_size %d must be a multiple of %d in [%d, %d]", o.block_size, dev::WAVE,
+              dev::WAVE, dev::MAX_BLOCK);
+    return VINE_E_ARG;
+  }
+  // The bound pruning (lower bounds from reverse searches, the upper bound
+  // from a minimum-delay path) adds a label's sums in another order than the
+  // label itself; that is exact only while every sum is an integer below 2^24.
+  // Other weights fall back to target pruning, which compares a label's own
+  // sums only and is exact for any non-negative weights.
+  if (o.prune == VINE_PRUNE_BOUNDS && !g->integer_weights) o.prune = VINE_PRUNE_TARGET;
   if (o.labels_per_wave < 0 || o.labels_per_wave > 64 || (o.labels_per_wave & (o.labels_per_wave - 1))) {
     set_error("vine_solver_create: labels_per_wave must be 0 or a power of two <= 64");
     return VINE_E_ARG;
@@ -367,13 +387,16 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
                      (o.delta_policy == VINE_DELTA_FIXED || o.delta_policy == VINE_DELTA_MLP) &&
                      g->max_deg <= dev::ARENA / 2;
   if (o.engine == VINE_ENGINE_WORKGROUP && !wg_ok) {
-    set_error("vine_solver_create: the workgroup engine needs VINE_PRUNE_BOUNDS, a fixed or learned window "
-            "and out-degrees <= %d (graph: %d)", dev::ARENA / 2, g->max_deg);
+    set_error("vine_solver_create: the workgroup engine needs VINE_PRUNE_BOUNDS (integer weights), a fixed or "
+            "learned window and out-degrees <= %d (graph: %d)", dev::ARENA / 2, g->max_deg);
     return VINE_E_ARG;
   }
-  if (o.use_graph || o.hash_slots) {
-    set_error("vine_solver_create: use_graph and hash_slots are reserved (rounds are enqueued "
-            "rounds_per_sync ahead; fronts live in the HBM pool) and must be 0");
+  if (o.use_graph < 0 || o.use_graph > 1) {
+    set_error("vine_solver_create: use_graph must be 0 or 1");
+    return VINE_E_ARG;
+  }
+  if (o.hash_slots) {
+    set_error("vine_solver_create: hash_slots is reserved (fronts live in the HBM pool) and must be 0");
     return VINE_E_ARG;
   }
   if (o.concurrency < 0 || o.concurrency > 16) {
@@ -424,8 +447,11 @@ vine_status create_single(const vine_graph* g, const vine_options& o, double mem
     vine_solver_destroy(s);
     return VINE_E_HIP;
   }
-  // per-(query, vertex) state
-  size_t per_qv = sizeof(int2) + sizeof(int32_t) + (o.prune == 2 ? 4 * sizeof(float) : 0);
+  // per-(query, vertex) state: pset, cand_head; with the bounds lbl, lbd, inq
+  // and the two bound-search queue entries
+  size_t per_qv = sizeof(int2) + sizeof(int32_t) +
+                  (o.prune == 2 ? sizeof(float) + sizeof(unsigned long long) + sizeof(uint32_t) + 2 * sizeof(int32_t)
+                                : 0);
   double remaining = (double)freeb - (double)per_qv * BN - 64.0 * (1 << 20);
   if (remaining <= 0) {
     set_error("vine_solver_create: %lld query slots x %lld vertices need %.1f GB, %.1f GB free",
@@ -693,11 +719,14 @@ vine_status poll(vine_solver* s) {
   return VINE_OK;
 }
 
+// sample: 1 brackets this round's kernels with HIP events, 0 never, -1 every
+// time_kernels-th round
 template <int PRUNE>
-vine_status launch_round(vine_solver* s, int gsz, vine_stats* st) {
+vine_status launch_round(vine_solver* s, int gsz, vine_stats* st, int sample_mode) {
   Ctx& c = s->ctx;
   const int tk = std::max(1, s->opt.time_kernels);
-  const bool sample = s->opt.time_kernels != 0 && (s->round_seq++ % tk) == 0;
+  const bool sample = s->opt.time_kernels != 0 &&
+                      (sample_mode < 0 ? (s->round_seq++ % tk) == 0 : sample_mode > 0);
   if (sample) st->timed_rounds++;
   {
     Timer t(s, 0, sample);
@@ -724,15 +753,105 @@ vine_status launch_round(vine_solver* s, int gsz, vine_stats* st) {
   return VINE_OK;
 }
 
-vine_status launch_round_any(vine_solver* s, int gsz, vine_stats* st) {
+vine_status launch_round_any(vine_solver* s, int gsz, vine_stats* st, int sample_mode = -1) {
   switch (s->opt.prune) {
     case 0:
-      return launch_round<0>(s, gsz, st);
+      return launch_round<0>(s, gsz, st, sample_mode);
     case 1:
-      return launch_round<1>(s, gsz, st);
+      return launch_round<1>(s, gsz, st, sample_mode);
     default:
-      return launch_round<2>(s, gsz, st);
+      return launch_round<2>(s, gsz, st, sample_mode);
   }
+}
+
+// use_graph: K rounds (expand, merge, pull/policy each) as one HIP graph.
+// Every round kernel reads its round number and frontier sizes from the
+// device counters, so the same K launches are valid for any K consecutive
+// rounds: the graph is captured once per (batch size, K) and replayed.  The
+// grid is the full s->grid (grid-stride loops cover any frontier size).
+// With time_kernels the first round of each chunk is launched directly,
+// bracketed by HIP events, and the graph holds the other K - 1.
+vine_status capture_rounds(vine_solver* s, int K) {
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->gexec = nullptr;
+  s->graph = nullptr;
+  const int gsz = s->grid;
+  Ctx& c = s->ctx;
+  HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t err = hipSuccess;
+  for (int k = 0; k < K && err == hipSuccess; k++) {
+    switch (s->opt.prune) {
+      case 0:
+        hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, 0>), dim3(gsz), dim3(s->block), 0,
+                           s->stream, c, s->lpw);
+        break;
+      case 1:
+        hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, 1>), dim3(gsz), dim3(s->block), 0,
+                           s->stream, c, s->lpw);
+        break;
+      default:
+        hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, 2>), dim3(gsz), dim3(s->block), 0,
+                           s->stream, c, s->lpw);
+    }
+    if (err == hipSuccess) err = hipGetLastError();
+    switch (s->opt.prune) {
+      case 0:
+        hipLaunchKernelGGL((dev::k_merge<c_label, 0>), dim3(gsz), dim3(s->block), 0, s->stream, c);
+        break;
+      case 1:
+        hipLaunchKernelGGL((dev::k_merge<c_label, 1>), dim3(gsz), dim3(s->block), 0, s->stream, c);
+        break;
+      default:
+        hipLaunchKernelGGL((dev::k_merge<c_label, 2>), dim3(gsz), dim3(s->block), 0, s->stream, c);
+    }
+    if (err == hipSuccess) err = hipGetLastError();
+    if (s->opt.delta_policy != VINE_DELTA_INF || c.stats_on) {
+      switch (s->opt.prune) {
+        case 0:
+          hipLaunchKernelGGL((dev::k_pull_policy<0>), dim3((s->B + 3) / 4), dim3(256), 0, s->stream, c, s->nb_active);
+          break;
+        case 1:
+          hipLaunchKernelGGL((dev::k_pull_policy<1>), dim3((s->B + 3) / 4), dim3(256), 0, s->stream, c, s->nb_active);
+          break;
+        default:
+          hipLaunchKernelGGL((dev::k_pull_policy<2>), dim3((s->B + 3) / 4), dim3(256), 0, s->stream, c, s->nb_active);
+      }
+      if (err == hipSuccess) err = hipGetLastError();
+    }
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s->stream, &g);
+  if (err == hipSuccess) err = ec;
+  if (err == hipSuccess) err = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+  s->graph = g;
+  if (err != hipSuccess) {
+    set_error("capturing %d rounds into a HIP graph: %s", K, hipGetErrorString(err));
+    (void)hipGetLastError();
+    return VINE_E_HIP;
+  }
+  s->graph_nb = s->nb_active;
+  s->graph_k = K;
+  s->graph_gsz = gsz;
+  return VINE_OK;
+}
+
+vine_status launch_rounds_graph(vine_solver* s, int K, vine_stats* st) {
+  int kg = K;
+  if (s->opt.time_kernels) {
+    vine_status ls = launch_round_any(s, s->grid, st, 1);
+    if (ls != VINE_OK) return ls;
+    kg = K - 1;
+  }
+  if (kg == 0) return VINE_OK;
+  if (!s->gexec || s->graph_nb != s->nb_active || s->graph_k != kg || s->graph_gsz != s->grid) {
+    vine_status cs = capture_rounds(s, kg);
+    if (cs != VINE_OK) return cs;
+  }
+  HIP_TRY(hipGraphLaunch(s->gexec, s->stream));
+  st->expand_launches += kg;
+  st->merge_launches += kg;
+  return VINE_OK;
 }
 
 vine_status fill_f32(vine_solver* s, float* p, float v, long long n) {
@@ -854,15 +973,24 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
       rule.reset(mx);
     }
     const int K = ar ? 1 : std::max(1, s->opt.rounds_per_sync);
+    // the AR rule sets the window from the host after every round, so it
+    // launches directly; the other policies may replay a captured chunk
+    const bool graph = s->opt.use_graph && !ar;
     long long fn_last = nb;
     unsigned long long exp_prev = 0;
     for (;;) {
-      for (int k = 0; k < K; k++) {
-        // the frontier may grow between polls: size the grid for 4x growth
-        // (grid-stride loops keep any size correct)
-        vine_status ls = launch_round_any(s, grid_for(s, fn_last * 4), st);
+      if (graph) {
+        vine_status ls = launch_rounds_graph(s, K, st);
         if (ls != VINE_OK) return ls;
-        rounds++;
+        rounds += K;
+      } else {
+        for (int k = 0; k < K; k++) {
+          // the frontier may grow between polls: size the grid for 4x growth
+          // (grid-stride loops keep any size correct)
+          vine_status ls = launch_round_any(s, grid_for(s, fn_last * 4), st);
+          if (ls != VINE_OK) return ls;
+          rounds++;
+        }
       }
       vine_status ps = poll(s);
       if (ps != VINE_OK) return ps;

@@ -34,6 +34,12 @@ namespace vine {
 namespace dev {
 
 constexpr int WAVE = 64;
+// Threads per workgroup of the round kernels (k_expand, k_merge, k_pull_policy)
+// at most: their per-wave LDS tables have MAX_BLOCK / WAVE rows, and the host
+// refuses a larger block_size before any launch.
+constexpr int MAX_BLOCK = 256;
+constexpr int MAX_WAVES = MAX_BLOCK / WAVE;
+static_assert(MAX_BLOCK % WAVE == 0 && MAX_WAVES >= 1, "round kernels run whole waves");
 constexpr float FINF = __builtin_huge_valf();
 constexpr float EXACT_LIMIT = 16777216.0f;  // 2^24
 constexpr int INIT_CAP = 4;                 // first Pareto-set allocation (entries)
@@ -206,10 +212,11 @@ __device__ __forceinline__ long long wave_alloc64_n(unsigned long long* ctr, int
 // Workgroup sums of per-thread counts, one atomic per counter and workgroup
 // (the statistics counters are shared by every workgroup of the grid, so
 // per-wave atomics on them serialise).  Every thread must call it: it
-// synchronises the workgroup.  Blocks have at most 256 threads.
+// synchronises the workgroup.  Blocks have at most MAX_BLOCK threads.
 template <int N>
 __device__ __forceinline__ void block_add(unsigned long long* const (&ctr)[N], const long long (&v)[N]) {
-  __shared__ long long red[N][4];
+  static_assert(N <= MAX_BLOCK, "one thread per counter");
+  __shared__ long long red[N][MAX_WAVES];
   const int lane = (int)__lane_id();
   const int wid = threadIdx.x / 64;
 #pragma unroll
@@ -771,7 +778,7 @@ __device__ __forceinline__ void far_append(const Ctx& c, Counters* ct, int q, in
 // reference's DOP count (expand the last DOP entries, carry the rest,
 // algorithm/cspp.cuh:179-216) in key units.
 template <typename Label_t, typename Constraint_t, typename Edge_t, int PRUNE>
-__global__ void __launch_bounds__(256) k_expand(Ctx c, int lpw) {
+__global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
   using LT = label_traits<Label_t>;
   Counters* ct = c.ct;
   const int r = ct->round_e;
@@ -791,9 +798,9 @@ __global__ void __launch_bounds__(256) k_expand(Ctx c, int lpw) {
   }
   const int lane = lane_id();
   // per-wave tables grouping a step's candidates by (query, vertex)
-  __shared__ int s_lead[4][GROUP_SLOTS];
-  __shared__ unsigned long long s_mask[4][GROUP_SLOTS];
-  for (int j = threadIdx.x; j < 4 * GROUP_SLOTS; j += blockDim.x) (&s_mask[0][0])[j] = 0ull;
+  __shared__ int s_lead[MAX_WAVES][GROUP_SLOTS];
+  __shared__ unsigned long long s_mask[MAX_WAVES][GROUP_SLOTS];
+  for (int j = threadIdx.x; j < MAX_WAVES * GROUP_SLOTS; j += blockDim.x) (&s_mask[0][0])[j] = 0ull;
   __syncthreads();
   const int wid = threadIdx.x / WAVE;
   const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
@@ -1026,7 +1033,7 @@ constexpr int MG = 16;  // lanes per (query, vertex) group
 
 
 template <typename Label_t, int PRUNE>
-__global__ void __launch_bounds__(256) k_merge(Ctx c) {
+__global__ void __launch_bounds__(MAX_BLOCK) k_merge(Ctx c) {
   using LT = label_traits<Label_t>;
   Counters* ct = c.ct;
   const int r = ct->round_m;
@@ -1494,7 +1501,7 @@ __device__ void policy_query(const Ctx& c, int q, int nq) {
 // After merge: per query (one wave each) the far-tier pull, then the policy
 // step when statistics are on (learned window or trace).
 template <int PRUNE>
-__global__ void __launch_bounds__(256) k_pull_policy(Ctx c, int nq) {
+__global__ void __launch_bounds__(MAX_BLOCK) k_pull_policy(Ctx c, int nq) {
   const long long q = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
   if (q >= c.B) return;  // whole waves exit together
   if (c.policy != 0) pull_query<PRUNE>(c, q);
