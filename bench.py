@@ -9,8 +9,17 @@ solvers on separate HIP streams and the launch shape that the autotune cache
 (vine_amd/models/autotune.json) holds for this graph and batch.  Inputs are in HBM before
 the timed region; answers stay in HBM.  Beside it: the plain single-query
 solver (one query per solve, fixed window) on the random-pair sweeps (bound
-1.1x and 1.5x), the roofline of the dominant kernel from HIP-event durations,
-and the CPU oracle timed on a bounded sample.
+1.1x and 1.5x) and on a sample of the batched workload, every batched answer
+compared with the oracle's (tests/golden/answers_ny_batch.npz), the host-buffer
+entry point with paths, the roofline of the dominant kernel from HIP-event
+durations, and the CPU oracle timed on bounded samples.
+
+Roofline bytes are the algorithmic bytes of the constrained search: 20 B per
+label expanded (12-B label + 8-B row_ptr pair), 12 B per arc relaxed (one
+{dst, len, delay} record) and 24 B per label inserted (12 B into the Pareto
+store + 12 B into the next frontier), counted by the solver itself.  Carried
+frontier entries that are read again, dominance-list reads and the bound
+searches are overhead, reported beside them.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 Prints one JSON line.
@@ -32,6 +41,100 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def cpu_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(args, g, qb, fx, cost_b, dly_b, feas_b):
+    """The oracle timed on the host: one core (queries in sequence) and all
+    threads (one query per thread).  Rows: the grid workload's 100 queries with
+    the reference's own search (restated cspp_cpu, no pruning); a sample of the
+    batched workload with the restated search plus its lower-bound pruning,
+    drawn from the queries that search finishes (its longest queries take
+    minutes to hours: whole-frontier rounds re-expand labels many times over);
+    the label-setting oracle on a uniform sample of the batched workload."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle as O
+    from vine_amd import workload as W
+
+    model, nproc = cpu_info()
+    threads = max(1, min(16, nproc or 1))
+
+    def timed(fn, idx, nthreads):
+        t1 = time.perf_counter()
+        if nthreads == 1:
+            out = [fn(i) for i in idx]
+        else:
+            with ThreadPoolExecutor(nthreads) as ex:
+                out = list(ex.map(fn, idx))
+        return time.perf_counter() - t1, out
+
+    res = {"cpu_model": model, "nproc": nproc, "threads": threads}
+    # grid workload, reference search
+    gg, qg = W.preset("grid64")
+    one = lambda i: O.cspp(gg, qg.src[i], qg.dst[i], qg.bound[i])
+    t_1, _ = timed(one, range(len(qg)), 1)
+    t_n, _ = timed(one, range(len(qg)), threads)
+    res["grid64_reference_search"] = {"queries": len(qg), "qps_1core": len(qg) / t_1, "qps_all": len(qg) / t_n}
+    log("cpu grid64: %.1f / %.1f queries/s (1 core / %d threads)" % (len(qg) / t_1, len(qg) / t_n, threads))
+
+    rng = np.random.default_rng(1)
+    pool = np.arange(len(qb))
+    if fx is not None and int((fx["restated"] == 1).sum()) >= args.cpu_queries:
+        pool = np.nonzero(fx["restated"] == 1)[0]
+    idx = rng.choice(pool, min(args.cpu_queries, len(pool)), replace=False)
+
+    def restated(i):
+        t1 = time.perf_counter()
+        try:
+            r = O.cspp(g, qb.src[i], qb.dst[i], qb.bound[i], prune_target=2, time_limit=args.cpu_limit)
+        except O.TimeLimit:
+            log("cpu: query %d stopped at the %.0f s cap" % (i, args.cpu_limit))
+            return None
+        log("cpu: query %d done in %.1f s" % (i, time.perf_counter() - t1))
+        return (r.feasible == bool(feas_b[i])) and (not r.feasible or (
+            np.float32(r.cost) == cost_b[i] and np.float32(r.delay) == dly_b[i]))
+
+    t_n, out = timed(restated, idx, threads)
+    done = [o for o in out if o is not None]
+    sub = idx[: max(1, len(idx) // 4)]
+    t_1, out1 = timed(restated, sub, 1)
+    value = len(idx) / t_n
+    res.update({"value": value, "unit": "queries/s", "cores": threads, "kind": "port",
+                "single_core_value": len(sub) / t_1,
+                "sample": "%d queries of the batched workload drawn (seed 1) from the %d of 4,096 that the restated "
+                          "cspp_cpu search with lower-bound pruning finishes within 20 s each; one query per thread "
+                          "on %d threads (single_core_value: %d of them in sequence on one core); %d/%d finished "
+                          "under the %.0f s cap, answers agree with the GPU: %s.  The other queries take minutes "
+                          "to hours each with this search, so the rate is an upper bound for the workload"
+                          % (len(idx), len(pool), threads, len(sub), len(done), len(idx), args.cpu_limit,
+                             all(done) and len(done) == len(idx))})
+    log("cpu batched sample: %.2f queries/s on %d threads, %.2f on one core" % (value, threads, len(sub) / t_1))
+
+    # label-setting oracle: every query finishes, uniform sample
+    uidx = rng.choice(len(qb), min(4 * args.cpu_queries, len(qb)), replace=False)
+    ls = lambda i: O.cspp_astar(g, qb.src[i], qb.dst[i], qb.bound[i])
+    t_n, outl = timed(ls, uidx, threads)
+    agree = all((r.feasible == bool(feas_b[i])) and (not r.feasible or (
+        np.float32(r.cost) == cost_b[i] and np.float32(r.delay) == dly_b[i])) for r, i in zip(outl, uidx))
+    res["label_setting"] = {"queries": len(uidx), "qps_all": len(uidx) / t_n, "threads": threads,
+                            "agree_with_gpu": agree,
+                            "note": "uniform sample of the batched workload; lexicographic bi-criteria A* with "
+                                    "exact reverse-Dijkstra bounds (two full Dijkstras per query dominate)"}
+    log("cpu label-setting: %.2f queries/s on %d threads" % (len(uidx) / t_n, threads))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -49,8 +152,8 @@ def main():
     ap.add_argument("--no-tuned", action="store_true", help="ignore the autotune cache (library launch defaults)")
     ap.add_argument("--plain-delta", type=float, default=12.0, help="window of the plain single-query solver")
     ap.add_argument("--plain-queries", type=int, default=100, help="per bound factor")
-    ap.add_argument("--cpu-queries", type=int, default=32)
-    ap.add_argument("--cpu-limit", type=float, default=12.0)
+    ap.add_argument("--cpu-queries", type=int, default=32, help="batched-workload queries in the CPU sample")
+    ap.add_argument("--cpu-limit", type=float, default=60.0, help="CPU seconds per query (a safety cap)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
@@ -129,23 +232,35 @@ def main():
     lat = solver.latency_us(nq)
     log("batched: %.1f queries/s, %.1f ms/step, feasible %d/%d" % (qps, ms_step, int(feas_b.sum()), nq))
 
-    # roofline of the dominant kernel (algorithmic bytes / HIP-event duration)
+    # answers against the oracle's, all queries (fixture written offline by
+    # tests/golden/make_full_answers.py from the same seeded workload)
+    exact = None
+    fx_path = os.path.join(ROOT, "tests", "golden", "answers_ny_batch.npz")
+    fx = None
+    if os.path.exists(fx_path):
+        fx = np.load(fx_path)
+        same_inputs = (np.array_equal(fx["src"], qb.src) and np.array_equal(fx["dst"], qb.dst)
+                       and np.array_equal(fx["bound"], qb.bound))
+        if same_inputs:
+            ok = (fx["feasible"].astype(bool) == feas_b.astype(bool)) & (
+                ~fx["feasible"].astype(bool) | ((fx["cost"] == cost_b) & (fx["delay"] == dly_b)))
+            exact = {"queries": nq, "agree": int(ok.sum()), "fixture": os.path.relpath(fx_path, ROOT)}
+        else:
+            exact = {"queries": nq, "agree": None, "fixture": "inputs differ from the fixture's"}
+            fx = None
+        log("answers vs oracle: %s" % exact)
+
+    # roofline: algorithmic bytes of the constrained search (counted by the
+    # solver) over HIP-event durations measured on the solver streams
     agg = {k: sum(s[k] for s in stats) for k in stats[0]}
-    # algorithmic bytes per unit of work (each item read or written once):
-    #   expand: 16 B per scanned frontier entry, 24 B per expanded label
-    #           (label + meta), 16 B per relaxed arc, 16 B per candidate pushed
-    #   merge:  16 B per candidate read, 37 B per inserted label (label 16,
-    #           meta 8, front entry 8 + id 4, dead flag 1)
-    #   bounds: 16 B arc + 8 B distance per relaxation
-    #   pull:   8 B chunk header + 8 B drain segment per 64 far records
+    B_X, B_E, B_I = 20.0, 12.0, 24.0  # bytes per label expanded / arc relaxed / label inserted
     # (HIP-event ms, launches timed, launches, algorithmic bytes of all launches)
     tr = agg["timed_rounds"]
     kern = {
-        "expand": (agg["ms_expand"], tr, agg["expand_launches"],
-                   16.0 * agg["scanned"] + 24.0 * agg["expanded"] + 16.0 * agg["relaxed"] + 16.0 * agg["candidates"]),
-        "merge": (agg["ms_merge"], tr, agg["merge_launches"], 16.0 * agg["candidates"] + 37.0 * agg["inserted"]),
-        "bounds": (agg["ms_sssp"], agg["sssp_launches"], agg["sssp_launches"], 24.0 * agg["sssp_relaxed"]),
-        "pull_policy": (agg["ms_pull"], tr, agg["expand_launches"], 16.0 / 64.0 * agg["far_pulled"]),
+        "expand": (agg["ms_expand"], tr, agg["expand_launches"], B_X * agg["expanded"] + B_E * agg["relaxed"]),
+        "merge": (agg["ms_merge"], tr, agg["merge_launches"], B_I * agg["inserted"]),
+        "bounds": (agg["ms_sssp"], agg["sssp_launches"], agg["sssp_launches"], 0.0),
+        "pull_policy": (agg["ms_pull"], tr, agg["expand_launches"], 0.0),
     }
     # dominant: the largest estimated time per step (timed share scaled to all launches)
     est = {k: v[0] * v[2] / max(v[1], 1) for k, v in kern.items()}
@@ -163,11 +278,41 @@ def main():
             m = re.match(r"traffic: kernel=(\w+) bytes_per_launch=(\d+)", line.strip())
             if m and m.group(1) == dom:
                 traffic, traffic_src = float(m.group(2)), os.path.relpath(path, ROOT)
+    step_bytes = (B_X * agg["expanded"] + B_E * agg["relaxed"] + B_I * agg["inserted"]) / args.steps
+    overhead = {
+        # a carried entry read again costs its 16-B record; the bound searches
+        # read a 16-B arc and 8 B of distance per relaxation
+        "rescanned_bytes_per_step": 16.0 * max(agg["scanned"] - agg["expanded"], 0) / args.steps,
+        "scanned_per_expanded": agg["scanned"] / max(agg["expanded"], 1),
+        "bound_search_bytes_per_step": 24.0 * agg["sssp_relaxed"] / args.steps,
+    }
+    if traffic is not None:
+        overhead["traffic_over_algorithmic"] = traffic / max(per_launch, 1.0)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
+                "bytes_rule": "20 B x labels expanded + 12 B x arcs relaxed (+ 24 B x labels inserted for merge)",
                 "algorithmic_bytes_per_launch": per_launch, "avg_launch_us": avg_s * 1e6,
                 "launches_per_step": launches / args.steps, "launches_timed": timed,
-                "kernel_ms_per_step_est": {k: v / args.steps for k, v in est.items()}}
+                "kernel_ms_per_step_est": {k: v / args.steps for k, v in est.items()},
+                "step": {"algorithmic_bytes": step_bytes, "achieved": step_bytes / (ms_step / 1000.0) / 1e9,
+                         "frac": step_bytes / (ms_step / 1000.0) / HBM_PEAK},
+                "overhead": overhead}
+    if fx is not None:
+        # the least work any exact label search does on this workload: the
+        # label-setting oracle's counters (each label expanded at most once)
+        mb = float(B_X * fx["X"].sum() + B_E * fx["E"].sum() + B_I * fx["I"].sum())
+        roofline["oracle_minimal"] = {"algorithmic_bytes": mb, "achieved": mb / (ms_step / 1000.0) / 1e9,
+                                      "frac": mb / (ms_step / 1000.0) / HBM_PEAK,
+                                      "gpu_expanded_over_minimal": agg["expanded"] / args.steps / max(fx["X"].sum(), 1)}
+
+    # the host-buffer entry point: queries up and answers and every path down
+    # over PCIe (not the headline value)
+    t1 = time.perf_counter()
+    ans_h = solver.solve(qb.src, qb.dst, qb.bound, paths=True)
+    dt_h = time.perf_counter() - t1
+    same_h = bool(np.array_equal(ans_h.cost, cost_b) and np.array_equal(ans_h.feasible.astype(np.int32), feas_b))
+    host_leg = {"qps_with_paths": nq / dt_h, "path_vertices": int(ans_h.path_off[-1]), "same_answers": same_h}
+    log("host buffers + paths: %.1f queries/s (%d path vertices)" % (nq / dt_h, int(ans_h.path_off[-1])))
 
     # plain single-query solver: the drop-in, one query per solve, with the
     # fixed window (the reference's own AR rule is far slower still: it never
@@ -198,43 +343,10 @@ def main():
             log("plain on batched sample: %.1f queries/s, same answers %s" % (len(chk) / dt, same))
         plain = res
 
-    # CPU oracle on a bounded sample of the same workload
+    # CPU oracle on bounded samples (the GPU box's host cores)
     cpu = None
     if not args.no_cpu and args.cpu_queries > 0:
-        from concurrent.futures import ThreadPoolExecutor
-
-        from oracle import oracle as O
-
-        rng = np.random.default_rng(1)
-        idx = rng.choice(nq, min(args.cpu_queries, nq), replace=False)
-        threads = min(16, os.cpu_count() or 1)
-
-        def one(i):
-            t1 = time.perf_counter()
-            try:
-                r = O.cspp(g, qb.src[i], qb.dst[i], qb.bound[i], prune_target=2, time_limit=args.cpu_limit)
-                ok = (r.feasible == bool(feas_b[i])) and (not r.feasible or (
-                    np.float32(r.cost) == cost_b[i] and np.float32(r.delay) == dly_b[i]))
-                return time.perf_counter() - t1, True, ok
-            except O.TimeLimit:
-                return max(time.perf_counter() - t1, args.cpu_limit), False, True
-
-        t1 = time.perf_counter()
-        with ThreadPoolExecutor(threads) as ex:
-            out = list(ex.map(one, idx))
-        wall = time.perf_counter() - t1
-        done = sum(1 for o in out if o[1])
-        agree = all(o[2] for o in out)
-        busy = sum(o[0] for o in out)
-        cpu = {"value": len(idx) / wall, "unit": "queries/s", "cores": threads, "kind": "port",
-               "single_core_value": len(idx) / busy,
-               "sample": "%d random queries of the batched workload, oracle (restated cspp_cpu with the same "
-                         "lower-bound pruning, whole-frontier rounds) on %d threads; %d finished, %d stopped at "
-                         "%.0f s CPU each and counted at that time (so the CPU rate is an upper bound); "
-                         "finished answers agree with the GPU: %s; single_core_value = queries / summed "
-                         "per-query time (one core, sequential)"
-                         % (len(idx), threads, done, len(idx) - done, args.cpu_limit, agree)}
-        log("cpu: %.2f queries/s (%d/%d finished)" % (cpu["value"], done, len(idx)))
+        cpu = cpu_baseline(args, g, qb, fx, cost_b, dly_b, feas_b)
 
     line = {
         "metric": "exact CSP queries/sec, NY-scale road graph: batched+learned vs plain GPU",
@@ -261,8 +373,11 @@ def main():
                                                            "candidates", "cand_chunks", "inserted", "sssp_relaxed",
                                                            "far_pulled")},
         "roofline": roofline,
+        "exact_vs_oracle": exact,
+        "host_buffers": host_leg,
         "plain": plain,
         "vs_plain": (qps / plain["1.1x"]["qps"]) if plain else None,
+        "vs_plain_same_workload": (qps / plain["batched_workload_sample"]["qps"]) if plain else None,
         "cpu_baseline": cpu,
     }
     solver.close()

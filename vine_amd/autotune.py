@@ -12,7 +12,9 @@ later run loads the choice instead of searching again:
 Searched, one coordinate at a time from the library defaults: the number of
 internal solvers on separate HIP streams (concurrency), frontier entries a
 wave takes per expand step (labels_per_wave), threads per workgroup of the
-round kernels (block_size) and workgroups per round kernel (grid_blocks).  Every configuration must return the same answers as the
+round kernels (block_size), workgroups per round kernel (grid_blocks) and
+whether the rounds between host checks replay as one captured HIP graph
+(use_graph).  Every configuration must return the same answers as the
 first one (they only change the launch shape); a difference raises.
 
     python -m vine_amd.autotune ny_batch        # tune the bench workload, update the cache
@@ -32,8 +34,8 @@ from . import workload as W
 
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models", "autotune.json")
 SPACE = (("concurrency", (1, 2, 3)), ("labels_per_wave", (16, 32, 64)), ("block_size", (64, 128, 256)),
-         ("grid_blocks", (256, 384, 768)))
-LIBRARY_DEFAULTS = {"concurrency": 1, "labels_per_wave": 64, "block_size": 128, "grid_blocks": 384}
+         ("grid_blocks", (256, 384, 768)), ("use_graph", (0, 1)))
+LIBRARY_DEFAULTS = {"concurrency": 1, "labels_per_wave": 64, "block_size": 128, "grid_blocks": 384, "use_graph": 0}
 
 
 def graph_key(g, max_batch, policy):
