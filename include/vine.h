@@ -115,7 +115,9 @@ typedef struct {
                                gives query i to part i % k and runs the parts from k host threads,
                                so one part's long tail overlaps the others' rounds.  With k > 1,
                                vine_solver_pareto's q is the query index (nq <= max_batch) */
-  int32_t reserved[1];
+  float near_windows;       /* the near tier (carried labels scanned again every round) holds keys up
+                               to this many windows past the query's front; later keys wait in the far
+                               tier's key buckets until the front comes within that reach (0 = 1) */
 } vine_options;
 
 typedef struct {
@@ -169,6 +171,15 @@ vine_status vine_solve(vine_solver* s, int32_t nq, const int32_t* src, const int
                        const float* bound, float* out_cost, float* out_delay,
                        int32_t* out_feasible, int64_t* out_path_off, int32_t* out_path,
                        int64_t path_cap, vine_stats* out_stats);
+
+/* Paths of the last solve again, without solving: for a caller whose path
+ * buffer was too small (vine_solve returned VINE_E_OVERFLOW with the needed
+ * size in out_path_off[nq]).  The last solve must have had exactly these nq
+ * queries in one batch (nq <= max_batch, so its label log is still resident);
+ * otherwise VINE_E_ARG.  Same layout and overflow rule as vine_solve.  No
+ * counterpart in the reference, which never extracts a path. */
+vine_status vine_solver_paths(vine_solver* s, int32_t nq, int64_t* out_path_off, int32_t* out_path,
+                              int64_t path_cap);
 
 /* Device buffers (HBM), no paths.  Costs/delays are +inf for infeasible queries. */
 vine_status vine_solve_device(vine_solver* s, int32_t nq, const int32_t* d_src, const int32_t* d_dst,

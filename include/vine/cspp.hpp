@@ -119,13 +119,16 @@ class cspp {
     delay_l.assign(ntask, 0.f);
     std::vector<int32_t> feas(ntask, 0);
     std::vector<int64_t> off(ntask + 1, 0);
-    std::vector<int32_t> path((size_t)ntask * 256 + 1);  // retried at the reported size if short
+    std::vector<int32_t> path((size_t)ntask * 256 + 1);  // grown to the reported size if short
     st = vine_solve(sv, ntask, src.data(), snk.data(), bound.data(), cost_l.data(), delay_l.data(), feas.data(),
                     off.data(), path.data(), (int64_t)path.size(), &stats);
     if (st == VINE_E_OVERFLOW && off[ntask] > (int64_t)path.size()) {
       path.resize(off[ntask]);
-      st = vine_solve(sv, ntask, src.data(), snk.data(), bound.data(), cost_l.data(), delay_l.data(),
-                      feas.data(), off.data(), path.data(), (int64_t)path.size(), &stats);
+      if (ntask <= options.max_batch)  // one batch: its paths are still on the device
+        st = vine_solver_paths(sv, ntask, off.data(), path.data(), (int64_t)path.size());
+      else
+        st = vine_solve(sv, ntask, src.data(), snk.data(), bound.data(), cost_l.data(), delay_l.data(),
+                        feas.data(), off.data(), path.data(), (int64_t)path.size(), &stats);
     }
     if (st == VINE_OK) {
       feasible_l.assign(feas.begin(), feas.end());

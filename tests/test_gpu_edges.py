@@ -73,3 +73,38 @@ def test_edges_family_solver(gpu):
         assert a.feasible.tolist() == [True, True, False]
         assert a.label[0].tolist() == [0, 0, 0, 0] and a.label[1].tolist() == [6, 9, 3, 0]
         assert a.path_of(1).tolist() == [0, 1, 2, 3]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("conc", [0, 3])
+def test_paths_without_resolving(gpu, conc):
+    """A path buffer too small: vine_solve returns VINE_E_OVERFLOW with the size
+    it needs, and vine_solver_paths reads the same batch's paths from the
+    device without solving again; a query count other than the last solve's
+    is refused."""
+    g, q = W.preset("grid64")
+    print("vine_solver_paths, concurrency %d" % conc, flush=True)
+    with S.Solver(g, max_batch=100, prune=S.PRUNE_BOUNDS, concurrency=conc) as s:
+        full = s.solve(q.src, q.dst, q.bound)
+        L = S._L()
+        nq = len(q)
+        cost = np.zeros(nq, np.float32)
+        delay = np.zeros(nq, np.float32)
+        feas = np.zeros(nq, np.int32)
+        off = np.zeros(nq + 1, np.int64)
+        small = np.zeros(8, np.int32)
+        rc = L.vine_solve(s.h, nq, q.src.ctypes.data, q.dst.ctypes.data, q.bound.ctypes.data, cost.ctypes.data,
+                          delay.ctypes.data, feas.ctypes.data, off.ctypes.data, small.ctypes.data, 8, None)
+        assert rc == S.VINE_E_OVERFLOW and off[-1] == full.path_off[-1]
+        assert np.array_equal(cost, full.cost)
+        # still too small: the offsets, then VINE_E_OVERFLOW again
+        off2 = np.zeros(nq + 1, np.int64)
+        assert L.vine_solver_paths(s.h, nq, off2.ctypes.data, small.ctypes.data, 8) == S.VINE_E_OVERFLOW
+        assert np.array_equal(off2, full.path_off)
+        path = np.zeros(int(off[-1]), np.int32)
+        assert L.vine_solver_paths(s.h, nq, off2.ctypes.data, path.ctypes.data, int(off[-1])) == S.VINE_OK
+        assert np.array_equal(path, full.path) and np.array_equal(off2, full.path_off)
+        assert L.vine_solver_paths(s.h, nq - 1, off2.ctypes.data, path.ctypes.data, int(off[-1])) == S.VINE_E_ARG
+        # the wrapper takes that route by itself when its first buffer is short
+        a = s.solve(q.src, q.dst, q.bound, path_cap=16)
+        assert np.array_equal(a.path, full.path) and np.array_equal(a.path_off, full.path_off)

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -63,6 +64,12 @@ struct vine_solver {
   int block = 256;
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
+  // no-allocation mode (vine_solve_device_static): every device buffer is cut
+  // from this arena instead of hipMalloc'd; count_only sizes a layout
+  char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  bool count_only = false;
+  Counters h_ct_local{};  // host copy of the counters when no pinned buffer is allocated
 
   // per query slot
   int32_t *d_qsrc = nullptr, *d_qdst = nullptr;
@@ -126,7 +133,6 @@ struct vine_solver {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_nb = -1, graph_k = 0, graph_gsz = 0;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // HIP-event brackets of timed launches, resolved at the end of each batch
   struct EvRec {
     hipEvent_t a, b;
@@ -144,6 +150,14 @@ struct vine_solver {
   std::vector<int32_t> q_rounds;
   int last_batch_q = 0;
   int nb_active = 0;  // queries in the current batch
+  // the last solve: its query count and whether it ran as one batch (then
+  // its paths are still in the label log, for vine_solver_paths)
+  int32_t last_nq = -1;
+  bool last_one_batch = false;
+  // path extraction buffers, grown on demand and kept (no allocation per solve)
+  long long* d_path_off = nullptr;
+  int32_t* d_path = nullptr;
+  long long path_off_cap = 0, path_cap = 0;
 
   // concurrency > 1: the parts do all the work, query i going to part
   // i % parts.size() as its (i / parts.size())-th query
@@ -160,6 +174,18 @@ vine_status create_split(const vine_graph* g, const vine_options& o, vine_solver
 
 static vine_status dev_alloc(vine_solver* s, void** p, size_t bytes) {
   if (bytes == 0) bytes = 16;
+  if (s->arena || s->count_only) {
+    // bump allocation, 256-byte aligned
+    const size_t at = (s->arena_used + 255) & ~(size_t)255;
+    if (!s->count_only && (at > s->arena_cap || bytes > s->arena_cap - at)) {
+      set_error("the static scratch of %zu bytes is too small for this configuration (needs more than %zu)",
+                s->arena_cap, at + bytes);
+      return VINE_E_ARG;
+    }
+    *p = s->count_only ? nullptr : s->arena + at;
+    s->arena_used = at + bytes;
+    return VINE_OK;
+  }
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
     set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
@@ -174,7 +200,7 @@ static vine_status dev_alloc(vine_solver* s, void** p, size_t bytes) {
   do {                                                                    \
     vine_status st_ = dev_alloc(s, (void**)&(ptr), (size_t)(bytes));       \
     if (st_ != VINE_OK) {                                                 \
-      vine_solver_destroy(s);                                             \
+      if (!s->arena) vine_solver_destroy(s);                              \
       return st_;                                                         \
     }                                                                     \
   } while (0)
@@ -335,25 +361,23 @@ vine_status vine_solver_destroy(vine_solver* s) {
   for (void* p : s->split_bufs) (void)hipFree(p);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (void* p : s->allocs) (void)hipFree(p);
+  if (s->d_path_off) (void)hipFree(s->d_path_off);
+  if (s->d_path) (void)hipFree(s->d_path);
   if (s->h_ct) (void)hipHostFree(s->h_ct);
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
-  for (auto& e : s->ev)
-    if (e) (void)hipEventDestroy(e);
   for (auto& e : s->evpool) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return VINE_OK;
 }
 
-vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vine_solver** out) {
-  if (!g || !out) {
-    set_error("vine_solver_create: null argument");
-    return VINE_E_ARG;
-  }
-  vine_options o;
-  if (opt) o = *opt;
-  else vine_options_default(&o);
+}  // extern "C"
+
+namespace {
+// Option checks shared by vine_solver_create and vine_solve_device_static;
+// may switch the pruning mode (see below).
+vine_status validate_options(const vine_graph* g, vine_options& o) {
   if (o.max_batch < 1 || o.prune < 0 || o.prune > 2 || o.delta_policy < 0 || o.delta_policy > 3 ||
       o.rounds_per_sync < 1 || o.grid_blocks < 0 || o.hash_slots < 0 || !(o.delta_fixed >= 0.f) ||
       o.bound_ball < 0 || o.bound_ball > 1) {
@@ -399,10 +423,30 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
     set_error("vine_solver_create: hash_slots is reserved (fronts live in the HBM pool) and must be 0");
     return VINE_E_ARG;
   }
+  if (!(o.near_windows >= 0.f) || o.near_windows > 64.f) {
+    set_error("vine_solver_create: near_windows must lie in [0, 64]");
+    return VINE_E_ARG;
+  }
   if (o.concurrency < 0 || o.concurrency > 16) {
     set_error("vine_solver_create: concurrency must lie in [0, 16]");
     return VINE_E_ARG;
   }
+  return VINE_OK;
+}
+}  // namespace
+
+extern "C" {
+
+vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vine_solver** out) {
+  if (!g || !out) {
+    set_error("vine_solver_create: null argument");
+    return VINE_E_ARG;
+  }
+  vine_options o;
+  if (opt) o = *opt;
+  else vine_options_default(&o);
+  vine_status v = validate_options(g, o);
+  if (v != VINE_OK) return v;
   if (o.concurrency > 1) return create_split(g, o, out);
   return create_single(g, o, 0.6, o.max_batch, out);
 }
@@ -411,11 +455,8 @@ vine_status vine_solver_create(const vine_graph* g, const vine_options* opt, vin
 
 namespace {
 
-// One solver on one stream.  Automatic capacities are sized for want_slots
-// query slots, capped by mem_frac of the free memory.
-vine_status create_single(const vine_graph* g, const vine_options& o, double mem_frac, long long want_slots,
-                          vine_solver** out) {
-  auto* s = new vine_solver();
+// Fields of a solver taken from its options (no device work).
+void init_fields(vine_solver* s, const vine_graph* g, const vine_options& o) {
   s->g = g;
   s->opt = o;
   s->B = o.max_batch;
@@ -428,62 +469,44 @@ vine_status create_single(const vine_graph* g, const vine_options& o, double mem
                      g->max_deg <= dev::ARENA / 2;
   s->wg = wg_ok && o.engine == VINE_ENGINE_WORKGROUP;
   s->lpw = o.labels_per_wave ? o.labels_per_wave : 64;
-  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) {
-    set_error("hipStreamCreate: %s", hipGetErrorString(e));
-    delete s;
-    return VINE_E_HIP;
-  }
-  for (auto& ev : s->ev)
-    if (hipEventCreate(&ev) != hipSuccess) {
-      set_error("hipEventCreate failed");
-      vine_solver_destroy(s);
-      return VINE_E_HIP;
-    }
-  const long long N = g->n, B = s->B, BN = N * B;
-  size_t freeb = 0, totalb = 0;
-  if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
-    set_error("hipMemGetInfo failed");
-    vine_solver_destroy(s);
-    return VINE_E_HIP;
-  }
-  // per-(query, vertex) state: pset, cand_head; with the bounds lbl, lbd, inq
-  // and the two bound-search queue entries
-  size_t per_qv = sizeof(int2) + sizeof(int32_t) +
-                  (o.prune == 2 ? sizeof(float) + sizeof(unsigned long long) + sizeof(uint32_t) + 2 * sizeof(int32_t)
-                                : 0);
-  double remaining = (double)freeb - (double)per_qv * BN - 64.0 * (1 << 20);
-  if (remaining <= 0) {
-    set_error("vine_solver_create: %lld query slots x %lld vertices need %.1f GB, %.1f GB free",
-            B, N, (double)per_qv * BN / 1e9, (double)freeb / 1e9);
-    vine_solver_destroy(s);
-    return VINE_E_OOM;
-  }
-  // auto capacities: share the remaining memory (labels 25 B, pool 12 B x 2,
-  // candidate chunks 144 B per two candidate slots, carry 32 B, bound frontier 16 B)
-  long long lab_cap = o.label_capacity;
-  if (!lab_cap) {
-    // sized to the problem (64 labels per (query, vertex) pair plus slack),
-    // capped by mem_frac of the free memory
-    double per = 25.0 + 2 * 12.0 + 0.25 * 144.0 + 0.5 * 32.0 + 0.25 * 16.0;
-    double want = 64.0 * (double)(N * want_slots) + (double)(1 << 22);
-    lab_cap = (long long)std::min(std::min(remaining * mem_frac / per, want), (double)INT32_MAX - 1);
-    lab_cap = std::max<long long>(lab_cap, 4 * B + 1024);
-  }
+}
+
+// Per-(query, vertex) state: pset, cand_head; with the bounds lbl, lbd, inq
+// and the two bound-search queue entries.
+size_t per_qv_bytes(const vine_options& o) {
+  return sizeof(int2) + sizeof(int32_t) +
+         (o.prune == 2 ? sizeof(float) + sizeof(unsigned long long) + sizeof(uint32_t) + 2 * sizeof(int32_t) : 0);
+}
+
+struct Caps {
+  long long lab = 0, pool = 0, cand = 0, carry = 0;
+};
+
+// The other capacities follow the label capacity unless the options set them.
+vine_status derive_caps(const vine_options& o, long long B, long long lab_cap, Caps* k) {
   if (lab_cap > INT32_MAX - 1 || lab_cap < B + 1) {
     set_error("vine_solver_create: label_capacity must be in [max_batch+1, 2^31-1)");
-    vine_solver_destroy(s);
     return VINE_E_ARG;
   }
-  long long pool_cap = o.pool_capacity ? o.pool_capacity : std::min<long long>(2 * lab_cap + 4 * B, 0xFFFFFFFFll);
-  long long cand_cap = o.cand_capacity ? o.cand_capacity : std::min<long long>(lab_cap / 2 + 1024, INT32_MAX - 1);
-  long long carry_cap = o.frontier_capacity ? o.frontier_capacity : std::min<long long>(lab_cap / 2 + 1024, INT32_MAX - 1);
-  if (pool_cap < 4 * B || pool_cap > 0xFFFFFFFFll || cand_cap < 64 || cand_cap > INT32_MAX - 1 ||
-      carry_cap < 64 || carry_cap > INT32_MAX - 1) {
+  k->lab = lab_cap;
+  k->pool = o.pool_capacity ? o.pool_capacity : std::min<long long>(2 * lab_cap + 4 * B, 0xFFFFFFFFll);
+  k->cand = o.cand_capacity ? o.cand_capacity : std::min<long long>(lab_cap / 2 + 1024, INT32_MAX - 1);
+  k->carry = o.frontier_capacity ? o.frontier_capacity : std::min<long long>(lab_cap / 2 + 1024, INT32_MAX - 1);
+  if (k->pool < 4 * B || k->pool > 0xFFFFFFFFll || k->cand < 64 || k->cand > INT32_MAX - 1 || k->carry < 64 ||
+      k->carry > INT32_MAX - 1) {
     set_error("vine_solver_create: capacities out of range");
-    vine_solver_destroy(s);
     return VINE_E_ARG;
   }
+  return VINE_OK;
+}
+
+// Every device buffer of one solver, through dev_alloc (hipMalloc, or the
+// arena of the no-allocation mode), then the kernels' context.
+vine_status alloc_buffers(vine_solver* s, const Caps& k) {
+  const vine_graph* g = s->g;
+  const vine_options& o = s->opt;
+  const long long N = g->n, B = s->B, BN = N * B;
+  const long long lab_cap = k.lab, pool_cap = k.pool, cand_cap = k.cand, carry_cap = k.carry;
   ALLOC(s->d_qsrc, sizeof(int32_t) * B);
   ALLOC(s->d_qdst, sizeof(int32_t) * B);
   ALLOC(s->d_qbound, sizeof(float) * B);
@@ -547,7 +570,9 @@ vine_status create_single(const vine_graph* g, const vine_options& o, double mem
   ALLOC(s->d_frec, sizeof(int4) * dev::FU * far_cap);
   ALLOC(s->d_ts, sizeof(unsigned long long) * dev::TS_CAP);
   ALLOC(s->d_ct, sizeof(Counters));
-  if (hipHostMalloc((void**)&s->h_ct, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
+  if (s->arena || s->count_only) {
+    s->h_ct = &s->h_ct_local;
+  } else if (hipHostMalloc((void**)&s->h_ct, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
     set_error("hipHostMalloc failed");
     vine_solver_destroy(s);
     return VINE_E_OOM;
@@ -614,9 +639,59 @@ vine_status create_single(const vine_graph* g, const vine_options& o, double mem
   c.frec = s->d_frec;
   c.far_cap = (int32_t)far_cap;
   c.far_floor = 0.0625f * g->mean_len;
+  c.far_mult = o.near_windows > 0.f ? o.near_windows : 1.0f;
   c.ts = s->d_ts;
   c.ct = s->d_ct;
   c.ellipse = o.bound_ball ? 0 : 1;
+  return VINE_OK;
+}
+
+// One solver on one stream.  Automatic capacities are sized for want_slots
+// query slots, capped by mem_frac of the free memory.
+vine_status create_single(const vine_graph* g, const vine_options& o, double mem_frac, long long want_slots,
+                          vine_solver** out) {
+  auto* s = new vine_solver();
+  init_fields(s, g, o);
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    set_error("hipStreamCreate: %s", hipGetErrorString(e));
+    delete s;
+    return VINE_E_HIP;
+  }
+  const long long N = g->n, B = s->B, BN = N * B;
+  size_t freeb = 0, totalb = 0;
+  if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
+    set_error("hipMemGetInfo failed");
+    vine_solver_destroy(s);
+    return VINE_E_HIP;
+  }
+  const size_t per_qv = per_qv_bytes(o);
+  double remaining = (double)freeb - (double)per_qv * BN - 64.0 * (1 << 20);
+  if (remaining <= 0) {
+    set_error("vine_solver_create: %lld query slots x %lld vertices need %.1f GB, %.1f GB free",
+            B, N, (double)per_qv * BN / 1e9, (double)freeb / 1e9);
+    vine_solver_destroy(s);
+    return VINE_E_OOM;
+  }
+  // auto capacities: share the remaining memory (labels 25 B, pool 12 B x 2,
+  // candidate chunks 144 B per two candidate slots, carry 32 B, bound frontier 16 B)
+  long long lab_cap = o.label_capacity;
+  if (!lab_cap) {
+    // sized to the problem (64 labels per (query, vertex) pair plus slack),
+    // capped by mem_frac of the free memory
+    double per = 25.0 + 2 * 12.0 + 0.25 * 144.0 + 0.5 * 32.0 + 0.25 * 16.0;
+    double want = 64.0 * (double)(N * want_slots) + (double)(1 << 22);
+    lab_cap = (long long)std::min(std::min(remaining * mem_frac / per, want), (double)INT32_MAX - 1);
+    lab_cap = std::max<long long>(lab_cap, 4 * B + 1024);
+  }
+  Caps k;
+  vine_status r = derive_caps(o, B, lab_cap, &k);
+  if (r != VINE_OK) {
+    vine_solver_destroy(s);
+    return r;
+  }
+  r = alloc_buffers(s, k);
+  if (r != VINE_OK) return r;  // alloc_buffers destroyed s
   *out = s;
   return VINE_OK;
 }
@@ -1171,6 +1246,10 @@ vine_status solve_impl(vine_solver* s, int32_t nq, const int32_t* d_src, const i
 
 vine_status ensure_io(vine_solver* s, int32_t nq) {
   if (nq <= s->in_cap) return VINE_OK;
+  if (s->arena && s->in_cap > 0) {
+    set_error("the static scratch holds the query staging buffers of one call only");
+    return VINE_E_ARG;
+  }
   for (void* p : {(void*)s->d_in_q, (void*)s->d_in_bound, (void*)s->d_out_cost, (void*)s->d_out_delay,
                   (void*)s->d_out_feas, (void*)s->d_out_plen}) {
     if (!p) continue;
@@ -1429,6 +1508,47 @@ vine_status solve_split_host(vine_solver* s, int32_t nq, const int32_t* src, con
   return VINE_OK;
 }
 
+// Paths of the nb queries of the batch whose label log is resident: lengths
+// plen[0..nb) (host), written to out_path in the caller's vertex ids.  The
+// device buffers grow on demand and stay with the handle.
+vine_status extract_paths(vine_solver* s, int nb, const int32_t* plen, int32_t* out_path) {
+  long long need = 0;
+  for (int i = 0; i < nb; i++) need += plen[i];
+  if (need == 0) return VINE_OK;
+  if (s->path_off_cap < nb + 1) {
+    if (s->d_path_off) (void)hipFree(s->d_path_off);
+    s->d_path_off = nullptr;
+    s->path_off_cap = 0;
+    hipError_t e = hipMalloc(&s->d_path_off, sizeof(long long) * (nb + 1));
+    if (e != hipSuccess) {
+      set_error("hipMalloc for path offsets: %s", hipGetErrorString(e));
+      return VINE_E_OOM;
+    }
+    s->path_off_cap = nb + 1;
+  }
+  if (s->path_cap < need) {
+    if (s->d_path) (void)hipFree(s->d_path);
+    s->d_path = nullptr;
+    s->path_cap = 0;
+    const long long cap = std::max<long long>(need, s->path_cap * 2);
+    hipError_t e = hipMalloc(&s->d_path, sizeof(int32_t) * cap);
+    if (e != hipSuccess) {
+      set_error("hipMalloc for paths: %s", hipGetErrorString(e));
+      return VINE_E_OOM;
+    }
+    s->path_cap = cap;
+  }
+  std::vector<long long> off(nb + 1, 0);
+  for (int i = 0; i < nb; i++) off[i + 1] = off[i] + plen[i];
+  HIP_TRY(hipMemcpyAsync(s->d_path_off, off.data(), sizeof(long long) * (nb + 1), hipMemcpyHostToDevice, s->stream));
+  hipLaunchKernelGGL(dev::k_paths, dim3((nb + 255) / 256), dim3(256), 0, s->stream, s->ctx, nb, s->d_path_off,
+                     s->d_path);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out_path, s->d_path, sizeof(int32_t) * need, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return VINE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1461,6 +1581,8 @@ vine_status vine_solve_device(vine_solver* s, int32_t nq, const int32_t* d_src, 
     }
   }
   vine_status r = solve_impl(s, nq, d_src, d_dst, d_bound, d_cost, d_delay, d_feasible, nullptr, &st, 0, nq);
+  s->last_nq = r == VINE_OK ? nq : -1;
+  s->last_one_batch = nq <= s->B;
   st.ms_total = now_ms() - t0;
   st.ms_kernels = st.ms_expand + st.ms_merge + st.ms_sssp + st.ms_pull;
   if (out_stats) *out_stats = st;
@@ -1497,7 +1619,6 @@ vine_status vine_solve(vine_solver* s, int32_t nq, const int32_t* src, const int
   // batch by batch and extract paths per batch
   std::vector<int32_t> plen(nq, 0);
   if (out_path_off) out_path_off[0] = 0;
-  int64_t written = 0;
   bool path_overflow = false;
   for (int32_t q0 = 0; q0 < nq; q0 += s->B) {
     int nb = std::min<int32_t>(s->B, nq - q0);
@@ -1509,41 +1630,16 @@ vine_status vine_solve(vine_solver* s, int32_t nq, const int32_t* src, const int
       for (int i = 0; i < nb; i++) out_path_off[q0 + i + 1] = out_path_off[q0 + i] + plen[q0 + i];
     }
     if (want_path) {
-      int64_t need = out_path_off[q0 + nb] - out_path_off[q0];
       if (out_path_off[q0 + nb] > path_cap) {
         path_overflow = true;
-      } else if (need > 0) {
-        long long* d_off = nullptr;
-        int32_t* d_path = nullptr;
-        HIP_TRY(hipMalloc(&d_off, sizeof(long long) * (nb + 1)));
-        hipError_t e = hipMalloc(&d_path, sizeof(int32_t) * need);
-        if (e != hipSuccess) {
-          (void)hipFree(d_off);
-          set_error("hipMalloc for paths: %s", hipGetErrorString(e));
-          return VINE_E_OOM;
-        }
-        std::vector<long long> off(nb + 1);
-        for (int i = 0; i <= nb; i++) off[i] = out_path_off[q0 + i] - out_path_off[q0];
-        e = hipMemcpy(d_off, off.data(), sizeof(long long) * (nb + 1), hipMemcpyHostToDevice);
-        if (e == hipSuccess) {
-          hipLaunchKernelGGL(dev::k_paths, dim3((nb + 255) / 256), dim3(256), 0, s->stream, s->ctx, nb, d_off,
-                             d_path);
-          e = hipGetLastError();
-        }
-        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-        if (e == hipSuccess)
-          e = hipMemcpy(out_path + out_path_off[q0], d_path, sizeof(int32_t) * need, hipMemcpyDeviceToHost);
-        (void)hipFree(d_off);
-        (void)hipFree(d_path);
-        if (e != hipSuccess) {
-          set_error("path extraction: %s", hipGetErrorString(e));
-          return VINE_E_HIP;
-        }
+      } else {
+        v = extract_paths(s, nb, plen.data() + q0, out_path + out_path_off[q0]);
+        if (v != VINE_OK) return v;
       }
-      written = out_path_off[q0 + nb];
     }
   }
-  (void)written;
+  s->last_nq = nq;
+  s->last_one_batch = nq <= s->B;
   HIP_TRY(hipMemcpy(out_cost, s->d_out_cost, sizeof(float) * nq, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(out_delay, s->d_out_delay, sizeof(float) * nq, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(out_feasible, s->d_out_feas, sizeof(int32_t) * nq, hipMemcpyDeviceToHost));
@@ -1556,6 +1652,68 @@ vine_status vine_solve(vine_solver* s, int32_t nq, const int32_t* src, const int
     return VINE_E_OVERFLOW;
   }
   return VINE_OK;
+}
+
+vine_status vine_solver_paths(vine_solver* s, int32_t nq, int64_t* out_path_off, int32_t* out_path,
+                              int64_t path_cap) {
+  if (!s || nq < 0 || !out_path_off || (path_cap > 0 && !out_path) || path_cap < 0) {
+    set_error("vine_solver_paths: invalid arguments");
+    return VINE_E_ARG;
+  }
+  if (!s->parts.empty()) {
+    // query i is part (i % k)'s query (i / k): each part's paths, interleaved back
+    const int k = (int)s->parts.size();
+    std::vector<std::vector<int64_t>> off(k);
+    std::vector<std::vector<int32_t>> path(k);
+    for (int p = 0; p < k; p++) {
+      const int32_t np = part_size(nq, k, p);
+      off[p].assign(np + 1, 0);
+      if (np == 0) continue;
+      vine_status r = vine_solver_paths(s->parts[p], np, off[p].data(), nullptr, 0);
+      if (r != VINE_OK && r != VINE_E_OVERFLOW) return r;
+      path[p].resize(std::max<int64_t>(off[p][np], 1));
+      r = vine_solver_paths(s->parts[p], np, off[p].data(), path[p].data(), (int64_t)path[p].size());
+      if (r != VINE_OK) return r;
+    }
+    out_path_off[0] = 0;
+    for (int32_t i = 0; i < nq; i++) {
+      const auto& o = off[i % k];
+      out_path_off[i + 1] = out_path_off[i] + (o[i / k + 1] - o[i / k]);
+    }
+    if (out_path_off[nq] > path_cap) {
+      set_error("vine_solver_paths: path_cap %lld too small (need %lld)", (long long)path_cap,
+                (long long)out_path_off[nq]);
+      return VINE_E_OVERFLOW;
+    }
+    for (int32_t i = 0; i < nq; i++) {
+      const auto& o = off[i % k];
+      const auto& pv = path[i % k];
+      std::copy(pv.begin() + o[i / k], pv.begin() + o[i / k + 1], out_path + out_path_off[i]);
+    }
+    return VINE_OK;
+  }
+  if (s->last_nq != nq || !s->last_one_batch) {
+    set_error("vine_solver_paths: the last solve must have had exactly these %d queries in one batch "
+              "(at most max_batch = %d); it had %d", nq, s->B, s->last_nq);
+    return VINE_E_ARG;
+  }
+  out_path_off[0] = 0;
+  if (nq == 0) return VINE_OK;
+  vine_status es = ensure_io(s, nq);
+  if (es != VINE_OK) return es;
+  std::vector<int32_t> plen(nq);
+  hipLaunchKernelGGL(dev::k_answers, dim3((nq + 255) / 256), dim3(256), 0, s->stream, s->ctx, nq, (float*)nullptr,
+                     (float*)nullptr, (int32_t*)nullptr, s->d_out_plen);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(plen.data(), s->d_out_plen, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (int32_t i = 0; i < nq; i++) out_path_off[i + 1] = out_path_off[i] + plen[i];
+  if (out_path_off[nq] > path_cap) {
+    set_error("vine_solver_paths: path_cap %lld too small (need %lld)", (long long)path_cap,
+              (long long)out_path_off[nq]);
+    return VINE_E_OVERFLOW;
+  }
+  return extract_paths(s, nq, plen.data(), out_path);
 }
 
 vine_status vine_solver_latency(const vine_solver* s, int32_t nq, double* out_us) {
@@ -1646,5 +1804,6 @@ int64_t vine_solver_pareto(const vine_solver* s, int32_t q, int32_t v, float* ou
   }
   return live;
 }
+
 
 }  // extern "C"

@@ -48,7 +48,6 @@ constexpr int CCHUNK = 8;                   // candidate records per list chunk
 constexpr int FU = 16;                      // far-tier allocation unit (records)
 constexpr int FSEG = 64;                    // records per drain segment (one per lane)
 constexpr int FCLS_MAX = 12;                // far chunks grow 16, 32 .. 65,536 records along a bucket
-constexpr float FAR_MULT = 1.0f;            // near tier covers FAR_MULT windows past the front
 constexpr int NBK = 32768;                  // far-tier key buckets per query
 constexpr int NBW = NBK / 64;               // occupancy words per query
 constexpr float MLP_LOG2_MIN = 0.f;         // learned window range: 1 ..
@@ -142,6 +141,7 @@ struct Ctx {
   int32_t far_cap;    // units
   int2* dl;           // [far_cap] drain list: segments {first unit, records <= FSEG} for the next expand
   float far_floor;    // smallest near-tier reach (key units)
+  float far_mult;     // the near tier reaches far_mult windows past the front
   int32_t* wq;       // [B][2][n] bound-search frontier queues
   int32_t* q_order;  // [B] bound-search dispatch order (null: slot order)
   int4* wnear;       // workgroup engine: [B][2][wn_cap] near-tier records per query
@@ -447,7 +447,7 @@ __global__ void k_init_batch(Ctx c, int nq) {
     c.q_fmin[q] = __float_as_uint(key);
     // far-tier buckets: from the source key (the smallest any label can have)
     // up to the upper bound when there is one
-    float reach = fmaxf(FAR_MULT * c.q_delta[q], c.far_floor);
+    float reach = fmaxf(c.far_mult * c.q_delta[q], c.far_floor);
     float bw = reach;
     if (PRUNE == 2 && best0.x < FINF) bw = fmaxf(bw, (best0.x - key) / (float)(NBK - 2));
     c.far_base[q] = key;
@@ -890,7 +890,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
     if (c.stats_on) {
       wave_add_keyed(c.q_exp, q, expd);
     }
-    // carried labels: near tier if within FAR_MULT windows of the front,
+    // carried labels: near tier if within far_mult windows of the front,
     // else the query's far tier (not scanned again until pulled)
     bool far = false;
     int fb = 0;
@@ -1317,7 +1317,7 @@ __device__ void pull_query(const Ctx& c, long long q) {
   const float2 best = c.q_best[q];
   const float fbase = c.far_base[q], fbw = c.far_bw[q];
   if (nb >= NBK) return;
-  const float reach = fmaxf(FAR_MULT * qdelta, c.far_floor);
+  const float reach = fmaxf(c.far_mult * qdelta, c.far_floor);
   if (fcnt <= 0) {
     // nothing parked: only keep the near/far boundary up with the front
     if (lane == 0 && front < FINF) c.far_next[q] = max(nb, far_bucket(c, (int)q, front + reach) + 1);
@@ -1516,9 +1516,9 @@ __global__ void k_answers(Ctx c, int nq, float* cost, float* delay, int32_t* fea
   if (q >= nq) return;
   int id = c.q_best_id[q];
   float2 b = c.q_best[q];
-  cost[q] = id >= 0 ? b.x : FINF;
-  delay[q] = id >= 0 ? b.y : FINF;
-  feasible[q] = id >= 0 ? 1 : 0;
+  if (cost) cost[q] = id >= 0 ? b.x : FINF;
+  if (delay) delay[q] = id >= 0 ? b.y : FINF;
+  if (feasible) feasible[q] = id >= 0 ? 1 : 0;
   if (plen) {
     int h = 0;
     for (int x = id; x >= 0; x = c.labs[x].w) h++;

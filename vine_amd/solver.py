@@ -51,7 +51,7 @@ class Options(ctypes.Structure):
         ("engine", ctypes.c_int32),
         ("labels_per_wave", ctypes.c_int32),
         ("concurrency", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 1),
+        ("near_windows", ctypes.c_float),
     ]
 
 
@@ -96,7 +96,7 @@ EXPORTS = (
     "vine_graph_destroy", "vine_solver_create", "vine_solver_destroy", "vine_solve",
     "vine_solve_device", "vine_solver_latency", "vine_solver_trace", "vine_solver_set_mlp",
     "vine_solver_pareto", "vine_solver_query_stats", "vine_family_solver_create", "vine_family_solver_destroy",
-    "vine_family_solve", "vine_family_front",
+    "vine_family_solve", "vine_family_front", "vine_solver_paths",
 )
 
 _declared = False
@@ -123,6 +123,7 @@ def _L():
             "vine_solver_query_stats": (i32, [vp, i32, vp, vp]),
             "vine_solver_set_mlp": (i32, [vp, vp, i64]),
             "vine_solver_pareto": (i64, [vp, i32, i32, vp, i64]),
+            "vine_solver_paths": (i32, [vp, i32, vp, vp, i64]),
             "vine_family_solver_create": (i32, [vp, i32, i32, i64, i32, ctypes.c_float, P(vp)]),
             "vine_family_solver_destroy": (i32, [vp]),
             "vine_family_solve": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, i64, vp]),
@@ -243,6 +244,11 @@ class Solver:
                                  ctypes.byref(st))
             if rc == VINE_E_OVERFLOW and paths and off[-1] > path_cap:
                 path_cap = int(off[-1])
+                if nq <= self.options.max_batch:
+                    # one batch: its paths are still on the device, read them again
+                    path = np.zeros(path_cap, np.int32)
+                    _check(_L().vine_solver_paths(self.h, nq, off.ctypes.data, path.ctypes.data, path_cap))
+                    break
                 continue
             _check(rc)
             break
