@@ -150,6 +150,8 @@ def main():
                     help="internal solvers on separate HIP streams (vine_options.concurrency); default: the "
                          "autotuned choice cached in vine_amd/models/autotune.json, else 3")
     ap.add_argument("--no-tuned", action="store_true", help="ignore the autotune cache (library launch defaults)")
+    ap.add_argument("--use-graph", type=int, default=None, help="override: replay rounds as a HIP graph (0/1)")
+    ap.add_argument("--near-windows", type=float, default=None, help="override: near-tier reach in windows")
     ap.add_argument("--plain-delta", type=float, default=12.0, help="window of the plain single-query solver")
     ap.add_argument("--plain-queries", type=int, default=100, help="per bound factor")
     ap.add_argument("--cpu-queries", type=int, default=32, help="batched-workload queries in the CPU sample")
@@ -195,6 +197,10 @@ def main():
         launch = tuned
     if args.concurrency is not None:
         launch["concurrency"] = args.concurrency
+    if args.use_graph is not None:
+        launch["use_graph"] = args.use_graph
+    if args.near_windows is not None:
+        launch["near_windows"] = args.near_windows
     log("launch options: %s (%s)" % (launch, "autotune cache" if tuned else "defaults"))
     solver = S.Solver(dg, time_kernels=TIME_EVERY, **base, **launch)
     if policy == S.DELTA_MLP:

@@ -100,7 +100,7 @@ __device__ void wg_pull(const Ctx& c, Counters* ct, int q, WgLds& S, int4* near_
   const int nb = c.far_next[q];
   if (nb >= NBK) return;
   const float front = __uint_as_float(S.fmin[cur]);
-  const float reach = fmaxf(FAR_MULT * S.delta, c.far_floor);
+  const float reach = fmaxf(c.far_mult * S.delta, c.far_floor);
   const int left = __hip_atomic_load(&c.far_cnt[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (left <= 0) {
     if (lane == 0 && front < FINF) c.far_next[q] = max(nb, far_bucket(c, q, front + reach) + 1);
