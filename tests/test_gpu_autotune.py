@@ -22,12 +22,11 @@ def test_autotune_search_and_cache(gpu, tmp_path):
     base = dict(max_batch=96, prune=S.PRUNE_BOUNDS, delta_policy=S.DELTA_MLP, delta_fixed=4.0, rounds_per_sync=8)
     print("autotune search, %d queries" % len(q), flush=True)
     opts = AT.tuned_options(dg, g, q, cache=cache, mlp=np.load(L.DEFAULT_MODEL), log=print, **base)
-    space = dict(AT.SPACE)
-    for k, vals in space.items():
-        assert opts[k] in vals, k
+    for names, vals in AT.SPACE:
+        assert tuple(opts[k] for k in names) in vals, names
     entry = json.load(open(cache))[AT.graph_key(g, 96, S.DELTA_MLP)]
-    assert len(entry["tried"]) == sum(len(v) for v in space.values())
-    assert AT.cached_options(g, cache=cache, **base) == {k: opts[k] for k in space}
+    assert len(entry["tried"]) == sum(len(v) for _, v in AT.SPACE)
+    assert AT.cached_options(g, cache=cache, **base) == {k: opts[k] for k in AT.KEYS}
     # the chosen configuration's answers are the oracle's
     with S.Solver(dg, **opts) as s:
         s.set_mlp(np.load(L.DEFAULT_MODEL))

@@ -33,8 +33,14 @@ from . import solver as S
 from . import workload as W
 
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models", "autotune.json")
-SPACE = (("concurrency", (1, 2, 3)), ("labels_per_wave", (16, 32, 64)), ("block_size", (64, 128, 256)),
-         ("grid_blocks", (256, 384, 768)), ("use_graph", (0, 1)))
+# coordinates searched one after another; a coordinate may set several options
+# together (streams and graph replay interact: replay removes the launch gaps
+# that other streams' kernels otherwise fill)
+SPACE = ((("concurrency", "use_graph"), ((1, 0), (1, 1), (2, 0), (2, 1), (3, 0), (3, 1))),
+         (("labels_per_wave",), ((16,), (32,), (64,))),
+         (("block_size",), ((64,), (128,), (256,))),
+         (("grid_blocks",), ((256,), (384,), (768,))))
+KEYS = tuple(k for names, _ in SPACE for k in names)
 LIBRARY_DEFAULTS = {"concurrency": 1, "labels_per_wave": 64, "block_size": 128, "grid_blocks": 384, "use_graph": 0}
 
 
@@ -72,10 +78,10 @@ def search(dg, q, base, mlp=None, reps=1, log=None):
     ref = None
     tried = []
     best_t = None
-    for name, values in SPACE:
+    for names, values in SPACE:
         scores = {}
         for v in values:
-            opts = dict(cur, **{name: v})
+            opts = dict(cur, **dict(zip(names, v)))
             t, a = _time(dg, q, opts, mlp, reps)
             if ref is None:
                 ref = a
@@ -83,11 +89,11 @@ def search(dg, q, base, mlp=None, reps=1, log=None):
                       and np.array_equal(ref.feasible, a.feasible)):
                 raise RuntimeError("launch configuration %r changed the answers" % (opts,))
             scores[v] = t
-            tried.append(({k: opts[k] for k, _ in SPACE}, len(q) / t))
+            tried.append(({k: opts[k] for k in KEYS}, len(q) / t))
             if log:
-                log("%s=%s: %.1f queries/s" % (name, v, len(q) / t))
+                log("%s=%s: %.1f queries/s" % ("/".join(names), "/".join(map(str, v)), len(q) / t))
         v_best = min(scores, key=scores.get)
-        cur[name] = v_best
+        cur.update(zip(names, v_best))
         best_t = scores[v_best]
     return cur, len(q) / best_t, tried
 
@@ -108,7 +114,7 @@ def tuned_options(dg, g, q, cache=DEFAULT_CACHE, mlp=None, reps=1, retune=False,
         return dict(base, **entries[key]["options"])
     opts, qps, tried = search(dg, q, base, mlp=mlp, reps=reps, log=log)
     if cache:
-        entries[key] = {"options": {k: opts[k] for k, _ in SPACE}, "queries_per_s": qps, "queries": len(q),
+        entries[key] = {"options": {k: opts[k] for k in KEYS}, "queries_per_s": qps, "queries": len(q),
                         "tried": [{"options": o, "queries_per_s": v} for o, v in tried]}
         os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
         with open(cache, "w") as f:
@@ -138,7 +144,7 @@ def main(argv=None):
     log = lambda m: print("[autotune] " + m, file=sys.stderr, flush=True)
     opts = tuned_options(dg, g, q, cache=args.cache, mlp=mlp, reps=args.reps, retune=True, log=log,
                          **bench_base(len(q)))
-    print(json.dumps({k: opts[k] for k, _ in SPACE}))
+    print(json.dumps({k: opts[k] for k in KEYS}))
 
 
 if __name__ == "__main__":
