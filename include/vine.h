@@ -186,6 +186,29 @@ vine_status vine_solve_device(vine_solver* s, int32_t nq, const int32_t* d_src, 
                               const float* d_bound, float* d_cost, float* d_delay,
                               int32_t* d_feasible, vine_stats* out_stats);
 
+/* No-allocation variant of vine_solver_create + vine_solve_device, for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  It calls no hipMalloc, hipFree or other allocating runtime
+ * routine: every buffer of the solve (per-query and per-(query, vertex) state,
+ * label log, Pareto pool, candidate lists, frontier tiers, counters, the MLP
+ * parameters) is cut from a static device array of VINE_STATIC_SCRATCH_BYTES
+ * that each GPU holds once the library is loaded on it, and the automatic
+ * capacities are sized to fill it.  Options as vine_solver_create, except
+ * that concurrency > 1, use_graph, time_kernels and record_trace are refused
+ * (VINE_E_ARG), as is a configuration whose buffers do not fit the array (the
+ * message names the bytes needed).  mlp_params (host, mlp_nparams floats, the
+ * vine_solver_set_mlp layout) only for VINE_DELTA_MLP.  The graph must live on
+ * the current device.  stream: a hipStream_t, or NULL for the default stream;
+ * the call returns when the answers are in d_cost / d_delay / d_feasible.
+ * Every call on one GPU shares that GPU's array (each call clears what it needs
+ * cleared), so calls that overlap on one GPU race: serialise them. */
+#define VINE_STATIC_SCRATCH_BYTES (64ull << 20)
+vine_status vine_solve_device_static(const vine_graph* g, const vine_options* opt, const float* mlp_params,
+                                     int64_t mlp_nparams, int32_t nq, const int32_t* d_src, const int32_t* d_dst,
+                                     const float* d_bound, float* d_cost, float* d_delay, int32_t* d_feasible,
+                                     void* stream, vine_stats* out_stats);
+
 /* Per-query finish time in microseconds from the start of its batch (last
  * solve), measured with the device wall clock. */
 vine_status vine_solver_latency(const vine_solver* s, int32_t nq, double* out_us);

@@ -1806,4 +1806,132 @@ int64_t vine_solver_pareto(const vine_solver* s, int32_t q, int32_t v, float* ou
 }
 
 
+// ---------------------------------------------------------------------------
+// No-allocation entry point.  Every device buffer of the solve is cut from
+// `scratch`, a file-scope device array of VINE_STATIC_SCRATCH_BYTES that each
+// GPU holds from the time the library is loaded on it until the process ends.
+// Host code never passes the array's name to the device: it keeps the address
+// hipGetSymbolAddress returned for each device (scratch_ptr) and uses that.
+}  // extern "C"
+
+static __device__ __align__(256) unsigned char scratch[VINE_STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[SCRATCH_DEVICES] = {};
+
+// The scratch address on the current device, looked up once per device.
+vine_status scratch_on_current_device(int* dev_out, char** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= SCRATCH_DEVICES) {
+    set_error("vine_solve_device_static: device %d out of range", dev);
+    return VINE_E_ARG;
+  }
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+      return VINE_E_HIP;
+    }
+    scratch_ptr[dev] = p;
+  }
+  *dev_out = dev;
+  *out = (char*)scratch_ptr[dev];
+  return VINE_OK;
+}
+}  // namespace
+
+extern "C" {
+
+vine_status vine_solve_device_static(const vine_graph* g, const vine_options* opt, const float* mlp_params,
+                                     int64_t mlp_nparams, int32_t nq, const int32_t* d_src, const int32_t* d_dst,
+                                     const float* d_bound, float* d_cost, float* d_delay, int32_t* d_feasible,
+                                     void* stream, vine_stats* out_stats) {
+  if (!g || !opt || nq < 0 || (nq > 0 && (!d_src || !d_dst || !d_bound || !d_cost || !d_delay || !d_feasible))) {
+    set_error("vine_solve_device_static: invalid arguments");
+    return VINE_E_ARG;
+  }
+  vine_options o = *opt;
+  vine_status v = validate_options(g, o);
+  if (v != VINE_OK) return v;
+  if (o.concurrency > 1 || o.use_graph || o.time_kernels || o.record_trace) {
+    set_error("vine_solve_device_static: concurrency > 1, use_graph, time_kernels and record_trace need "
+              "allocations and are not available here");
+    return VINE_E_ARG;
+  }
+  if (o.delta_policy == VINE_DELTA_MLP && (!mlp_params || mlp_nparams != vine::mlp::NPARAM)) {
+    set_error("vine_solve_device_static: VINE_DELTA_MLP needs the %d network parameters", vine::mlp::NPARAM);
+    return VINE_E_ARG;
+  }
+  int dev = -1;
+  char* base = nullptr;
+  v = scratch_on_current_device(&dev, &base);
+  if (v != VINE_OK) return v;
+  if (dev != g->device) {
+    set_error("vine_solve_device_static: the graph lives on device %d, the current device is %d", g->device, dev);
+    return VINE_E_ARG;
+  }
+  const long long B = o.max_batch;
+  // the largest label capacity whose buffers fit the scratch (the other
+  // capacities follow it), by bisection over sizing passes that only add up
+  // the buffers; room is left for the MLP parameters and query staging
+  const double room = 0.98 * (double)VINE_STATIC_SCRATCH_BYTES - 64.0 * 1024;
+  auto bytes_for = [&](long long lc, size_t* out) -> vine_status {
+    vine_solver probe;
+    init_fields(&probe, g, o);
+    probe.count_only = true;
+    Caps k;
+    vine_status r = derive_caps(o, B, lc, &k);
+    if (r == VINE_OK) r = alloc_buffers(&probe, k);
+    *out = probe.arena_used;
+    return r;
+  };
+  long long lab_cap = o.label_capacity;
+  size_t need = 0;
+  if (!lab_cap) {
+    long long lo = B + 1024, hi = INT32_MAX - 1;
+    v = bytes_for(lo, &need);
+    if (v != VINE_OK) return v;
+    if ((double)need > room) {
+      set_error("vine_solve_device_static: %lld query slots x %d vertices need %zu bytes, more than the %zu-byte "
+                "static scratch holds", B, g->n, need, (size_t)VINE_STATIC_SCRATCH_BYTES);
+      return VINE_E_ARG;
+    }
+    while (lo < hi) {
+      const long long mid = lo + (hi - lo + 1) / 2;
+      v = bytes_for(mid, &need);
+      if (v == VINE_OK && (double)need <= room) lo = mid;
+      else hi = mid - 1;
+    }
+    lab_cap = lo;
+  } else {
+    v = bytes_for(lab_cap, &need);
+    if (v != VINE_OK) return v;
+    if ((double)need > room) {
+      set_error("vine_solve_device_static: label_capacity %lld needs %zu bytes, more than the %zu-byte static "
+                "scratch holds", lab_cap, need, (size_t)VINE_STATIC_SCRATCH_BYTES);
+      return VINE_E_ARG;
+    }
+  }
+  vine_solver s;
+  init_fields(&s, g, o);
+  s.arena = base;
+  s.arena_cap = VINE_STATIC_SCRATCH_BYTES;
+  s.stream = (hipStream_t)stream;
+  Caps k;
+  v = derive_caps(o, B, lab_cap, &k);
+  if (v == VINE_OK) v = alloc_buffers(&s, k);
+  if (v == VINE_OK && o.delta_policy == VINE_DELTA_MLP) v = vine_solver_set_mlp(&s, mlp_params, mlp_nparams);
+  if (v == VINE_OK) v = vine_solve_device(&s, nq, d_src, d_dst, d_bound, d_cost, d_delay, d_feasible, out_stats);
+  // nothing to free: the buffers belong to the scratch, the stream to the caller
+  s.stream = nullptr;
+  s.h_ct = nullptr;
+  s.allocs.clear();
+  return v;
+}
+
 }  // extern "C"

@@ -96,7 +96,7 @@ EXPORTS = (
     "vine_graph_destroy", "vine_solver_create", "vine_solver_destroy", "vine_solve",
     "vine_solve_device", "vine_solver_latency", "vine_solver_trace", "vine_solver_set_mlp",
     "vine_solver_pareto", "vine_solver_query_stats", "vine_family_solver_create", "vine_family_solver_destroy",
-    "vine_family_solve", "vine_family_front", "vine_solver_paths",
+    "vine_family_solve", "vine_family_front", "vine_solver_paths", "vine_solve_device_static",
 )
 
 _declared = False
@@ -124,6 +124,7 @@ def _L():
             "vine_solver_set_mlp": (i32, [vp, vp, i64]),
             "vine_solver_pareto": (i64, [vp, i32, i32, vp, i64]),
             "vine_solver_paths": (i32, [vp, i32, vp, vp, i64]),
+            "vine_solve_device_static": (i32, [vp, P(Options), vp, i64, i32, vp, vp, vp, vp, vp, vp, vp, P(Stats)]),
             "vine_family_solver_create": (i32, [vp, i32, i32, i64, i32, ctypes.c_float, P(vp)]),
             "vine_family_solver_destroy": (i32, [vp]),
             "vine_family_solve": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, i64, vp]),
