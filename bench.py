@@ -152,6 +152,7 @@ def main():
     ap.add_argument("--no-tuned", action="store_true", help="ignore the autotune cache (library launch defaults)")
     ap.add_argument("--use-graph", type=int, default=None, help="override: replay rounds as a HIP graph (0/1)")
     ap.add_argument("--near-windows", type=float, default=None, help="override: near-tier reach in windows")
+    ap.add_argument("--hash-slots", type=int, default=None, help="override: LDS candidate table slots (0 = off)")
     ap.add_argument("--plain-delta", type=float, default=12.0, help="window of the plain single-query solver")
     ap.add_argument("--plain-queries", type=int, default=100, help="per bound factor")
     ap.add_argument("--cpu-queries", type=int, default=32, help="batched-workload queries in the CPU sample")
@@ -201,6 +202,8 @@ def main():
         launch["use_graph"] = args.use_graph
     if args.near_windows is not None:
         launch["near_windows"] = args.near_windows
+    if args.hash_slots is not None:
+        launch["hash_slots"] = args.hash_slots
     log("launch options: %s (%s)" % (launch, "autotune cache" if tuned else "defaults"))
     solver = S.Solver(dg, time_kernels=TIME_EVERY, **base, **launch)
     if policy == S.DELTA_MLP:
@@ -377,7 +380,7 @@ def main():
         "latency_us": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "work_per_step": {k: agg[k] / args.steps for k in ("rounds", "sssp_rounds", "scanned", "expanded", "relaxed",
                                                            "candidates", "cand_chunks", "inserted", "sssp_relaxed",
-                                                           "far_pulled")},
+                                                           "far_pulled", "hash_dropped", "hash_overflow")},
         "roofline": roofline,
         "exact_vs_oracle": exact,
         "host_buffers": host_leg,

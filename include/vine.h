@@ -93,7 +93,12 @@ typedef struct {
   int32_t block_size;       /* threads per workgroup of the round kernels (64, 128, 192 or 256; anything
                                else is refused with VINE_E_ARG) */
   int32_t grid_blocks;      /* workgroups per round kernel at most (0 = 384) */
-  int32_t hash_slots;       /* reserved, must be 0 (no LDS candidate table is built) */
+  int32_t hash_slots;       /* first-level candidate table in LDS, slots per workgroup of expand (0 = none;
+                               else a power of two in [16, 1024]): candidates of one workgroup for the same
+                               (query, vertex) meet in a slot of 4 and those another one dominates never
+                               reach HBM; no slot or a full slot sends a candidate to HBM directly.  With
+                               VINE_DELTA_MLP the network's expansion forecast sizes the active part each
+                               round (about two slots per predicted candidate of a workgroup) */
   int64_t label_capacity;   /* label log entries (0 = auto from free memory) */
   int64_t cand_capacity;    /* candidate slots per round (0 = auto); half as many list chunks are allocated */
   int64_t pool_capacity;    /* Pareto-set pool entries (0 = auto) */
@@ -149,6 +154,8 @@ typedef struct {
   int64_t timed_rounds;     /* rounds whose kernels the ms_* figures cover (time_kernels) */
   int64_t far_units;        /* far-tier units (16 entries each) allocated, summed over batches */
   int64_t cand_chunks;      /* candidate-list chunks folded by the merge kernel (records / chunks = fill) */
+  int64_t hash_dropped;     /* candidates the LDS table dropped (dominated by a candidate of the workgroup) */
+  int64_t hash_overflow;    /* candidates that found no slot or a full one (appended to HBM directly) */
 } vine_stats;
 
 void vine_options_default(vine_options* opt);

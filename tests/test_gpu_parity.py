@@ -299,3 +299,48 @@ def test_graph_replay_streams(gpu):
         for rep in range(2):
             ans = s.solve(q.src, q.dst, q.bound)
             _compare(g, q, ref, ans)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("slots", [16, 128, 1024])
+def test_lds_candidate_table(gpu, grid_case, slots):
+    """The first-level candidate table in LDS (hash_slots): a tiny table sends
+    most candidates to the overflow path (appended to HBM directly), a large
+    one keeps them; answers equal the oracle's either way, with a fixed window
+    and with the learned window that sizes the table each round."""
+    from vine_amd import learn as L
+    g, q, ref = grid_case
+    print("LDS candidate table, %d slots" % slots, flush=True)
+    for policy, delta in ((S.DELTA_FIXED, 300.0), (S.DELTA_MLP, 4.0), (S.DELTA_INF, 0.0)):
+        with S.Solver(g, max_batch=25, prune=2, delta_policy=policy, delta_fixed=delta, hash_slots=slots) as s:
+            if policy == S.DELTA_MLP:
+                s.set_mlp(np.load(L.DEFAULT_MODEL))
+            ans = s.solve(q.src, q.dst, q.bound)
+        _compare(g, q, ref, ans)
+        st = ans.stats
+        print("  policy %d: candidates %d, dropped %d, overflow %d" % (policy, st["candidates"], st["hash_dropped"],
+                                                                      st["hash_overflow"]), flush=True)
+        if slots == 16 and policy == S.DELTA_INF:
+            assert st["hash_overflow"] > 0
+
+
+@pytest.mark.gpu
+def test_lds_candidate_table_knn(gpu):
+    """The table on the geometric sample with the reference's one-to-all search
+    (full Pareto set at t compared) and with the bounds."""
+    g = W.knn(20000, seed=5)
+    q = W.hop_queries(g, 30, hop_lo=5, hop_hi=80, f_permille=(1050, 1500), seed=5)
+    ref = [O.cspp(g, q.src[i], q.dst[i], q.bound[i], prune_target=2) for i in range(len(q))]
+    print("LDS candidate table, knn20k", flush=True)
+    for prune, slots in ((2, 64), (2, 1024), (1, 256)):
+        with S.Solver(g, max_batch=16, prune=prune, delta_policy=S.DELTA_FIXED, delta_fixed=500.0,
+                      hash_slots=slots) as s:
+            ans = s.solve(q.src, q.dst, q.bound)
+        _compare(g, q, ref, ans)
+    with S.Solver(g, max_batch=30, prune=0, hash_slots=32) as s:
+        ans = s.solve(q.src[:6], q.dst[:6], q.bound[:6])
+        for i in range(6):
+            full = O.cspp(g, q.src[i], q.dst[i], q.bound[i])
+            got = s.pareto(i, q.dst[i])
+            assert np.array_equal(got, full.pareto), i
+    assert ans.stats["hash_dropped"] + ans.stats["hash_overflow"] > 0

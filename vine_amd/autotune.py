@@ -12,9 +12,10 @@ later run loads the choice instead of searching again:
 Searched, one coordinate at a time from the library defaults: the number of
 internal solvers on separate HIP streams (concurrency), frontier entries a
 wave takes per expand step (labels_per_wave), threads per workgroup of the
-round kernels (block_size), workgroups per round kernel (grid_blocks) and
+round kernels (block_size), workgroups per round kernel (grid_blocks),
 whether the rounds between host checks replay as one captured HIP graph
-(use_graph).  Every configuration must return the same answers as the
+(use_graph) and the slots of the first-level candidate table in LDS
+(hash_slots).  Every configuration must return the same answers as the
 first one (they only change the launch shape); a difference raises.
 
     python -m vine_amd.autotune ny_batch        # tune the bench workload, update the cache
@@ -39,9 +40,11 @@ DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models
 SPACE = ((("concurrency", "use_graph"), ((1, 0), (1, 1), (2, 0), (2, 1), (3, 0), (3, 1))),
          (("labels_per_wave",), ((16,), (32,), (64,))),
          (("block_size",), ((64,), (128,), (256,))),
-         (("grid_blocks",), ((256,), (384,), (768,))))
+         (("grid_blocks",), ((256,), (384,), (768,))),
+         (("hash_slots",), ((0,), (64,), (256,), (1024,))))
 KEYS = tuple(k for names, _ in SPACE for k in names)
-LIBRARY_DEFAULTS = {"concurrency": 1, "labels_per_wave": 64, "block_size": 128, "grid_blocks": 384, "use_graph": 0}
+LIBRARY_DEFAULTS = {"concurrency": 1, "labels_per_wave": 64, "block_size": 128, "grid_blocks": 384, "use_graph": 0,
+                    "hash_slots": 0}
 
 
 def graph_key(g, max_batch, policy):

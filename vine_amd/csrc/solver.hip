@@ -81,6 +81,7 @@ struct vine_solver {
   int32_t *d_qexp = nullptr, *d_qins = nullptr, *d_qnear = nullptr;
   long long* d_qexptot = nullptr;
   float* d_qfcast = nullptr;
+  float* d_hash_pred = nullptr;
   float* d_trace = nullptr;
   long long trace_cap = 0;
   // per (query, vertex)
@@ -419,8 +420,9 @@ vine_status validate_options(const vine_graph* g, vine_options& o) {
     set_error("vine_solver_create: use_graph must be 0 or 1");
     return VINE_E_ARG;
   }
-  if (o.hash_slots) {
-    set_error("vine_solver_create: hash_slots is reserved (fronts live in the HBM pool) and must be 0");
+  if (o.hash_slots && (o.hash_slots < 16 || o.hash_slots > dev::HASH_MAX_SLOTS ||
+                       (o.hash_slots & (o.hash_slots - 1)))) {
+    set_error("vine_solver_create: hash_slots must be 0 or a power of two in [16, %d]", dev::HASH_MAX_SLOTS);
     return VINE_E_ARG;
   }
   if (!(o.near_windows >= 0.f) || o.near_windows > 64.f) {
@@ -520,6 +522,7 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
   ALLOC(s->d_qnear, sizeof(int32_t) * B);
   ALLOC(s->d_qexptot, sizeof(long long) * B);
   ALLOC(s->d_qfcast, sizeof(float) * B);
+  ALLOC(s->d_hash_pred, sizeof(float) * 2);
   if (o.record_trace > 0) {
     s->trace_cap = 8LL << 20;
     ALLOC(s->d_trace, sizeof(float) * 10 * s->trace_cap);
@@ -598,6 +601,10 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
   c.q_near = s->d_qnear;
   c.q_exptot = s->d_qexptot;
   c.q_fcast = s->d_qfcast;
+  c.hash_slots = o.hash_slots;
+  c.hash_mlp = (o.hash_slots > 0 && o.delta_policy == VINE_DELTA_MLP) ? 1 : 0;
+  c.hash_pred = s->d_hash_pred;
+  c.avg_deg = g->n > 0 ? (float)g->m / (float)g->n : 0.f;
   c.policy = o.delta_policy;
   c.stats_on = (o.delta_policy == VINE_DELTA_MLP || o.record_trace > 0) ? 1 : 0;
   c.mlp = nullptr;
@@ -794,6 +801,9 @@ vine_status poll(vine_solver* s) {
   return VINE_OK;
 }
 
+// Dynamic LDS of k_expand: the first-level candidate table.
+size_t expand_smem(const vine_solver* s) { return (size_t)s->opt.hash_slots * sizeof(dev::HashSlot); }
+
 // sample: 1 brackets this round's kernels with HIP events, 0 never, -1 every
 // time_kernels-th round
 template <int PRUNE>
@@ -806,7 +816,7 @@ vine_status launch_round(vine_solver* s, int gsz, vine_stats* st, int sample_mod
   {
     Timer t(s, 0, sample);
     hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, PRUNE>), dim3(gsz),
-                       dim3(s->block), 0, s->stream, c, s->lpw);
+                       dim3(s->block), expand_smem(s), s->stream, c, s->lpw);
     HIP_TRY(hipGetLastError());
     HIP_TRY(t.stop());
   }
@@ -858,16 +868,16 @@ vine_status capture_rounds(vine_solver* s, int K) {
   for (int k = 0; k < K && err == hipSuccess; k++) {
     switch (s->opt.prune) {
       case 0:
-        hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, 0>), dim3(gsz), dim3(s->block), 0,
-                           s->stream, c, s->lpw);
+        hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, 0>), dim3(gsz), dim3(s->block),
+                           expand_smem(s), s->stream, c, s->lpw);
         break;
       case 1:
-        hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, 1>), dim3(gsz), dim3(s->block), 0,
-                           s->stream, c, s->lpw);
+        hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, 1>), dim3(gsz), dim3(s->block),
+                           expand_smem(s), s->stream, c, s->lpw);
         break;
       default:
-        hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, 2>), dim3(gsz), dim3(s->block), 0,
-                           s->stream, c, s->lpw);
+        hipLaunchKernelGGL((dev::k_expand<c_label, c_constraint<c_label>, c_edge, 2>), dim3(gsz), dim3(s->block),
+                           expand_smem(s), s->stream, c, s->lpw);
     }
     if (err == hipSuccess) err = hipGetLastError();
     switch (s->opt.prune) {
@@ -1137,6 +1147,8 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
   st->far_units += (int64_t)s->h_ct->far_chunks;
   st->cand_chunks += (int64_t)s->h_ct->cand_chunks;
   st->scanned += (int64_t)s->h_ct->scanned;
+  st->hash_dropped += (int64_t)s->h_ct->hash_dropped;
+  st->hash_overflow += (int64_t)s->h_ct->hash_overflow;
   st->batches++;
   return VINE_OK;
 }
@@ -1320,6 +1332,8 @@ void merge_stats(vine_stats& a, const vine_stats& b) {
   a.timed_rounds += b.timed_rounds;
   a.far_units += b.far_units;
   a.cand_chunks += b.cand_chunks;
+  a.hash_dropped += b.hash_dropped;
+  a.hash_overflow += b.hash_overflow;
 }
 
 // Per-query records of the parts, in the caller's query order.

@@ -78,6 +78,8 @@ struct Counters {
   long long far_live;                // records currently in far tiers
   unsigned long long far_pulled;     // records moved far -> near
   unsigned long long near_carried, far_appended, scanned;
+  unsigned long long hash_dropped;   // candidates dropped by the LDS table (dominated by another candidate)
+  unsigned long long hash_overflow;  // candidates that found no room in the LDS table (appended directly)
   long long trace_n;  // scheduler feature rows recorded
 };
 
@@ -140,6 +142,15 @@ struct Ctx {
   int4* frec;         // [far_cap * FU] records
   int32_t far_cap;    // units
   int2* dl;           // [far_cap] drain list: segments {first unit, records <= FSEG} for the next expand
+  // first-level candidate table in LDS (hash_slots > 0): per workgroup,
+  // hash_slots slots keyed by (query, vertex), each a small Pareto set of
+  // HASH_K candidates; hash_pred[2] (by round parity) is the learned policy's
+  // predicted expansions of the next round, from which the active capacity
+  // follows (hash_mlp)
+  int32_t hash_slots;
+  int32_t hash_mlp;
+  float* hash_pred;
+  float avg_deg;
   float far_floor;    // smallest near-tier reach (key units)
   float far_mult;     // the near tier reaches far_mult windows past the front
   int32_t* wq;       // [B][2][n] bound-search frontier queues
@@ -469,6 +480,10 @@ __global__ void k_init_batch(Ctx c, int nq) {
     c.far_cnt[q] = 0;
   }
   c.q_fmin[c.B + q] = __float_as_uint(FINF);
+  if (q == 0 && c.hash_pred) {
+    c.hash_pred[0] = -1.f;
+    c.hash_pred[1] = -1.f;
+  }
   c.q_last[q] = -1;
   c.q_exp[q] = 0;
   c.q_ins[q] = 0;
@@ -772,6 +787,130 @@ __device__ __forceinline__ void far_append(const Ctx& c, Counters* ct, int q, in
   }
 }
 
+// Append a group of candidate records for one (query, vertex) to its HBM
+// list: `gm` holds the group's lanes (those with ok set), `lead` one of them,
+// `h` the list head the lead read.  The lead claims slots in the head chunk
+// for all of them with one atomic and publishes new full chunks for the rest
+// as one CAS-linked run; a list that was empty is recorded as touched.  A head
+// read before other lanes published theirs is still a chunk of this round's
+// list; the CAS links behind whatever is current.  Wave-uniform call.
+__device__ __forceinline__ void append_group(const Ctx& c, Counters* ct, int cur, bool ok, long long qv, int h,
+                                             int lead, unsigned long long gm, const int4& rec) {
+  const int lane = lane_id();
+  const int gn = __popcll(gm);
+  const int rank = __popcll(gm & ((1ull << lane) - 1ull));
+  h = __shfl(h, lead, WAVE);
+  int s0 = CCHUNK;  // first slot the group claimed in the head chunk
+  if (ok && lane == lead && h >= 0) s0 = atomicAdd(&c.chd[h].x, gn);
+  s0 = __shfl(s0, lead, WAVE);
+  const int nfit = min(max(CCHUNK - s0, 0), gn);
+  if (ok && rank < nfit) c.crec[(long long)h * CCHUNK + s0 + rank] = rec;
+  const int rest = gn - nfit;  // records for new chunks
+  bool first = false;
+  int nc = -1;
+  if (ok && lane == lead && rest > 0) {
+    const int k = (rest + CCHUNK - 1) / CCHUNK;
+    nc = atomicAdd(&ct->cand_ctr[cur], k);
+    if (nc < 0 || nc > c.chunk_cap - k) {
+      atomicOr(&ct->overflow, OVF_CAND);
+      nc = -1;
+    } else {
+      // chunks nc .. nc+k-1, full but the last; the last is the new head
+      // and links back to nc, which links to the list as it was
+      for (int j = 1; j < k; j++) c.chd[nc + j] = make_int2(j < k - 1 ? CCHUNK : rest - (k - 1) * CCHUNK, nc + j - 1);
+      int expected = h;
+      for (;;) {
+        c.chd[nc] = make_int2(k == 1 ? rest : CCHUNK, expected);
+        __threadfence();
+        const int prev = atomicCAS(&c.cand_head[qv], expected, nc + k - 1);
+        if (prev == expected) break;
+        expected = prev;  // another lane published first: link behind it
+      }
+      first = (expected < 0);
+    }
+  }
+  nc = __shfl(nc, lead, WAVE);
+  if (ok && rank >= nfit && nc >= 0) {
+    const int j = rank - nfit;
+    c.crec[(long long)(nc + j / CCHUNK) * CCHUNK + j % CCHUNK] = rec;
+  }
+  const int tix = wave_alloc32(&ct->touched_n[cur], first);
+  if (first) {
+    if (tix < c.chunk_cap) c.touched[tix] = qv;
+    else atomicOr(&ct->overflow, OVF_TOUCH);
+  }
+}
+
+// First-level candidate table in LDS, the counterpart of the reference's
+// per-block candidate hash (algorithm/bucket.cuh:29-41, 79-119): candidates of
+// one workgroup for the same (query, vertex) meet in a slot holding a small
+// Pareto set, so candidates dominated by another one of the workgroup never
+// reach HBM.  A slot is claimed by a 64-bit CAS on its key (linear probing,
+// HASH_PROBES probes), updated under a per-slot LDS lock; a candidate that
+// finds no slot, a full slot or a lock held too long is appended to HBM
+// directly (the overflow path).  The slots are flushed to the HBM lists at the
+// end of the kernel.
+constexpr int HASH_K = 4;          // candidates kept per slot
+constexpr int HASH_PROBES = 8;     // linear probes before a candidate overflows
+constexpr int HASH_SPIN = 64;      // lock attempts before a candidate overflows
+constexpr int HASH_MAX_SLOTS = 1024;
+constexpr unsigned long long HASH_EMPTY = ~0ull;
+struct HashSlot {
+  unsigned long long key;  // (query, vertex) index, HASH_EMPTY = free
+  int lock;
+  int cnt;
+  int4 ent[HASH_K];        // {d bits, delay bits, parent label, 0}
+};
+
+__device__ __forceinline__ int hash_slot_of(unsigned long long key, int slots) {
+  return (int)(((key * 0x9E3779B97F4A7C15ull) >> 40) & (unsigned long long)(slots - 1));
+}
+
+// The slot of `key` (claimed if free), or -1 after HASH_PROBES probes.
+__device__ __forceinline__ int hash_find(HashSlot* T, int slots, unsigned long long key) {
+  int h = hash_slot_of(key, slots);
+  for (int p = 0; p < HASH_PROBES; p++) {
+    unsigned long long k = T[h].key;
+    if (k == HASH_EMPTY) k = atomicCAS(&T[h].key, HASH_EMPTY, key);
+    if (k == HASH_EMPTY || k == key) return h;
+    h = (h + 1) & (slots - 1);
+  }
+  return -1;
+}
+
+// Offer candidate x to slot h: 1 = kept or dominated (handled in LDS),
+// 0 = the slot is full or busy (the caller appends x to HBM).  *dropped is set
+// when x was dominated by a slot entry.
+template <typename Label_t>
+__device__ __forceinline__ int hash_offer(HashSlot& S, int vid, const int4& x, bool* dropped) {
+  using LT = label_traits<Label_t>;
+  int spin = 0;
+  while (atomicCAS(&S.lock, 0, 1) != 0)
+    if (++spin >= HASH_SPIN) return 0;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  const Label_t xl = LT::unpack(vid, make_float2(__int_as_float(x.x), __int_as_float(x.y)));
+  int n = S.cnt, res = 1;
+  bool dom = false;
+  for (int j = 0; j < n && !dom; j++)
+    dom = Label_t::dominate(LT::unpack(vid, make_float2(__int_as_float(S.ent[j].x), __int_as_float(S.ent[j].y))), xl);
+  if (dom) {
+    *dropped = true;
+  } else {
+    int w = 0;  // entries x dominates leave the slot
+    for (int j = 0; j < n; j++) {
+      const int4 e = S.ent[j];
+      if (!Label_t::dominate(xl, LT::unpack(vid, make_float2(__int_as_float(e.x), __int_as_float(e.y)))))
+        S.ent[w++] = e;
+    }
+    if (w < HASH_K) S.ent[w++] = x;
+    else res = 0;
+    S.cnt = w;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  atomicExch(&S.lock, 0);
+  return res;
+}
+
 // Expand (PRUNE: VINE_PRUNE_*).  Every frontier entry is checked; the live
 // ones with key <= the query's smallest key + window are expanded, the rest
 // carried to the near or far tier.  The window plays the role of the
@@ -801,10 +940,31 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
   __shared__ int s_lead[MAX_WAVES][GROUP_SLOTS];
   __shared__ unsigned long long s_mask[MAX_WAVES][GROUP_SLOTS];
   for (int j = threadIdx.x; j < MAX_WAVES * GROUP_SLOTS; j += blockDim.x) (&s_mask[0][0])[j] = 0ull;
+  // first-level candidate table (dynamic LDS of hash_slots slots); the
+  // learned policy sizes its active part from the expansions it predicts for
+  // this round: about two slots per candidate of a workgroup
+  extern __shared__ __align__(16) unsigned char dyn_smem[];
+  HashSlot* T = (HashSlot*)dyn_smem;
+  int hslots = c.hash_slots;
+  if (hslots > 0 && c.hash_mlp) {
+    const float pred = c.hash_pred[r & 1];
+    if (pred >= 0.f) {
+      const float per_wg = pred * c.avg_deg / (float)gridDim.x;
+      int want = 16;
+      while ((float)want < 2.f * per_wg && want < hslots) want <<= 1;
+      hslots = want;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) c.hash_pred[(r + 1) & 1] = 0.f;
+  }
+  for (int j = threadIdx.x; j < hslots; j += blockDim.x) {
+    T[j].key = HASH_EMPTY;
+    T[j].lock = 0;
+    T[j].cnt = 0;
+  }
   __syncthreads();
   const int wid = threadIdx.x / WAVE;
   const long long nwaves = (long long)gridDim.x * (blockDim.x / WAVE);
-  int n_exp = 0, n_rel = 0, n_cand = 0, n_far = 0, n_near = 0, n_scan = 0;
+  int n_exp = 0, n_rel = 0, n_cand = 0, n_far = 0, n_near = 0, n_scan = 0, n_hdrop = 0, n_hover = 0;
   const uint32_t* fmin_cur = c.q_fmin + (size_t)cur * c.B;
   uint32_t* fmin_nxt = c.q_fmin + (size_t)nxt * c.B;
 
@@ -951,71 +1111,73 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
         // other in merge
         if (ok && front_dominates<Label_t>(c.pool_cd + (uint32_t)ps.x, pset_cnt(ps), son.vid, son)) ok = false;
       }
-      // Lock-free append to the (query, vertex) chunk list.  The wave's
-      // candidates for one (query, vertex) append as a group (several labels
-      // of one vertex reach the same neighbours in the same step): its leader
-      // claims slots in the head chunk for all of them, then publishes new
-      // full chunks for the rest in one run.  A head read before other lanes
-      // published theirs is still a chunk of this round's list; the CAS links
-      // behind whatever is current.
+      // The wave's candidates for one (query, vertex) form a group (several
+      // labels of one vertex reach the same neighbours in the same step).
       int lead = lane;
       unsigned long long gm = 1ull << lane;
       wave_group_by_key(s_lead[wid], s_mask[wid], ok, qv, lane, lead, gm);
-      const int gn = __popcll(gm);
-      const int rank = __popcll(gm & ((1ull << lane) - 1ull));
-      h = __shfl(h, lead, WAVE);
       int4 rec = make_int4(0, 0, -1, 0);
       if (ok) {
         const float2 pk = LT::pack(son);
         rec = make_int4(__float_as_int(pk.x), __float_as_int(pk.y), oid, 0);
       }
-      int s0 = CCHUNK;  // first slot the group claimed in the head chunk
-      if (ok && lane == lead && h >= 0) s0 = atomicAdd(&c.chd[h].x, gn);
-      s0 = __shfl(s0, lead, WAVE);
-      const int nfit = min(max(CCHUNK - s0, 0), gn);
-      if (ok && rank < nfit) c.crec[(long long)h * CCHUNK + s0 + rank] = rec;
-      const int rest = gn - nfit;  // records for new chunks
-      bool first = false;
-      int nc = -1;
-      if (ok && lane == lead && rest > 0) {
-        const int k = (rest + CCHUNK - 1) / CCHUNK;
-        nc = atomicAdd(&ct->cand_ctr[cur], k);
-        if (nc < 0 || nc > c.chunk_cap - k) {
-          atomicOr(&ct->overflow, OVF_CAND);
-          nc = -1;
-        } else {
-          // chunks nc .. nc+k-1, full but the last; the last is the new head
-          // and links back to nc, which links to the list as it was
-          for (int j = 1; j < k; j++)
-            c.chd[nc + j] = make_int2(j < k - 1 ? CCHUNK : rest - (k - 1) * CCHUNK, nc + j - 1);
-          int expected = h;
-          for (;;) {
-            c.chd[nc] = make_int2(k == 1 ? rest : CCHUNK, expected);
-            __threadfence();
-            const int prev = atomicCAS(&c.cand_head[qv], expected, nc + k - 1);
-            if (prev == expected) break;
-            expected = prev;  // another lane published first: link behind it
-          }
-          first = (expected < 0);
-        }
-      }
-      nc = __shfl(nc, lead, WAVE);
-      if (ok && rank >= nfit && nc >= 0) {
-        const int j = rank - nfit;
-        c.crec[(long long)(nc + j / CCHUNK) * CCHUNK + j % CCHUNK] = rec;
+      if (hslots > 0) {
+        // first level: the group's lead finds the slot, then its members are
+        // offered to it one at a time (members of one group share a slot;
+        // other groups' slots are distinct)
+        int slot = -1;
+        if (ok && lane == lead) slot = hash_find(T, hslots, (unsigned long long)qv);
+        slot = __shfl(slot, lead, WAVE);
+        const int rank = __popcll(gm & ((1ull << lane) - 1ull));
+        int gmax = ok ? __popcll(gm) : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) gmax = max(gmax, __shfl_xor(gmax, o, WAVE));
+        bool held = false, dropped = false;
+        for (int k = 0; k < gmax; k++)
+          if (ok && slot >= 0 && rank == k) held = hash_offer<Label_t>(T[slot], son.vid, rec, &dropped) != 0;
+        n_hdrop += dropped ? 1 : 0;
+        n_hover += (ok && !held) ? 1 : 0;
+        // the rest go to HBM now, as a group
+        ok = ok && !held;
+        gm = __ballot(ok) & gm;
+        lead = gm ? __ffsll((long long)gm) - 1 : lane;
       }
       n_cand += ok ? 1 : 0;
-      int tix = wave_alloc32(&ct->touched_n[cur], first);
-      if (first) {
-        if (tix < c.chunk_cap) c.touched[tix] = qv;
-        else atomicOr(&ct->overflow, OVF_TOUCH);
-      }
+      append_group(c, ct, cur, ok, qv, h, lead, gm, rec);
     });
   }
-  unsigned long long* const ctrs[7] = {&ct->expanded, &ct->relaxed,      &ct->cands,  (unsigned long long*)&ct->far_live,
-                                       &ct->far_appended, &ct->near_carried, &ct->scanned};
-  const long long vals[7] = {n_exp, n_rel, n_cand, n_far, n_far, n_near, n_scan};
-  block_add<7>(ctrs, vals);
+  if (hslots > 0) {
+    // flush the table: 16 slots per wave step, lanes 4j .. 4j+3 holding slot
+    // j's entries, one group per slot
+    __syncthreads();
+    const int nw = blockDim.x / WAVE;
+    for (int base = wid * (WAVE / HASH_K); base < hslots; base += nw * (WAVE / HASH_K)) {
+      const int sl = base + lane / HASH_K, e = lane % HASH_K;
+      bool valid = false;
+      long long qv = 0;
+      int4 rec = make_int4(0, 0, -1, 0);
+      int h = -1;
+      if (sl < hslots) {
+        const unsigned long long key = T[sl].key;
+        if (key != HASH_EMPTY && e < T[sl].cnt) {
+          valid = true;
+          qv = (long long)key;
+          rec = T[sl].ent[e];
+          h = __hip_atomic_load(&c.cand_head[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      const unsigned long long gm = __ballot(valid) & (((1ull << HASH_K) - 1ull) << (lane & ~(HASH_K - 1)));
+      const int lead = gm ? __ffsll((long long)gm) - 1 : lane;
+      n_cand += valid ? 1 : 0;
+      append_group(c, ct, cur, valid, qv, h, lead, gm, rec);
+    }
+  }
+  unsigned long long* const ctrs[9] = {&ct->expanded,     &ct->relaxed,      &ct->cands,
+                                       (unsigned long long*)&ct->far_live,   &ct->far_appended,
+                                       &ct->near_carried, &ct->scanned,      &ct->hash_dropped,
+                                       &ct->hash_overflow};
+  const long long vals[9] = {n_exp, n_rel, n_cand, n_far, n_far, n_near, n_scan, n_hdrop, n_hover};
+  block_add<9>(ctrs, vals);
 }
 
 // ---------------------------------------------------------------------------
@@ -1494,6 +1656,9 @@ __device__ void policy_query(const Ctx& c, int q, int nq) {
     if (lane == 0) {
       c.q_delta[q] = exp2f(l);
       c.q_fcast[q] = y[1];
+      // the second output forecasts log2(1 + expansions): their sum over the
+      // queries sizes the next round's first-level candidate table
+      if (c.hash_mlp && y[1] == y[1]) atomicAdd(&c.hash_pred[rn & 1], fmaxf(exp2f(fminf(y[1], 30.f)) - 1.f, 0.f));
     }
   }
 }

@@ -85,6 +85,8 @@ class Stats(ctypes.Structure):
         ("timed_rounds", ctypes.c_int64),
         ("far_units", ctypes.c_int64),
         ("cand_chunks", ctypes.c_int64),
+        ("hash_dropped", ctypes.c_int64),
+        ("hash_overflow", ctypes.c_int64),
     ]
 
     def as_dict(self):
