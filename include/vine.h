@@ -63,7 +63,10 @@ enum {
   VINE_DELTA_INF = 0,   /* expand the whole frontier every round */
   VINE_DELTA_FIXED = 1, /* expand labels with key <= min key + delta_fixed */
   VINE_DELTA_AR = 2,    /* the reference's DOP rule driven by an AR(3) model of log2 frontier size */
-  VINE_DELTA_MLP = 3    /* per-query window predicted by the on-device MLP (vine_solver_set_mlp) */
+  VINE_DELTA_MLP = 3,   /* per-query window predicted by the on-device MLP (vine_solver_set_mlp) */
+  VINE_DELTA_DOP = 4    /* the reference's DOP rule with its own count semantics (algorithm/cspp.cuh:179-216):
+                           each round expands the last DOP entries of the batch's frontier (all of them when
+                           DOP >= frontier) and carries the rest; no key window */
 };
 
 /* How the label search is scheduled. */

@@ -52,6 +52,7 @@ def test_grid_parity(gpu, grid_case, prune, batch):
 @pytest.mark.parametrize("policy,delta,engine", [(S.DELTA_FIXED, 0.0, S.ENGINE_ROUNDS),
                                                  (S.DELTA_FIXED, 2000.0, S.ENGINE_ROUNDS),
                                                  (S.DELTA_AR, 0.0, S.ENGINE_AUTO),
+                                                 (S.DELTA_DOP, 0.0, S.ENGINE_AUTO),
                                                  (S.DELTA_FIXED, 0.0, S.ENGINE_WORKGROUP),
                                                  (S.DELTA_FIXED, 300.0, S.ENGINE_WORKGROUP),
                                                  (S.DELTA_FIXED, 2000.0, S.ENGINE_WORKGROUP)])
@@ -344,3 +345,21 @@ def test_lds_candidate_table_knn(gpu):
             got = s.pareto(i, q.dst[i])
             assert np.array_equal(got, full.pareto), i
     assert ans.stats["hash_dropped"] + ans.stats["hash_overflow"] > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("max_labels", [2e4, 2e5, 0.0])
+def test_dop_count_semantics(gpu, grid_case, max_labels):
+    """The reference's DOP rule with its count semantics: a round expands the
+    last DOP frontier entries and carries the rest.  A small MAX makes the
+    count bind (more rounds than whole-frontier rounds); answers equal the
+    oracle's either way."""
+    g, q, ref = grid_case
+    print("DOP count semantics, MAX %g" % max_labels, flush=True)
+    with S.Solver(g, max_batch=25, prune=2, delta_policy=S.DELTA_INF) as s:
+        whole = s.solve(q.src, q.dst, q.bound)
+    with S.Solver(g, max_batch=25, prune=2, delta_policy=S.DELTA_DOP, ar_max_labels=max_labels) as s:
+        ans = s.solve(q.src, q.dst, q.bound)
+    _compare(g, q, ref, ans)
+    if max_labels == 2e4:
+        assert ans.stats["rounds"] > whole.stats["rounds"]

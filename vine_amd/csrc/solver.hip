@@ -379,7 +379,7 @@ namespace {
 // Option checks shared by vine_solver_create and vine_solve_device_static;
 // may switch the pruning mode (see below).
 vine_status validate_options(const vine_graph* g, vine_options& o) {
-  if (o.max_batch < 1 || o.prune < 0 || o.prune > 2 || o.delta_policy < 0 || o.delta_policy > 3 ||
+  if (o.max_batch < 1 || o.prune < 0 || o.prune > 2 || o.delta_policy < 0 || o.delta_policy > 4 ||
       o.rounds_per_sync < 1 || o.grid_blocks < 0 || o.hash_slots < 0 || !(o.delta_fixed >= 0.f) ||
       o.bound_ball < 0 || o.bound_ball > 1) {
     set_error("vine_solver_create: invalid options");
@@ -606,6 +606,7 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
   c.hash_pred = s->d_hash_pred;
   c.avg_deg = g->n > 0 ? (float)g->m / (float)g->n : 0.f;
   c.policy = o.delta_policy;
+  c.dop_count = -1;
   c.stats_on = (o.delta_policy == VINE_DELTA_MLP || o.record_trace > 0) ? 1 : 0;
   c.mlp = nullptr;
   c.trace = s->d_trace;
@@ -998,6 +999,7 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
     if (s->opt.delta_policy == VINE_DELTA_FIXED) dv = s->opt.delta_fixed;
     if (s->opt.delta_policy == VINE_DELTA_MLP) dv = s->opt.delta_fixed > 0 ? s->opt.delta_fixed : 0.25f * s->g->mean_len;
     if (s->opt.delta_policy == VINE_DELTA_AR) dv = delta_ar;
+    // VINE_DELTA_DOP: no key window, a count of frontier entries per round
     vine_status fs = fill_f32(s, s->d_qdelta, dv, s->B);
     if (fs != VINE_OK) return fs;
   }
@@ -1046,8 +1048,10 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
     rounds = s->h_ct->round_e;
   } else {
     // scheduling
-    const bool ar = s->opt.delta_policy == VINE_DELTA_AR;
+    const bool count = s->opt.delta_policy == VINE_DELTA_DOP;
+    const bool ar = s->opt.delta_policy == VINE_DELTA_AR || count;
     DopRule rule;
+    c.dop_count = -1;
     if (ar) {
       double mx = s->opt.ar_max_labels;
       if (mx <= 0) {
@@ -1056,6 +1060,7 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
         mx = (double)fb / 12.0;  // free memory / sizeof(c_label), algorithm/cspp.cuh:43-47
       }
       rule.reset(mx);
+      if (count) c.dop_count = std::max<int64_t>(rule.dop(), 1);
     }
     const int K = ar ? 1 : std::max(1, s->opt.rounds_per_sync);
     // the AR rule sets the window from the host after every round, so it
@@ -1094,7 +1099,13 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
         set_error("solver made no progress after %lld rounds", (long long)rounds);
         return VINE_E_OVERFLOW;
       }
-      if (ar) {
+      if (count) {
+        // the reference's count semantics (algorithm/cspp.cuh:179-216): the
+        // next round expands the last DOP entries of the frontier (all of them
+        // when DOP >= frontier) and carries the rest
+        rule.update((int64_t)fn);
+        c.dop_count = std::max<int64_t>(rule.dop(), 1);
+      } else if (ar) {
         // the reference's DOP rule picks how many labels the next round should
         // expand; a multiplicative controller turns that count into the key
         // window (DOP >= frontier means the whole frontier, as in the reference)

@@ -164,6 +164,7 @@ struct Ctx {
   // scheduler: per-query round statistics, MLP, feature trace
   int32_t stats_on;
   int32_t policy;     // VINE_DELTA_*
+  long long dop_count;  // VINE_DELTA_DOP: frontier entries the round expands (the last ones), -1 = off
   int32_t* q_exp;     // [B] labels expanded this round
   int32_t* q_ins;     // [B] labels inserted this round
   int32_t* q_near;    // [B] near-tier entries queued this round
@@ -930,6 +931,8 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
   const long long lstart = ct->lstart[cur];
   const long long lend = min((long long)ct->label_ctr, c.label_cap);
   const long long fn = nold + (lend - lstart);
+  // count semantics: only the last dop_count entries may be expanded
+  const long long first_exp = (c.dop_count >= 0 && fn > c.dop_count) ? fn - c.dop_count : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ct->lstart[nxt] = lend;
     ct->round_m = r;
@@ -1036,7 +1039,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
         if (live && PRUNE != 0 && lex_ge(key, rd, b.x, b.y)) live = false;
         if (live) {
           c.q_last[q] = r;
-          if (!(key > lim)) {
+          if (!(key > lim) && i >= first_exp) {
             row = row0;
             deg = row1 - row0;
             n_exp++;

@@ -26,15 +26,19 @@ from . import workload as W
 
 
 def parse_policy(text):
+    """(policy, window, AR MAX or None); ar:<MAX> and dop:<MAX> set the rule's MAX."""
     if text == "mlp":
-        return S.DELTA_MLP, 4.0
-    if text == "ar":
-        return S.DELTA_AR, 0.0
+        return S.DELTA_MLP, 4.0, None
+    for name, pol in (("ar", S.DELTA_AR), ("dop", S.DELTA_DOP)):
+        if text == name:
+            return pol, 0.0, None
+        if text.startswith(name + ":"):
+            return pol, 0.0, float(text.split(":", 1)[1])
     if text == "inf":
-        return S.DELTA_INF, 0.0
+        return S.DELTA_INF, 0.0, None
     if text.startswith("fixed:"):
-        return S.DELTA_FIXED, float(text.split(":", 1)[1])
-    raise ValueError("policy must be mlp, ar, inf or fixed:<window>")
+        return S.DELTA_FIXED, float(text.split(":", 1)[1]), None
+    raise ValueError("policy must be mlp, ar[:MAX], dop[:MAX], inf or fixed:<window>")
 
 
 def main(argv=None):
@@ -49,6 +53,7 @@ def main(argv=None):
     ap.add_argument("--label-capacity", type=float, default=6e8)
     ap.add_argument("--ar-max-labels", type=float, default=0.0,
                     help="MAX of the AR rule (0 = the reference's: free memory / 12 B)")
+    ap.add_argument("--chunk", type=int, default=0, help="solve the queries in calls of this many (progress lines)")
     args = ap.parse_args(argv)
 
     t0 = time.time()
@@ -62,24 +67,40 @@ def main(argv=None):
     params = np.load(args.model) if os.path.exists(args.model) else None
     ref = None
     for text in args.policies:
-        policy, delta = parse_policy(text)
+        policy, delta, mx = parse_policy(text)
         cap = int(args.label_capacity)
         line = {"workload": args.workload, "policy": text, "queries": len(qs), "batch": args.batch,
                 "concurrency": args.concurrency}
         try:
             with S.Solver(dg, max_batch=args.batch, prune=S.PRUNE_BOUNDS, delta_policy=policy, delta_fixed=delta,
-                          rounds_per_sync=16 if policy != S.DELTA_AR else 1, engine=args.engine,
+                          rounds_per_sync=16 if policy not in (S.DELTA_AR, S.DELTA_DOP) else 1,
+                          engine=args.engine,
                           concurrency=args.concurrency,
                           label_capacity=cap, pool_capacity=2 * cap, cand_capacity=100_000_000,
-                          frontier_capacity=cap // 2, ar_max_labels=args.ar_max_labels) as s:
+                          frontier_capacity=cap // 2,
+                          ar_max_labels=args.ar_max_labels if mx is None else mx) as s:
                 if policy == S.DELTA_MLP:
                     if params is None:
                         raise RuntimeError("no model at %s" % args.model)
                     s.set_mlp(params)
                 t1 = time.perf_counter()
-                a = s.solve(qs.src, qs.dst, qs.bound, paths=False)
+                if args.chunk:
+                    parts, lat = [], []
+                    for c0 in range(0, len(qs), args.chunk):
+                        sub = qs.subset(np.arange(c0, min(c0 + args.chunk, len(qs))))
+                        parts.append(s.solve(sub.src, sub.dst, sub.bound, paths=False))
+                        lat.append(s.latency_us(len(sub)))
+                        print("[evaluate] %s: %d/%d queries, %.1f s" % (text, min(c0 + args.chunk, len(qs)), len(qs),
+                                                                       time.perf_counter() - t1),
+                              file=sys.stderr, flush=True)
+                    a = S.Answer(np.concatenate([p.cost for p in parts]), np.concatenate([p.delay for p in parts]),
+                                 np.concatenate([p.feasible for p in parts]), None, None,
+                                 {k: sum(p.stats[k] for p in parts) for k in parts[0].stats})
+                    lat = np.concatenate(lat)
+                else:
+                    a = s.solve(qs.src, qs.dst, qs.bound, paths=False)
+                    lat = s.latency_us(len(qs))
                 dt = time.perf_counter() - t1
-                lat = s.latency_us(len(qs))
             st = a.stats
             line.update({"seconds": dt, "qps": len(qs) / dt, "rounds": st["rounds"], "expanded": st["expanded"],
                          "inserted": st["inserted"], "feasible": int(a.feasible.sum()),

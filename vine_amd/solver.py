@@ -14,7 +14,7 @@ import numpy as np
 from ._lib import lib
 
 PRUNE_NONE, PRUNE_TARGET, PRUNE_BOUNDS = 0, 1, 2
-DELTA_INF, DELTA_FIXED, DELTA_AR, DELTA_MLP = 0, 1, 2, 3
+DELTA_INF, DELTA_FIXED, DELTA_AR, DELTA_MLP, DELTA_DOP = 0, 1, 2, 3, 4
 ENGINE_AUTO, ENGINE_ROUNDS, ENGINE_WORKGROUP = 0, 1, 2
 
 VINE_OK, VINE_E_ARG, VINE_E_OOM, VINE_E_HIP, VINE_E_OVERFLOW, VINE_E_RANGE = 0, -1, -2, -3, -4, -5
