@@ -86,7 +86,7 @@ struct vine_solver {
   long long trace_cap = 0;
   // per (query, vertex)
   int2* d_pset = nullptr;
-  int32_t* d_cand_head = nullptr;
+  unsigned long long* d_cand_head = nullptr;
   float* d_lbl = nullptr;
   unsigned long long* d_lbd = nullptr;
   uint32_t* d_inq = nullptr;
@@ -476,7 +476,7 @@ void init_fields(vine_solver* s, const vine_graph* g, const vine_options& o) {
 // Per-(query, vertex) state: pset, cand_head; with the bounds lbl, lbd, inq
 // and the two bound-search queue entries.
 size_t per_qv_bytes(const vine_options& o) {
-  return sizeof(int2) + sizeof(int32_t) +
+  return sizeof(int2) + sizeof(unsigned long long) +
          (o.prune == 2 ? sizeof(float) + sizeof(unsigned long long) + sizeof(uint32_t) + 2 * sizeof(int32_t) : 0);
 }
 
@@ -528,7 +528,7 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
     ALLOC(s->d_trace, sizeof(float) * 10 * s->trace_cap);
   }
   ALLOC(s->d_pset, sizeof(int2) * BN);
-  ALLOC(s->d_cand_head, sizeof(int32_t) * BN);
+  ALLOC(s->d_cand_head, sizeof(unsigned long long) * BN);
   if (o.prune == 2) {
     ALLOC(s->d_lbl, sizeof(float) * BN);
     ALLOC(s->d_lbd, sizeof(unsigned long long) * BN);
@@ -983,7 +983,7 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
   const long long BN = (long long)s->B * c.n;
   // reset per-(query, vertex) headers and counters
   HIP_TRY(hipMemsetAsync(s->d_pset, 0, sizeof(int2) * BN, s->stream));
-  HIP_TRY(hipMemsetAsync(s->d_cand_head, 0xFF, sizeof(int32_t) * BN, s->stream));
+  HIP_TRY(hipMemsetAsync(s->d_cand_head, 0xFF, sizeof(unsigned long long) * BN, s->stream));
   HIP_TRY(hipMemsetAsync(s->d_far_head, 0xFF, sizeof(int32_t) * s->B * dev::NBK, s->stream));
   HIP_TRY(hipMemsetAsync(s->d_far_occ, 0, sizeof(unsigned long long) * s->B * dev::NBW, s->stream));
   std::memset(s->h_ct, 0, sizeof(Counters));

@@ -103,7 +103,9 @@ struct Ctx {
   int32_t* q_last;   // last round with live frontier labels
   // per (query, vertex)
   int2* pset;        // {pool offset, cnt | capcode << 26}
-  int32_t* cand_head;
+  // per (query, vertex) candidate list head: chunk index << 32 | records
+  // claimed in that chunk (may exceed CCHUNK; ~0 = empty)
+  unsigned long long* cand_head;
   float* lbl;
   unsigned long long* lbd;  // packed (delay bits << 32 | len bits): lexicographic
                             // (delay, len) distance to t; the len part is the
@@ -790,22 +792,36 @@ __device__ __forceinline__ void far_append(const Ctx& c, Counters* ct, int q, in
 
 // Append a group of candidate records for one (query, vertex) to its HBM
 // list: `gm` holds the group's lanes (those with ok set), `lead` one of them,
-// `h` the list head the lead read.  The lead claims slots in the head chunk
-// for all of them with one atomic and publishes new full chunks for the rest
-// as one CAS-linked run; a list that was empty is recorded as touched.  A head
-// read before other lanes published theirs is still a chunk of this round's
-// list; the CAS links behind whatever is current.  Wave-uniform call.
-__device__ __forceinline__ void append_group(const Ctx& c, Counters* ct, int cur, bool ok, long long qv, int h,
-                                             int lead, unsigned long long gm, const int4& rec) {
+// `hw` the head word the lead read.  Everything appenders of one kernel share
+// lives in the 64-bit head word (head chunk << 32 | records claimed in it), so
+// no fence is needed: the lead claims slots in the head chunk for the whole
+// group with one 64-bit atomicAdd on the word, and records that do not fit go
+// into new chunks that one CAS publishes as the new head (the chunk links and
+// the replaced head's final count are plain stores, read by merge after the
+// kernel boundary).  A list that was empty is recorded as touched.
+// Wave-uniform call.
+constexpr unsigned long long HEAD_EMPTY = ~0ull;
+__device__ __forceinline__ int head_chunk(unsigned long long w) { return (int)(uint32_t)(w >> 32); }
+__device__ __forceinline__ int head_count(unsigned long long w) { return (int)min((uint32_t)w, (uint32_t)CCHUNK); }
+
+__device__ __forceinline__ void append_group(const Ctx& c, Counters* ct, int cur, bool ok, long long qv,
+                                             unsigned long long hw, int lead, unsigned long long gm,
+                                             const int4& rec) {
   const int lane = lane_id();
   const int gn = __popcll(gm);
   const int rank = __popcll(gm & ((1ull << lane) - 1ull));
-  h = __shfl(h, lead, WAVE);
+  const int hidx = __shfl(head_chunk(hw), lead, WAVE);
   int s0 = CCHUNK;  // first slot the group claimed in the head chunk
-  if (ok && lane == lead && h >= 0) s0 = atomicAdd(&c.chd[h].x, gn);
+  int hcur = -1;
+  if (ok && lane == lead && hidx >= 0) {
+    const unsigned long long old = atomicAdd(&c.cand_head[qv], (unsigned long long)gn);
+    hcur = head_chunk(old);  // the head now (it never returns to empty within a round)
+    s0 = (int)min((uint32_t)old, (uint32_t)CCHUNK);
+  }
   s0 = __shfl(s0, lead, WAVE);
+  hcur = __shfl(hcur, lead, WAVE);
   const int nfit = min(max(CCHUNK - s0, 0), gn);
-  if (ok && rank < nfit) c.crec[(long long)h * CCHUNK + s0 + rank] = rec;
+  if (ok && rank < nfit) c.crec[(long long)hcur * CCHUNK + s0 + rank] = rec;
   const int rest = gn - nfit;  // records for new chunks
   bool first = false;
   int nc = -1;
@@ -817,17 +833,26 @@ __device__ __forceinline__ void append_group(const Ctx& c, Counters* ct, int cur
       nc = -1;
     } else {
       // chunks nc .. nc+k-1, full but the last; the last is the new head
-      // and links back to nc, which links to the list as it was
-      for (int j = 1; j < k; j++) c.chd[nc + j] = make_int2(j < k - 1 ? CCHUNK : rest - (k - 1) * CCHUNK, nc + j - 1);
-      int expected = h;
+      // and links back to nc, which links to the list as it was.  A head
+      // chunk's count lives in the head word: its header's count is written
+      // once, by the wave that replaces it as head (one writer per field, as
+      // plain stores from different XCDs reach memory in no fixed order)
+      const int last_cnt = rest - (k - 1) * CCHUNK;
+      for (int j = 1; j < k - 1; j++) c.chd[nc + j] = make_int2(CCHUNK, nc + j - 1);
+      if (k > 1) c.chd[nc + k - 1].y = nc + k - 2;
+      const unsigned long long neww = ((unsigned long long)(uint32_t)(nc + k - 1) << 32) | (uint32_t)last_cnt;
+      unsigned long long expected =
+          __hip_atomic_load(&c.cand_head[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       for (;;) {
-        c.chd[nc] = make_int2(k == 1 ? rest : CCHUNK, expected);
-        __threadfence();
-        const int prev = atomicCAS(&c.cand_head[qv], expected, nc + k - 1);
+        if (k == 1) c.chd[nc].y = head_chunk(expected);
+        else c.chd[nc] = make_int2(CCHUNK, head_chunk(expected));
+        const unsigned long long prev = atomicCAS(&c.cand_head[qv], expected, neww);
         if (prev == expected) break;
-        expected = prev;  // another lane published first: link behind it
+        expected = prev;  // another wave published or claimed first: retry on its word
       }
-      first = (expected < 0);
+      // the replaced head's final count, for merge
+      if (head_chunk(expected) >= 0) c.chd[head_chunk(expected)].x = head_count(expected);
+      first = head_chunk(expected) < 0;
     }
   }
   nc = __shfl(nc, lead, WAVE);
@@ -1002,32 +1027,44 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
         id = (int)(lstart + (i - nold));
       }
       if (id >= 0) {
-        // A tail round is a chain of dependent loads, so every load this
-        // entry may need is issued as soon as its address is known, in two
-        // steps (by id, then by query and vertex), whether or not the entry
-        // turns out dead, pruned or outside the window.
+        // A tail round is a chain of dependent loads, so the loads an entry
+        // needs are issued as soon as their addresses are known.  A fresh
+        // label (made by the last merge) is known only by its id: its record
+        // and meta, then everything by query and vertex.  A carried entry
+        // brings its query, key and pruning delay, so its window and pruning
+        // checks need only per-query state; its label and row are read only
+        // when it is expanded (most carried entries are carried again, and
+        // their label loads would be wasted).
         const bool dead = c.dead[id] != 0;
-        const int4 L = c.labs[id];
-        const int2 meta = fresh ? c.lmeta[id] : make_int2(q, __float_as_int(key));
+        int4 L = make_int4(0, 0, 0, 0);
+        int2 meta = make_int2(q, __float_as_int(key));
+        if (fresh) {
+          L = c.labs[id];
+          meta = c.lmeta[id];
+        }
         // an id reserved by merge but never filled is dead and holds stale
         // contents: its query and vertex are used only clamped into range,
         // for loads whose values are then ignored
         const int qq = min(max(meta.x, 0), c.B - 1);
-        const int vv = min(max(L.x, 0), c.n - 1);
         const float2 b = c.q_best[qq];
         const float lim = __uint_as_float(fmin_cur[qq]) + c.q_delta[qq];
-        const int qdst = c.q_dst[qq];
-        const float qbound = c.q_bound[qq];
-        const float lbdv = (PRUNE == 2 && fresh) ? lbd_of(c, (long long)qq * c.n + vv) : 0.f;
-        const int row0 = c.row_ptr[vv];
-        const int row1 = c.row_ptr[vv + 1];
+        int qdst = 0, row0 = 0, row1 = 0;
+        float qbound = 0.f, lbdv = 0.f;
+        if (fresh) {
+          const int vv = min(max(L.x, 0), c.n - 1);
+          qdst = c.q_dst[qq];
+          qbound = c.q_bound[qq];
+          if (PRUNE == 2) lbdv = lbd_of(c, (long long)qq * c.n + vv);
+          row0 = c.row_ptr[vv];
+          row1 = c.row_ptr[vv + 1];
+        }
         bool live = !dead;
         if (live) {
           q = meta.x;
           key = __int_as_float(meta.y);
-          lab = LT::unpack(L.x, make_float2(__int_as_float(L.y), __int_as_float(L.z)));
         }
         if (live && fresh) {
+          lab = LT::unpack(L.x, make_float2(__int_as_float(L.y), __int_as_float(L.z)));
           rd = lab.delay;
           if (PRUNE != 0 && lab.vid == qdst) live = false;  // answers are not extended
           if (live && PRUNE == 2) {
@@ -1040,6 +1077,13 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
         if (live) {
           c.q_last[q] = r;
           if (!(key > lim) && i >= first_exp) {
+            if (!fresh) {
+              L = c.labs[id];
+              lab = LT::unpack(L.x, make_float2(__int_as_float(L.y), __int_as_float(L.z)));
+              const int vv = min(max(L.x, 0), c.n - 1);
+              row0 = c.row_ptr[vv];
+              row1 = c.row_ptr[vv + 1];
+            }
             row = row0;
             deg = row1 - row0;
             n_exp++;
@@ -1090,7 +1134,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
       bool ok = false;
       Label_t son;
       long long qv = 0;
-      int h = -1;
+      unsigned long long hw = HEAD_EMPTY;
       if (act) {
         n_rel++;
         int4 e = c.edges[pos];
@@ -1101,7 +1145,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
         qv = (long long)oq * c.n + son.vid;
         // everything the checks and the append read at (query, vertex), at once
         const int2 ps = c.pset[qv];
-        h = __hip_atomic_load(&c.cand_head[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        hw = __hip_atomic_load(&c.cand_head[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const float lbdv = PRUNE == 2 ? lbd_of(c, qv) : 0.f;
         const float lblv = PRUNE == 2 ? c.lbl[qv] : 0.f;
         if (ok && PRUNE == 1 && son.vid != c.q_dst[oq] && lex_ge(son.d, son.delay, b.x, b.y)) ok = false;
@@ -1146,7 +1190,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
         lead = gm ? __ffsll((long long)gm) - 1 : lane;
       }
       n_cand += ok ? 1 : 0;
-      append_group(c, ct, cur, ok, qv, h, lead, gm, rec);
+      append_group(c, ct, cur, ok, qv, hw, lead, gm, rec);
     });
   }
   if (hslots > 0) {
@@ -1159,20 +1203,20 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
       bool valid = false;
       long long qv = 0;
       int4 rec = make_int4(0, 0, -1, 0);
-      int h = -1;
+      unsigned long long hw = HEAD_EMPTY;
       if (sl < hslots) {
         const unsigned long long key = T[sl].key;
         if (key != HASH_EMPTY && e < T[sl].cnt) {
           valid = true;
           qv = (long long)key;
           rec = T[sl].ent[e];
-          h = __hip_atomic_load(&c.cand_head[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          hw = __hip_atomic_load(&c.cand_head[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
       const unsigned long long gm = __ballot(valid) & (((1ull << HASH_K) - 1ull) << (lane & ~(HASH_K - 1)));
       const int lead = gm ? __ffsll((long long)gm) - 1 : lane;
       n_cand += valid ? 1 : 0;
-      append_group(c, ct, cur, valid, qv, h, lead, gm, rec);
+      append_group(c, ct, cur, valid, qv, hw, lead, gm, rec);
     }
   }
   unsigned long long* const ctrs[9] = {&ct->expanded,     &ct->relaxed,      &ct->cands,
@@ -1232,13 +1276,15 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_merge(Ctx c) {
     const bool have = ti < tn;
     long long qv = 0;
     int q = 0, v = 0, h = -1, cnt = 0, cap = 0, tq = -1;
+    unsigned long long hw = HEAD_EMPTY;
     uint32_t off = 0;
     float lb = 0.f;
     if (have) {
       qv = c.touched[ti];
       q = (int)(qv / c.n);
       v = (int)(qv - (long long)q * c.n);
-      h = c.cand_head[qv];
+      hw = c.cand_head[qv];
+      h = head_chunk(hw);
       int2 ps = c.pset[qv];
       off = (uint32_t)ps.x;
       cnt = pset_cnt(ps);
@@ -1246,7 +1292,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_merge(Ctx c) {
       tq = c.q_dst[q];
       if (PRUNE == 2) lb = c.lbl[qv];
     }
-    if (have && gl == 0) c.cand_head[qv] = -1;
+    if (have && gl == 0) c.cand_head[qv] = HEAD_EMPTY;
     // the front's last entry stays in registers across the candidates
     float2 lastv = (have && cnt > 0) ? c.pool_cd[off + cnt - 1] : make_float2(-1.f, FINF);
     // smallest key and count of the labels this group inserts, published once
@@ -1262,6 +1308,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_merge(Ctx c) {
     int4 hrec = make_int4(0, 0, -1, 0);
     if (h >= 0) {
       hd = c.chd[h];
+      hd.x = head_count(hw);  // the head chunk's count lives in the head word
       if (gl < CCHUNK) hrec = c.crec[(long long)h * CCHUNK + gl];
     }
     while (__ballot(h >= 0)) {
