@@ -363,3 +363,57 @@ def test_dop_count_semantics(gpu, grid_case, max_labels):
     _compare(g, q, ref, ans)
     if max_labels == 2e4:
         assert ans.stats["rounds"] > whole.stats["rounds"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lagrange", [0, 1])
+@pytest.mark.parametrize("policy,delta", [(S.DELTA_FIXED, 200.0), (S.DELTA_INF, 0.0)])
+def test_lagrange_bound_grid(gpu, grid_case, lagrange, policy, delta):
+    """The bound pruning with and without the Lagrangian search: the same
+    answers as the oracle on the grid workload (len 1-1000 against delay 1-100,
+    so the multipliers are large)."""
+    g, q, ref = grid_case
+    print("grid64 lagrange=%d policy=%d" % (lagrange, policy), flush=True)
+    with S.Solver(g, max_batch=100, prune=2, delta_policy=policy, delta_fixed=delta, lagrange_bound=lagrange) as s:
+        ans = s.solve(q.src, q.dst, q.bound)
+    _compare(g, q, ref, ans)
+
+
+@pytest.mark.gpu
+def test_lagrange_bound_tight_knn(gpu):
+    """Tight bounds (1.02-1.10 x the minimum delay) on a 20k-vertex road-like
+    graph, where the Lagrangian bound prunes most: answers and paths equal the
+    label-setting oracle's with and without it, and it expands fewer labels."""
+    g = W.knn(20000, seed=11)
+    q = W.hop_queries(g, 48, hop_lo=20, hop_hi=120, f_permille=(1020, 1100), seed=11)
+    ref = [O.cspp_astar(g, q.src[i], q.dst[i], q.bound[i]) for i in range(len(q))]
+    expanded = {}
+    for lagrange in (0, 1):
+        print("knn20k tight lagrange=%d" % lagrange, flush=True)
+        with S.Solver(g, max_batch=48, prune=2, delta_policy=S.DELTA_FIXED, delta_fixed=40.0,
+                      lagrange_bound=lagrange) as s:
+            ans = s.solve(q.src, q.dst, q.bound)
+        _compare(g, q, ref, ans)
+        expanded[lagrange] = ans.stats["expanded"]
+    assert expanded[1] < expanded[0], expanded
+
+
+@pytest.mark.gpu
+def test_lagrange_bound_large_weights(gpu):
+    """Lengths up to 100,000 per arc: the Lagrangian sums (8 x length plus k x
+    delay) pass 2^24 on long paths while the plain sums stay below it; such
+    values are not used, and every answer is still the oracle's."""
+    g0, q = W.preset("grid64")
+    g = W.Graph(g0.row_ptr, g0.dst, (g0.len * 100.0).astype(np.float32), g0.delay)
+    ref = [O.cspp(g, q.src[i], q.dst[i], q.bound[i], prune_target=2) for i in range(len(q))]
+    assert max(r.cost for r in ref if r.feasible) * 8 > 2 ** 24
+    with S.Solver(g, max_batch=100, prune=2, delta_policy=S.DELTA_FIXED, delta_fixed=20000.0, lagrange_bound=1) as s:
+        ans = s.solve(q.src, q.dst, q.bound)
+    _compare(g, q, ref, ans)
+
+
+@pytest.mark.gpu
+def test_lagrange_bound_refused_values(gpu, grid_case):
+    g, q, _ = grid_case
+    with pytest.raises(S.VineError):
+        S.Solver(g, max_batch=4, prune=2, lagrange_bound=2)

@@ -87,7 +87,8 @@ struct vine_solver {
   // per (query, vertex)
   int2* d_pset = nullptr;
   unsigned long long* d_cand_head = nullptr;
-  float* d_lbl = nullptr;
+  float2* d_lbl = nullptr;  // {lbl, lbw} per (query, vertex)
+  float* d_qlk = nullptr;
   unsigned long long* d_lbd = nullptr;
   uint32_t* d_inq = nullptr;
   int32_t* d_wq = nullptr;
@@ -221,6 +222,7 @@ void vine_options_default(vine_options* o) {
   o->grid_blocks = 0;
   o->hash_slots = 0;
   o->ar_max_labels = 0.0;
+  o->lagrange_bound = 1;
 }
 
 int32_t vine_device_count(void) {
@@ -425,6 +427,10 @@ vine_status validate_options(const vine_graph* g, vine_options& o) {
     set_error("vine_solver_create: hash_slots must be 0 or a power of two in [16, %d]", dev::HASH_MAX_SLOTS);
     return VINE_E_ARG;
   }
+  if (o.lagrange_bound != 0 && o.lagrange_bound != 1) {
+    set_error("vine_solver_create: lagrange_bound must be 0 or 1");
+    return VINE_E_ARG;
+  }
   if (!(o.near_windows >= 0.f) || o.near_windows > 64.f) {
     set_error("vine_solver_create: near_windows must lie in [0, 64]");
     return VINE_E_ARG;
@@ -477,7 +483,7 @@ void init_fields(vine_solver* s, const vine_graph* g, const vine_options& o) {
 // and the two bound-search queue entries.
 size_t per_qv_bytes(const vine_options& o) {
   return sizeof(int2) + sizeof(unsigned long long) +
-         (o.prune == 2 ? sizeof(float) + sizeof(unsigned long long) + sizeof(uint32_t) + 2 * sizeof(int32_t) : 0);
+         (o.prune == 2 ? sizeof(float2) + sizeof(unsigned long long) + sizeof(uint32_t) + 2 * sizeof(int32_t) : 0);
 }
 
 struct Caps {
@@ -530,7 +536,8 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
   ALLOC(s->d_pset, sizeof(int2) * BN);
   ALLOC(s->d_cand_head, sizeof(unsigned long long) * BN);
   if (o.prune == 2) {
-    ALLOC(s->d_lbl, sizeof(float) * BN);
+    ALLOC(s->d_lbl, sizeof(float2) * BN);
+    ALLOC(s->d_qlk, sizeof(float) * B);
     ALLOC(s->d_lbd, sizeof(unsigned long long) * BN);
     ALLOC(s->d_inq, sizeof(uint32_t) * BN);
     ALLOC(s->d_wq, sizeof(int32_t) * 2 * BN);
@@ -615,6 +622,8 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
   c.pset = s->d_pset;
   c.cand_head = s->d_cand_head;
   c.lbl = s->d_lbl;
+  c.q_lk = s->d_qlk;
+  c.lagrange = (o.prune == 2 && o.lagrange_bound) ? 1 : 0;
   c.lbd = s->d_lbd;
   c.inq = s->d_inq;
   c.wq = s->d_wq;
@@ -1010,7 +1019,7 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
   }
   if (s->opt.prune == 2) {
     vine_status fs = fill_f32(s, (float*)s->d_lbd, dev::FINF, 2 * BN);
-    if (fs == VINE_OK) fs = fill_f32(s, s->d_lbl, dev::FINF, BN);
+    if (fs == VINE_OK) fs = fill_f32(s, (float*)s->d_lbl, dev::FINF, 2 * BN);
     if (fs != VINE_OK) return fs;
     HIP_TRY(hipMemsetAsync(s->d_inq, 0, sizeof(uint32_t) * BN, s->stream));
     vine_status ss = run_bounds(s, nb, st);

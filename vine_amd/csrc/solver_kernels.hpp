@@ -106,7 +106,12 @@ struct Ctx {
   // per (query, vertex) candidate list head: chunk index << 32 | records
   // claimed in that chunk (may exceed CCHUNK; ~0 = empty)
   unsigned long long* cand_head;
-  float* lbl;
+  // {lbl, lbw}: lbl = length to t (restricted to lbd <= bound); lbw = the
+  // Lagrangian distance to t under weights LSCALE * len + q_lk[q] * delay
+  // (same restriction; +inf where not searched or q_lk[q] = 0)
+  float2* lbl;
+  float* q_lk;       // [B] Lagrange multiplier x LSCALE of the query (0 = none)
+  int32_t lagrange;  // compute lbw in the bound searches
   unsigned long long* lbd;  // packed (delay bits << 32 | len bits): lexicographic
                             // (delay, len) distance to t; the len part is the
                             // length of a minimum-delay path
@@ -386,6 +391,28 @@ __device__ __forceinline__ float hi_of(unsigned long long p) { return __uint_as_
 __device__ __forceinline__ float lo_of(unsigned long long p) { return __uint_as_float((uint32_t)p); }
 __device__ __forceinline__ float lbd_of(const Ctx& c, long long qv) { return hi_of(c.lbd[qv]); }
 
+// Key and lower bound on the final cost of any feasible completion of a label
+// (d, delay) at a vertex with bounds lw = {lbl, lbw}, for a query with
+// multiplier k / LSCALE and delay bound bnd.  Two bounds, the larger is kept:
+//   d + lbl                                  (shortest remaining length)
+//   d + (lbw + k * (delay - bnd)) / LSCALE   (Lagrangian relaxation: a completion
+//     p with delay(p) <= bnd - delay has LSCALE*len(p) + k*delay(p) >= lbw, so
+//     LSCALE*len(p) >= lbw - k*(bnd - delay))
+// With integer weights the final cost is an integer, so the second bound is
+// rounded up; it is evaluated in double, where every term is exact (lbw is
+// used only below 2^24, where the search computed it exactly).  The maximum
+// of two consistent bounds is consistent: a child's key is never below its
+// parent's.
+constexpr float LSCALE = 8.f;
+__device__ __forceinline__ float lower_key(float d, float delay, float2 lw, float k, float bnd) {
+  float key = d + lw.x;
+  if (lw.y < EXACT_LIMIT) {
+    const double r = ceil(((double)lw.y + (double)k * ((double)delay - (double)bnd)) * (1.0 / (double)LSCALE));
+    key = fmaxf(key, __double2float_rd((double)d + r));
+  }
+  return key;
+}
+
 __device__ __forceinline__ int far_bucket(const Ctx& c, int q, float key) {
   float t = (key - c.far_base[q]) / c.far_bw[q];
   if (!(t < (float)(NBK - 1))) return NBK - 1;
@@ -439,7 +466,7 @@ __global__ void k_init_batch(Ctx c, int nq) {
     int s = c.q_src[q];
     long long qs = (long long)q * c.n + s;
     float key = 0.f;
-    if (PRUNE == 2) key = c.lbl[qs];
+    if (PRUNE == 2) key = lower_key(0.f, 0.f, c.lbl[qs], c.q_lk[q], c.q_bound[q]);
     c.labs[q] = make_int4(s, __float_as_int(0.f), __float_as_int(0.f), -1);
     c.lmeta[q] = make_int2(q, __float_as_int(key));
     c.dead[q] = 0;
@@ -513,10 +540,20 @@ __global__ void k_init_batch(Ctx c, int nq) {
 //   WHICH = 1: len to t, restricted to vertices with lbd <= bound: only those
 //   can lie on a feasible completion, so the restricted distance is a valid
 //   lower bound for every label that passes the delay test.
+//   WHICH = 3: the Lagrangian distance lbw to t under the integer weights
+//   LSCALE * len + lk * delay, with the same restriction.  Sums below 2^24 are
+//   exact; a larger value only arises from sums that are, and is not used.
 constexpr int BOUND_BLOCK = 512;
 
+// distance field of search WHICH at (query, vertex) qv
 template <int WHICH>
-__device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* s_min, int& relaxed, int& rounds) {
+__device__ __forceinline__ float* bound_dist(const Ctx& c, long long qv) {
+  return WHICH == 3 ? &c.lbl[qv].y : &c.lbl[qv].x;
+}
+
+template <int WHICH>
+__device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* s_min, int& relaxed, int& rounds,
+                          float lk = 0.f) {
   const int n = c.n;
   const long long qb = (long long)q * n;
   int32_t* que[2] = {c.wq + 2 * qb, c.wq + 2 * qb + n};
@@ -528,7 +565,7 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
   if (tid == 0) {
     const int root = (WHICH == 2) ? c.q_src[q] : c.q_dst[q];
     if (WHICH == 0) c.lbd[qb + root] = pack2(0.f, 0.f);
-    else c.lbl[qb + root] = 0.f;
+    else *bound_dist<WHICH>(c, qb + root) = 0.f;
     c.inq[qb + root] = 1;
     que[0][0] = root;
     s_n[0] = 1;
@@ -561,7 +598,7 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
         u = qin[i];
         const long long qv = qb + u;
         if (WHICH == 0) du = hi_of(__hip_atomic_load(&c.lbd[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        else du = load_agent(&c.lbl[qv]);
+        else du = load_agent(bound_dist<WHICH>(c, qv));
         if (du > lim_key) {
           carry = true;
         } else {
@@ -577,7 +614,7 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
             du = hi_of(pk);
             dl2 = lo_of(pk);
           } else {
-            du = load_agent(&c.lbl[qv]);
+            du = load_agent(bound_dist<WHICH>(c, qv));
           }
           row = rp[u];
           deg = rp[u + 1] - row;
@@ -603,14 +640,17 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
           float nd;
           if (WHICH == 2) {
             nd = odu + __int_as_float(e.z);
-            if (nd <= bnd && nd < c.lbl[wv]) {
-              const unsigned old = atomicMin((unsigned*)&c.lbl[wv], __float_as_uint(nd));
+            float* dp = bound_dist<2>(c, wv);
+            if (nd <= bnd && nd < *dp) {
+              const unsigned old = atomicMin((unsigned*)dp, __float_as_uint(nd));
               lowered = __float_as_uint(nd) < old;
             }
-          } else if (WHICH == 1) {
-            nd = odu + __int_as_float(e.y);
-            if (lbd_of(c, wv) <= bnd && nd < c.lbl[wv]) {
-              const unsigned old = atomicMin((unsigned*)&c.lbl[wv], __float_as_uint(nd));
+          } else if (WHICH == 1 || WHICH == 3) {
+            if (WHICH == 1) nd = odu + __int_as_float(e.y);
+            else nd = odu + (LSCALE * __int_as_float(e.y) + lk * __int_as_float(e.z));
+            float* dp = bound_dist<WHICH>(c, wv);
+            if (lbd_of(c, wv) <= bnd && nd < *dp) {
+              const unsigned old = atomicMin((unsigned*)dp, __float_as_uint(nd));
               lowered = __float_as_uint(nd) < old;
             }
           } else {
@@ -619,7 +659,7 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
             // with the forward search done, only vertices with fwd + lbd <=
             // bound: every vertex of a feasible s-t path lies there, and so
             // does every vertex of such a vertex's shortest path to t
-            const float lim = c.ellipse ? bnd - c.lbl[wv] : bnd;
+            const float lim = c.ellipse ? bnd - c.lbl[wv].x : bnd;
             if (nd <= lim && pk < c.lbd[wv]) {
               const unsigned long long old = atomicMin(&c.lbd[wv], pk);
               lowered = pk < old;
@@ -645,6 +685,64 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
     cur = nxt;
     __syncthreads();
   }
+}
+
+// The multiplier of the Lagrangian bound, by the first wave of a bound-search
+// workgroup once lbl and lbd are final.  Two points of the query's (delay,
+// len) trade-off are known: a minimum-delay path (D0, UB), from lbd(s), and a
+// shortest path inside the bound region (D1, L1), whose delay is summed by
+// walking from s along tight arcs (lbl(v) = len + lbl(w), exact for integer
+// sums).  The slope between them, -(UB - L1) / (D1 - D0), is the multiplier
+// at which both paths cost the same; any non-negative multiplier gives a valid
+// bound, this one a tight bound for bounds between D0 and D1.  0 (no fourth
+// search) when the shortest path is feasible or as short as the minimum-delay
+// one, or when the walk does not reach t within WALK_MAX steps.
+constexpr int WALK_MAX = 1 << 17;
+constexpr float LK_MAX = 4096.f;
+
+__device__ float lagrange_multiplier(const Ctx& c, int q) {
+  const int lane = lane_id();
+  const long long qb = (long long)q * c.n;
+  const int s = c.q_src[q], t = c.q_dst[q];
+  const float bnd = c.q_bound[q];
+  const float L1 = c.lbl[qb + s].x;
+  const unsigned long long pk = c.lbd[qb + s];
+  const float D0 = hi_of(pk), UB = lo_of(pk);
+  if (!(L1 < FINF) || !(D0 <= bnd) || !(UB > L1)) return 0.f;
+  float D1 = 0.f;
+  int v = s;
+  float lv = L1;
+  for (int step = 0; v != t; step++) {
+    if (step >= WALK_MAX) return 0.f;
+    const int r0 = c.row_ptr[v], r1 = c.row_ptr[v + 1];
+    int next = -1;
+    float dnext = 0.f, lnext = 0.f;
+    for (int base = r0; base < r1 && next < 0; base += WAVE) {
+      const int p = base + lane;
+      int4 e = make_int4(0, 0, 0, 0);
+      float lw = FINF;
+      bool tight = false;
+      if (p < r1) {
+        e = c.edges[p];
+        lw = c.lbl[qb + e.x].x;
+        tight = lw < FINF && __int_as_float(e.y) + lw == lv;
+      }
+      const unsigned long long m = __ballot(tight);
+      if (m) {
+        const int l = __ffsll((long long)m) - 1;
+        next = __shfl(e.x, l, WAVE);
+        dnext = __shfl(__int_as_float(e.z), l, WAVE);
+        lnext = __shfl(lw, l, WAVE);
+      }
+    }
+    if (next < 0) return 0.f;
+    D1 += dnext;
+    v = next;
+    lv = lnext;
+  }
+  if (!(D1 > bnd) || !(D1 > D0)) return 0.f;
+  const float k = rintf(LSCALE * (UB - L1) / (D1 - D0));
+  return fminf(fmaxf(k, 1.f), LK_MAX);
 }
 
 // Workgroups are dispatched in blockIdx order, so the slots are taken
@@ -702,14 +800,32 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
   __syncthreads();
   if (c.ellipse) {
     // lbl held fwd: back to +inf for the len search
-    float* l = c.lbl + (long long)q * c.n;
-    for (int j = threadIdx.x; j < c.n; j += blockDim.x) l[j] = FINF;
+    float2* l = c.lbl + (long long)q * c.n;
+    for (int j = threadIdx.x; j < c.n; j += blockDim.x) l[j].x = FINF;
   }
   __threadfence();
   __syncthreads();
   wg_search<1>(c, q, delta_len, s_n, s_min, relaxed, r1);
+  int r3 = 0;
+  if (c.lagrange) {
+    __shared__ float s_lk;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x < WAVE) {
+      const float lk = lagrange_multiplier(c, q);
+      if (threadIdx.x == 0) {
+        s_lk = lk;
+        c.q_lk[q] = lk;
+      }
+    }
+    __syncthreads();
+    const float lk = s_lk;
+    if (lk > 0.f) wg_search<3>(c, q, LSCALE * delta_len + lk * delta_delay, s_n, s_min, relaxed, r3, lk);
+  } else if (threadIdx.x == 0) {
+    c.q_lk[q] = 0.f;
+  }
   wave_add(&c.ct->sssp_relaxed, relaxed);
-  if (threadIdx.x == 0) atomicMax(&c.ct->sround, r0 + r1 + r2);
+  if (threadIdx.x == 0) atomicMax(&c.ct->sround, r0 + r1 + r2 + r3);
 }
 
 // ---------------------------------------------------------------------------
@@ -1123,6 +1239,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
     float ld = lab.d, ldl = lab.delay;
     // the owner's query bound and best answer travel with the label
     const float obound = (i < fn && id >= 0) ? c.q_bound[q] : 0.f;
+    const float olk = (PRUNE == 2 && i < fn && id >= 0) ? c.q_lk[q] : 0.f;
     const float2 obest = (PRUNE != 0 && i < fn && id >= 0) ? c.q_best[q] : make_float2(0.f, 0.f);
     wave_flat(deg, row, [&](int owner, int pos, bool act) {
       // shuffles by every lane (a lane reads its owner's registers)
@@ -1131,6 +1248,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
       const int oid = __shfl(id, owner, WAVE);
       const float2 b = make_float2(__shfl(obest.x, owner, WAVE), __shfl(obest.y, owner, WAVE));
       const float bound = __shfl(obound, owner, WAVE);
+      const float lk = PRUNE == 2 ? __shfl(olk, owner, WAVE) : 0.f;
       bool ok = false;
       Label_t son;
       long long qv = 0;
@@ -1147,11 +1265,11 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
         const int2 ps = c.pset[qv];
         hw = __hip_atomic_load(&c.cand_head[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const float lbdv = PRUNE == 2 ? lbd_of(c, qv) : 0.f;
-        const float lblv = PRUNE == 2 ? c.lbl[qv] : 0.f;
+        const float2 lw = PRUNE == 2 ? c.lbl[qv] : make_float2(0.f, FINF);
         if (ok && PRUNE == 1 && son.vid != c.q_dst[oq] && lex_ge(son.d, son.delay, b.x, b.y)) ok = false;
         if (ok && PRUNE == 2) {
           const float rd = son.delay + lbdv;
-          if (rd > C.delay_L || lex_ge(son.d + lblv, rd, b.x, b.y)) ok = false;
+          if (rd > C.delay_L || lex_ge(lower_key(son.d, son.delay, lw, lk, bound), rd, b.x, b.y)) ok = false;
         }
         // read-only dominance check against the vertex's front (no writer
         // runs now); candidates of this round are checked against each
@@ -1278,7 +1396,8 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_merge(Ctx c) {
     int q = 0, v = 0, h = -1, cnt = 0, cap = 0, tq = -1;
     unsigned long long hw = HEAD_EMPTY;
     uint32_t off = 0;
-    float lb = 0.f;
+    float2 lb = make_float2(0.f, FINF);
+    float qlk = 0.f, qbnd = 0.f;
     if (have) {
       qv = c.touched[ti];
       q = (int)(qv / c.n);
@@ -1290,7 +1409,11 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_merge(Ctx c) {
       cnt = pset_cnt(ps);
       cap = pset_cap(ps);
       tq = c.q_dst[q];
-      if (PRUNE == 2) lb = c.lbl[qv];
+      if (PRUNE == 2) {
+        lb = c.lbl[qv];
+        qlk = c.q_lk[q];
+        qbnd = c.q_bound[q];
+      }
     }
     if (have && gl == 0) c.cand_head[qv] = HEAD_EMPTY;
     // the front's last entry stays in registers across the candidates
@@ -1457,7 +1580,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_merge(Ctx c) {
             n_ins++;
             c.pool_cd[off + start] = make_float2(cd, cdl);
             c.pool_id[off + start] = (int)lid;
-            float key = LT::key(cl) + lb;
+            const float key = PRUNE == 2 ? lower_key(cd, cdl, lb, qlk, qbnd) : LT::key(cl);
             c.labs[lid] = make_int4(v, __float_as_int(cd), __float_as_int(cdl), par);
             c.lmeta[lid] = make_int2(q, __float_as_int(key));
             c.dead[lid] = 0;

@@ -201,6 +201,7 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
   const long long qb = (long long)q * c.n;
   const int tq = c.q_dst[q];
   const float bnd = c.q_bound[q];
+  const float lk = c.q_lk[q];
   int4* nearb[2] = {c.wnear + (2LL * q) * c.wn_cap, c.wnear + (2LL * q + 1) * c.wn_cap};
   for (int i = tid; i < HSLOTS; i += WG_BLOCK) {
     S.key[i] = -1;
@@ -308,7 +309,7 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
         const long long qv = qb + son.vid;
         const float srd = son.delay + lbd_of(c, qv);
         if (srd > C.delay_L) return;
-        if (lex_ge(son.d + c.lbl[qv], srd, best.x, best.y)) return;
+        if (lex_ge(lower_key(son.d, son.delay, c.lbl[qv], lk, bnd), srd, best.x, best.y)) return;
         const int2 ps = c.pset[qv];
         if (front_dominates<Label_t>(c.pool_cd + (uint32_t)ps.x, pset_cnt(ps), son.vid, son)) return;
         n_cand++;
@@ -348,7 +349,7 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
         const int2 ps = c.pset[qv];
         uint32_t off = (uint32_t)ps.x;
         int cnt = pset_cnt(ps), cap = pset_cap(ps);
-        const float lb = c.lbl[qv];
+        const float2 lb = c.lbl[qv];
         const float lbdv = lbd_of(c, qv);
         for (int idx = S.head[h]; idx >= 0;) {
           const int4 r = S.rec[idx];
@@ -458,7 +459,7 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
                 __builtin_amdgcn_wave_barrier();
               }
             }
-            nkey = LT::key(cl) + lb;
+            nkey = lower_key(cd, cdl, lb, lk, bnd);
             if (gl == 0) {
               my_ins++;
               c.pool_cd[off + start] = make_float2(cd, cdl);

@@ -52,6 +52,7 @@ class Options(ctypes.Structure):
         ("labels_per_wave", ctypes.c_int32),
         ("concurrency", ctypes.c_int32),
         ("near_windows", ctypes.c_float),
+        ("lagrange_bound", ctypes.c_int32),
     ]
 
 
