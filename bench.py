@@ -160,7 +160,9 @@ def main():
     ap.add_argument("--use-graph", type=int, default=None, help="override: replay rounds as a HIP graph (0/1)")
     ap.add_argument("--near-windows", type=float, default=None, help="override: near-tier reach in windows")
     ap.add_argument("--hash-slots", type=int, default=None, help="override: LDS candidate table slots (0 = off)")
-    ap.add_argument("--lagrange-bound", type=int, default=None, help="override: Lagrangian bound search (0/1)")
+    ap.add_argument("--lagrange-bound", type=int, default=None,
+                    help="override: Lagrange multipliers tried per query (0 = no Lagrangian bound)")
+    ap.add_argument("--bound-ball", type=int, default=None, help="override: reverse bound searches over whole balls")
     ap.add_argument("--plain-delta", type=float, default=12.0, help="window of the plain single-query solver")
     ap.add_argument("--plain-queries", type=int, default=100, help="per bound factor")
     ap.add_argument("--cpu-queries", type=int, default=32, help="batched-workload queries in the CPU sample")
@@ -220,6 +222,8 @@ def main():
         launch["hash_slots"] = args.hash_slots
     if args.lagrange_bound is not None:
         launch["lagrange_bound"] = args.lagrange_bound
+    if args.bound_ball is not None:
+        launch["bound_ball"] = args.bound_ball
     log("launch options: %s (%s)" % (launch, source))
     solver = S.Solver(dg, time_kernels=TIME_EVERY, **base, **launch)
     if policy == S.DELTA_MLP:

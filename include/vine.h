@@ -126,10 +126,11 @@ typedef struct {
   float near_windows;       /* the near tier (carried labels scanned again every round) holds keys up
                                to this many windows past the query's front; later keys wait in the far
                                tier's key buckets until the front comes within that reach (0 = 1) */
-  int32_t lagrange_bound;   /* with VINE_PRUNE_BOUNDS: 1 (vine_options_default) adds a fourth reverse search per
-                               query, under the weights 8 len + k delay with k / 8 the slope between a
-                               minimum-delay and a shortest path, whose Lagrangian bound on the remaining length
-                               keys and prunes labels together with the plain length bound (same answers);
+  int32_t lagrange_bound;   /* with VINE_PRUNE_BOUNDS, n in 1..8: up to n more reverse searches per query (4 in
+                               vine_options_default) under the weights 256 len + k delay, k / 256 a Lagrange
+                               multiplier found by hull iteration from a minimum-delay and a shortest path; the
+                               best one's bound on the remaining length, d + (lbw + k (delay - bound)) / 256,
+                               keys and prunes labels together with the plain length bound (same answers).
                                0: the length bound alone */
 } vine_options;
 

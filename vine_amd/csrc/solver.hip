@@ -222,7 +222,7 @@ void vine_options_default(vine_options* o) {
   o->grid_blocks = 0;
   o->hash_slots = 0;
   o->ar_max_labels = 0.0;
-  o->lagrange_bound = 1;
+  o->lagrange_bound = 4;
 }
 
 int32_t vine_device_count(void) {
@@ -427,8 +427,8 @@ vine_status validate_options(const vine_graph* g, vine_options& o) {
     set_error("vine_solver_create: hash_slots must be 0 or a power of two in [16, %d]", dev::HASH_MAX_SLOTS);
     return VINE_E_ARG;
   }
-  if (o.lagrange_bound != 0 && o.lagrange_bound != 1) {
-    set_error("vine_solver_create: lagrange_bound must be 0 or 1");
+  if (o.lagrange_bound < 0 || o.lagrange_bound > 8) {
+    set_error("vine_solver_create: lagrange_bound must lie in [0, 8]");
     return VINE_E_ARG;
   }
   if (!(o.near_windows >= 0.f) || o.near_windows > 64.f) {
@@ -623,7 +623,7 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
   c.cand_head = s->d_cand_head;
   c.lbl = s->d_lbl;
   c.q_lk = s->d_qlk;
-  c.lagrange = (o.prune == 2 && o.lagrange_bound) ? 1 : 0;
+  c.lagrange = o.prune == 2 ? o.lagrange_bound : 0;
   c.lbd = s->d_lbd;
   c.inq = s->d_inq;
   c.wq = s->d_wq;

@@ -111,7 +111,7 @@ struct Ctx {
   // (same restriction; +inf where not searched or q_lk[q] = 0)
   float2* lbl;
   float* q_lk;       // [B] Lagrange multiplier x LSCALE of the query (0 = none)
-  int32_t lagrange;  // compute lbw in the bound searches
+  int32_t lagrange;  // multipliers tried by the Lagrangian bound searches (0: no lbw)
   unsigned long long* lbd;  // packed (delay bits << 32 | len bits): lexicographic
                             // (delay, len) distance to t; the len part is the
                             // length of a minimum-delay path
@@ -398,19 +398,25 @@ __device__ __forceinline__ float lbd_of(const Ctx& c, long long qv) { return hi_
 //   d + (lbw + k * (delay - bnd)) / LSCALE   (Lagrangian relaxation: a completion
 //     p with delay(p) <= bnd - delay has LSCALE*len(p) + k*delay(p) >= lbw, so
 //     LSCALE*len(p) >= lbw - k*(bnd - delay))
-// With integer weights the final cost is an integer, so the second bound is
-// rounded up; it is evaluated in double, where every term is exact (lbw is
-// used only below 2^24, where the search computed it exactly).  The maximum
-// of two consistent bounds is consistent: a child's key is never below its
-// parent's.
-constexpr float LSCALE = 8.f;
+// lbw is an integer held in the bits of lw.y (LBW_NONE: not computed).  With
+// integer weights the final cost is an integer, so the second bound is rounded
+// up; it is evaluated in double, where every term is exact.  The maximum of two
+// consistent bounds is consistent: a child's key is never below its parent's.
+constexpr float LSCALE = 256.f;
+constexpr uint32_t LBW_NONE = 0x7F800000u;  // the bits of +inf, which the batch reset writes
 __device__ __forceinline__ float lower_key(float d, float delay, float2 lw, float k, float bnd) {
   float key = d + lw.x;
-  if (lw.y < EXACT_LIMIT) {
-    const double r = ceil(((double)lw.y + (double)k * ((double)delay - (double)bnd)) * (1.0 / (double)LSCALE));
+  const uint32_t wb = __float_as_uint(lw.y);
+  if (wb < LBW_NONE) {
+    const double r = ceil(((double)wb + (double)k * ((double)delay - (double)bnd)) * (1.0 / (double)LSCALE));
     key = fmaxf(key, __double2float_rd((double)d + r));
   }
   return key;
+}
+
+// Lagrangian arc weight LSCALE * len + k * delay of integer-valued len and delay
+__device__ __forceinline__ unsigned long long lagr_weight(int4 e, float k) {
+  return (unsigned long long)(LSCALE * __int_as_float(e.y)) + (unsigned long long)k * (unsigned long long)__int_as_float(e.z);
 }
 
 __device__ __forceinline__ int far_bucket(const Ctx& c, int q, float key) {
@@ -541,8 +547,9 @@ __global__ void k_init_batch(Ctx c, int nq) {
 //   can lie on a feasible completion, so the restricted distance is a valid
 //   lower bound for every label that passes the delay test.
 //   WHICH = 3: the Lagrangian distance lbw to t under the integer weights
-//   LSCALE * len + lk * delay, with the same restriction.  Sums below 2^24 are
-//   exact; a larger value only arises from sums that are, and is not used.
+//   LSCALE * len + lk * delay, with the same restriction, as unsigned integers
+//   in the bits of lbl.y (sums saturate at LBW_NONE, which is never stored);
+//   the window works on their float values.
 constexpr int BOUND_BLOCK = 512;
 
 // distance field of search WHICH at (query, vertex) qv
@@ -593,11 +600,13 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
       const int i = base + lane;
       int deg = 0, row = 0, u = 0;
       float du = 0.f, dl2 = 0.f;  // distance (delay for WHICH 0 and 2, len for 1); len tie key
+      uint32_t dub = 0;           // WHICH 3: the integer distance (du is its float value)
       bool carry = false;
       if (i < ncur) {
         u = qin[i];
         const long long qv = qb + u;
         if (WHICH == 0) du = hi_of(__hip_atomic_load(&c.lbd[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        else if (WHICH == 3) du = (float)__float_as_uint(load_agent(bound_dist<WHICH>(c, qv)));
         else du = load_agent(bound_dist<WHICH>(c, qv));
         if (du > lim_key) {
           carry = true;
@@ -613,6 +622,9 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
             const unsigned long long pk = __hip_atomic_load(&c.lbd[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             du = hi_of(pk);
             dl2 = lo_of(pk);
+          } else if (WHICH == 3) {
+            dub = __float_as_uint(load_agent(bound_dist<WHICH>(c, qv)));
+            du = (float)dub;
           } else {
             du = load_agent(bound_dist<WHICH>(c, qv));
           }
@@ -629,6 +641,7 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
       wave_flat(deg, row, [&](int owner, int pos, bool act) {
         const float odu = __shfl(du, owner, WAVE);
         const float odl = __shfl(dl2, owner, WAVE);
+        const uint32_t odub = WHICH == 3 ? (uint32_t)__shfl((int)dub, owner, WAVE) : 0u;
         bool push = false;
         int w = 0;
         if (act) {
@@ -645,13 +658,21 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
               const unsigned old = atomicMin((unsigned*)dp, __float_as_uint(nd));
               lowered = __float_as_uint(nd) < old;
             }
-          } else if (WHICH == 1 || WHICH == 3) {
-            if (WHICH == 1) nd = odu + __int_as_float(e.y);
-            else nd = odu + (LSCALE * __int_as_float(e.y) + lk * __int_as_float(e.z));
-            float* dp = bound_dist<WHICH>(c, wv);
+          } else if (WHICH == 1) {
+            nd = odu + __int_as_float(e.y);
+            float* dp = bound_dist<1>(c, wv);
             if (lbd_of(c, wv) <= bnd && nd < *dp) {
               const unsigned old = atomicMin((unsigned*)dp, __float_as_uint(nd));
               lowered = __float_as_uint(nd) < old;
+            }
+          } else if (WHICH == 3) {
+            const unsigned long long s64 = (unsigned long long)odub + lagr_weight(e, lk);
+            const uint32_t ndb = s64 < (unsigned long long)LBW_NONE ? (uint32_t)s64 : LBW_NONE;
+            nd = (float)ndb;
+            unsigned* dp = (unsigned*)bound_dist<3>(c, wv);
+            if (lbd_of(c, wv) <= bnd && ndb < *dp) {
+              const unsigned old = atomicMin(dp, ndb);
+              lowered = ndb < old;
             }
           } else {
             nd = odu + __int_as_float(e.z);
@@ -687,62 +708,149 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
   }
 }
 
-// The multiplier of the Lagrangian bound, by the first wave of a bound-search
-// workgroup once lbl and lbd are final.  Two points of the query's (delay,
-// len) trade-off are known: a minimum-delay path (D0, UB), from lbd(s), and a
-// shortest path inside the bound region (D1, L1), whose delay is summed by
-// walking from s along tight arcs (lbl(v) = len + lbl(w), exact for integer
-// sums).  The slope between them, -(UB - L1) / (D1 - D0), is the multiplier
-// at which both paths cost the same; any non-negative multiplier gives a valid
-// bound, this one a tight bound for bounds between D0 and D1.  0 (no fourth
-// search) when the shortest path is feasible or as short as the minimum-delay
-// one, or when the walk does not reach t within WALK_MAX steps.
+// The multiplier of the Lagrangian bound, chosen by a few steps of the
+// classical hull iteration once lbl and lbd are final.  Two points of the
+// query's (delay, len) trade-off start it: a minimum-delay path (D0, UB), from
+// lbd(s), and a shortest path inside the bound region (D1, L1), whose delay is
+// summed by walking from s along tight arcs.  The slope between the feasible
+// point P0 and the infeasible point P1 is the multiplier at which both cost the
+// same; a search under it yields a multiplier-shortest path, walked the same
+// way, which replaces P0 or P1 by the side of the bound it falls on.  The
+// multiplier whose bound at s is largest is kept.  Any multiplier gives a valid
+// bound, so the choice only affects how much is pruned.  No Lagrangian search
+// (k = 0) when the shortest path is feasible or as short as the minimum-delay
+// one, or when a walk does not reach t within WALK_MAX steps.
 constexpr int WALK_MAX = 1 << 17;
-constexpr float LK_MAX = 4096.f;
+constexpr float LK_MAX = 65536.f;
 
-__device__ float lagrange_multiplier(const Ctx& c, int q) {
+struct LagrState {
+  double best_lb;     // largest bound at s so far
+  float p0d, p0l;     // feasible point (delay, len)
+  float p1d, p1l;     // infeasible point
+  float k;            // multiplier to search next (LSCALE units)
+  float best_k;       // multiplier of best_lb (0: none)
+  float last_k;       // multiplier whose lbw is in memory
+  int stop;
+};
+
+// Walk from s to t along tight arcs of lbl (WHICH 1) or of lbw under
+// multiplier k (WHICH 3), by one wave; sums of len and delay, false if t is
+// not reached.
+template <int WHICH>
+__device__ bool walk_tight(const Ctx& c, int q, float k, double& sum_len, double& sum_delay) {
   const int lane = lane_id();
   const long long qb = (long long)q * c.n;
-  const int s = c.q_src[q], t = c.q_dst[q];
-  const float bnd = c.q_bound[q];
-  const float L1 = c.lbl[qb + s].x;
-  const unsigned long long pk = c.lbd[qb + s];
-  const float D0 = hi_of(pk), UB = lo_of(pk);
-  if (!(L1 < FINF) || !(D0 <= bnd) || !(UB > L1)) return 0.f;
-  float D1 = 0.f;
-  int v = s;
-  float lv = L1;
+  const int t = c.q_dst[q];
+  int v = c.q_src[q];
+  float lv = c.lbl[qb + v].x;
+  uint32_t wv = __float_as_uint(c.lbl[qb + v].y);
+  sum_len = 0.0;
+  sum_delay = 0.0;
   for (int step = 0; v != t; step++) {
-    if (step >= WALK_MAX) return 0.f;
+    if (step >= WALK_MAX) return false;
     const int r0 = c.row_ptr[v], r1 = c.row_ptr[v + 1];
     int next = -1;
-    float dnext = 0.f, lnext = 0.f;
+    float nl = 0.f, nd = 0.f, nlv = 0.f;
+    uint32_t nwv = 0;
     for (int base = r0; base < r1 && next < 0; base += WAVE) {
       const int p = base + lane;
       int4 e = make_int4(0, 0, 0, 0);
-      float lw = FINF;
+      float2 lw = make_float2(FINF, FINF);
       bool tight = false;
       if (p < r1) {
         e = c.edges[p];
-        lw = c.lbl[qb + e.x].x;
-        tight = lw < FINF && __int_as_float(e.y) + lw == lv;
+        lw = c.lbl[qb + e.x];
+        if (WHICH == 1) {
+          tight = lw.x < FINF && __int_as_float(e.y) + lw.x == lv;
+        } else {
+          const uint32_t w = __float_as_uint(lw.y);
+          tight = w < LBW_NONE && (unsigned long long)w + lagr_weight(e, k) == (unsigned long long)wv;
+        }
       }
       const unsigned long long m = __ballot(tight);
       if (m) {
         const int l = __ffsll((long long)m) - 1;
         next = __shfl(e.x, l, WAVE);
-        dnext = __shfl(__int_as_float(e.z), l, WAVE);
-        lnext = __shfl(lw, l, WAVE);
+        nl = __shfl(__int_as_float(e.y), l, WAVE);
+        nd = __shfl(__int_as_float(e.z), l, WAVE);
+        nlv = __shfl(lw.x, l, WAVE);
+        nwv = (uint32_t)__shfl((int)__float_as_uint(lw.y), l, WAVE);
       }
     }
-    if (next < 0) return 0.f;
-    D1 += dnext;
+    if (next < 0) return false;
+    sum_len += nl;
+    sum_delay += nd;
     v = next;
-    lv = lnext;
+    lv = nlv;
+    wv = nwv;
   }
-  if (!(D1 > bnd) || !(D1 > D0)) return 0.f;
-  const float k = rintf(LSCALE * (UB - L1) / (D1 - D0));
+  return true;
+}
+
+__device__ __forceinline__ float hull_multiplier(const LagrState& ls) {
+  const float k = rintf(LSCALE * (ls.p0l - ls.p1l) / (ls.p1d - ls.p0d));
   return fminf(fmaxf(k, 1.f), LK_MAX);
+}
+
+// first wave: the starting points and the first multiplier
+__device__ void lagr_begin(const Ctx& c, int q, LagrState& ls) {
+  const long long qb = (long long)q * c.n;
+  const int s = c.q_src[q];
+  const float bnd = c.q_bound[q];
+  const float L1 = c.lbl[qb + s].x;
+  const unsigned long long pk = c.lbd[qb + s];
+  const float D0 = hi_of(pk), UB = lo_of(pk);
+  double wl = 0.0, wd = 0.0;
+  bool go = L1 < FINF && D0 <= bnd && UB > L1 && walk_tight<1>(c, q, 0.f, wl, wd);
+  go = go && wd > (double)bnd && wd > (double)D0;
+  if (lane_id() == 0) {
+    ls.best_lb = -1e300;
+    ls.best_k = 0.f;
+    ls.last_k = 0.f;
+    ls.p0d = D0;
+    ls.p0l = UB;
+    ls.p1d = (float)wd;
+    ls.p1l = L1;
+    ls.stop = go ? 0 : 1;
+    ls.k = go ? hull_multiplier(ls) : 0.f;
+  }
+}
+
+// first wave, after the search under ls.k: keep the better bound, move the
+// hull point, next multiplier (or stop)
+__device__ void lagr_step(const Ctx& c, int q, LagrState& ls) {
+  const long long qb = (long long)q * c.n;
+  const int s = c.q_src[q];
+  const float bnd = c.q_bound[q];
+  const float k = ls.k;
+  const uint32_t ws = __float_as_uint(c.lbl[qb + s].y);
+  double wl = 0.0, wd = 0.0;
+  const bool ok = ws < LBW_NONE && walk_tight<3>(c, q, k, wl, wd);
+  if (lane_id() == 0) {
+    ls.last_k = k;
+    if (ws < LBW_NONE) {
+      const double lb = ((double)ws - (double)k * (double)bnd) / (double)LSCALE;
+      if (lb > ls.best_lb) {
+        ls.best_lb = lb;
+        ls.best_k = k;
+      }
+    }
+    if (!ok) {
+      ls.stop = 1;
+      return;
+    }
+    if (wd > (double)bnd) {
+      ls.p1d = (float)wd;
+      ls.p1l = (float)wl;
+    } else {
+      ls.p0d = (float)wd;
+      ls.p0l = (float)wl;
+    }
+    float nk = k;
+    if (ls.p1d > ls.p0d && ls.p0l > ls.p1l) nk = hull_multiplier(ls);
+    ls.stop = (nk == k || nk == ls.best_k) ? 1 : 0;
+    ls.k = nk;
+  }
 }
 
 // Workgroups are dispatched in blockIdx order, so the slots are taken
@@ -808,19 +916,42 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
   wg_search<1>(c, q, delta_len, s_n, s_min, relaxed, r1);
   int r3 = 0;
   if (c.lagrange) {
-    __shared__ float s_lk;
+    __shared__ LagrState ls;
     __threadfence();
     __syncthreads();
-    if (threadIdx.x < WAVE) {
-      const float lk = lagrange_multiplier(c, q);
-      if (threadIdx.x == 0) {
-        s_lk = lk;
-        c.q_lk[q] = lk;
+    if (threadIdx.x < WAVE) lagr_begin(c, q, ls);
+    __syncthreads();
+    float2* lq = c.lbl + (long long)q * c.n;
+    for (int it = 0; it < c.lagrange && !ls.stop; it++) {
+      const float k = ls.k;
+      if (it > 0) {
+        // the last search's lbw is replaced
+        for (int j = threadIdx.x; j < c.n; j += blockDim.x) lq[j].y = __uint_as_float(LBW_NONE);
+        __threadfence();
+        __syncthreads();
+      }
+      int rr = 0;
+      wg_search<3>(c, q, LSCALE * delta_len + k * delta_delay, s_n, s_min, relaxed, rr, k);
+      r3 += rr;
+      __threadfence();
+      __syncthreads();
+      if (threadIdx.x < WAVE) lagr_step(c, q, ls);
+      __syncthreads();
+    }
+    const float bk = ls.best_k;
+    if (bk != ls.last_k) {
+      // lbw in memory belongs to another multiplier: cleared, and searched
+      // again under the best one if there is one
+      for (int j = threadIdx.x; j < c.n; j += blockDim.x) lq[j].y = __uint_as_float(LBW_NONE);
+      __threadfence();
+      __syncthreads();
+      if (bk > 0.f) {
+        int rr = 0;
+        wg_search<3>(c, q, LSCALE * delta_len + bk * delta_delay, s_n, s_min, relaxed, rr, bk);
+        r3 += rr;
       }
     }
-    __syncthreads();
-    const float lk = s_lk;
-    if (lk > 0.f) wg_search<3>(c, q, LSCALE * delta_len + lk * delta_delay, s_n, s_min, relaxed, r3, lk);
+    if (threadIdx.x == 0) c.q_lk[q] = bk;
   } else if (threadIdx.x == 0) {
     c.q_lk[q] = 0.f;
   }

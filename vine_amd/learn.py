@@ -50,11 +50,15 @@ def log(msg):
     print("[learn] " + msg, file=sys.stderr, flush=True)
 
 
+LAGRANGE = None  # multiplier searches of the bound searches (None: the library default)
+
+
 def solver_for(dg, batch, policy, delta, trace=0, **kw):
+    extra = {} if LAGRANGE is None else {"lagrange_bound": LAGRANGE}
     return S.Solver(dg, max_batch=batch, prune=S.PRUNE_BOUNDS, delta_policy=policy, delta_fixed=delta,
                     rounds_per_sync=16, record_trace=trace, label_capacity=kw.get("label_capacity", 600_000_000),
                     pool_capacity=kw.get("pool_capacity", 1_200_000_000), cand_capacity=100_000_000,
-                    frontier_capacity=400_000_000)
+                    frontier_capacity=400_000_000, **extra)
 
 
 def collect(dg, q, batch, candidates=CANDIDATES):
@@ -190,7 +194,11 @@ def main():
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--dagger", type=int, default=2)
     ap.add_argument("--wide", action="store_true", help="also offer wide windows to the heaviest queries")
+    ap.add_argument("--lagrange-bound", type=int, default=None,
+                    help="multiplier searches of the bound searches (default: the library's)")
     args = ap.parse_args()
+    global LAGRANGE
+    LAGRANGE = args.lagrange_bound
     g = W.knn(264346, seed=0)
     q = W.hop_queries(g, args.queries, seed=args.seed)
     log("training queries: %d (seed %d)" % (len(q), args.seed))
@@ -223,6 +231,7 @@ def main():
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.save(args.out, params)
     meta = {"queries": len(q), "seed": args.seed, "candidates": wins, "round_cost": round_cost,
+            "lagrange_bound": LAGRANGE,
             "rows": int(X.shape[0]), "loss_curve": [float(c) for c in curve],
             "best_hist": [int(x) for x in np.bincount(best, minlength=len(wins))],
             "window_seconds": {str(d): runs[d]["seconds"] for d in runs}}
