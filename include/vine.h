@@ -126,12 +126,16 @@ typedef struct {
   float near_windows;       /* the near tier (carried labels scanned again every round) holds keys up
                                to this many windows past the query's front; later keys wait in the far
                                tier's key buckets until the front comes within that reach (0 = 1) */
-  int32_t lagrange_bound;   /* with VINE_PRUNE_BOUNDS, n in 1..8: up to n more reverse searches per query (4 in
-                               vine_options_default) under the weights 256 len + k delay, k / 256 a Lagrange
-                               multiplier found by hull iteration from a minimum-delay and a shortest path; the
-                               best one's bound on the remaining length, d + (lbw + k (delay - bound)) / 256,
-                               keys and prunes labels together with the plain length bound (same answers).
-                               0: the length bound alone */
+  int32_t lagrange_bound;   /* with VINE_PRUNE_BOUNDS, a Lagrangian bound on the remaining length: a reverse
+                               search per query under the integer weights 256 len + k delay gives lbw, and a
+                               label (d, delay) at v gets the bound d + (lbw(v) + k (delay - bound)) / 256,
+                               which keys and prunes it together with the plain length bound (same answers).
+                               k / 256 is a Lagrange multiplier found from a minimum-delay and a shortest path:
+                               0: no Lagrangian bound;
+                               1: one search, at the slope between those two paths;
+                               2 (vine_options_default): one search under four multipliers at once (0.25,
+                                  0.4, 0.55 and 0.75 x that slope), then one hull step; the best of the five;
+                               3..8: that many searches of the sequential hull iteration, the best kept */
 } vine_options;
 
 typedef struct {

@@ -222,7 +222,7 @@ void vine_options_default(vine_options* o) {
   o->grid_blocks = 0;
   o->hash_slots = 0;
   o->ar_max_labels = 0.0;
-  o->lagrange_bound = 4;
+  o->lagrange_bound = 2;
 }
 
 int32_t vine_device_count(void) {

@@ -558,9 +558,21 @@ __device__ __forceinline__ float* bound_dist(const Ctx& c, long long qv) {
   return WHICH == 3 ? &c.lbl[qv].y : &c.lbl[qv].x;
 }
 
+// WHICH = 4 keeps four Lagrangian distances per (query, vertex) in the label
+// search's pset and cand_head words, which are free while the bound searches
+// run (the workgroup restores them before it ends)
+__device__ __forceinline__ uint32_t* grid_field(const Ctx& c, long long qv, int j) {
+  return j < 2 ? (uint32_t*)&c.pset[qv] + j : (uint32_t*)&c.cand_head[qv] + (j - 2);
+}
+
+__device__ __forceinline__ uint32_t load_agent_u32p(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int WHICH>
 __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* s_min, int& relaxed, int& rounds,
-                          float lk = 0.f) {
+                          float lk = 0.f, float4 lk4 = make_float4(0.f, 0.f, 0.f, 0.f)) {
+  const float lks[4] = {lk4.x, lk4.y, lk4.z, lk4.w};
   const int n = c.n;
   const long long qb = (long long)q * n;
   int32_t* que[2] = {c.wq + 2 * qb, c.wq + 2 * qb + n};
@@ -572,6 +584,8 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
   if (tid == 0) {
     const int root = (WHICH == 2) ? c.q_src[q] : c.q_dst[q];
     if (WHICH == 0) c.lbd[qb + root] = pack2(0.f, 0.f);
+    else if (WHICH == 4)
+      for (int j = 0; j < 4; j++) *grid_field(c, qb + root, j) = 0u;
     else *bound_dist<WHICH>(c, qb + root) = 0.f;
     c.inq[qb + root] = 1;
     que[0][0] = root;
@@ -601,12 +615,14 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
       int deg = 0, row = 0, u = 0;
       float du = 0.f, dl2 = 0.f;  // distance (delay for WHICH 0 and 2, len for 1); len tie key
       uint32_t dub = 0;           // WHICH 3: the integer distance (du is its float value)
+      uint32_t dg[4] = {0u, 0u, 0u, 0u};  // WHICH 4: the four distances (du orders by the first)
       bool carry = false;
       if (i < ncur) {
         u = qin[i];
         const long long qv = qb + u;
         if (WHICH == 0) du = hi_of(__hip_atomic_load(&c.lbd[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         else if (WHICH == 3) du = (float)__float_as_uint(load_agent(bound_dist<WHICH>(c, qv)));
+        else if (WHICH == 4) du = (float)load_agent_u32p(grid_field(c, qv, 0));
         else du = load_agent(bound_dist<WHICH>(c, qv));
         if (du > lim_key) {
           carry = true;
@@ -625,6 +641,10 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
           } else if (WHICH == 3) {
             dub = __float_as_uint(load_agent(bound_dist<WHICH>(c, qv)));
             du = (float)dub;
+          } else if (WHICH == 4) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) dg[j] = load_agent_u32p(grid_field(c, qv, j));
+            du = (float)dg[0];
           } else {
             du = load_agent(bound_dist<WHICH>(c, qv));
           }
@@ -642,6 +662,9 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
         const float odu = __shfl(du, owner, WAVE);
         const float odl = __shfl(dl2, owner, WAVE);
         const uint32_t odub = WHICH == 3 ? (uint32_t)__shfl((int)dub, owner, WAVE) : 0u;
+        uint32_t odg[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) odg[j] = WHICH == 4 ? (uint32_t)__shfl((int)dg[j], owner, WAVE) : 0u;
         bool push = false;
         int w = 0;
         if (act) {
@@ -664,6 +687,21 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
             if (lbd_of(c, wv) <= bnd && nd < *dp) {
               const unsigned old = atomicMin((unsigned*)dp, __float_as_uint(nd));
               lowered = __float_as_uint(nd) < old;
+            }
+          } else if (WHICH == 4) {
+            nd = FINF;
+            if (lbd_of(c, wv) <= bnd) {
+#pragma unroll
+              for (int j = 0; j < 4; j++) {
+                const unsigned long long s64 = (unsigned long long)odg[j] + lagr_weight(e, lks[j]);
+                const uint32_t ndb = s64 < (unsigned long long)LBW_NONE ? (uint32_t)s64 : LBW_NONE;
+                if (j == 0) nd = (float)ndb;
+                uint32_t* fp = grid_field(c, wv, j);
+                if (ndb < *fp) {
+                  const unsigned old = atomicMin(fp, ndb);
+                  lowered = lowered || ndb < old;
+                }
+              }
             }
           } else if (WHICH == 3) {
             const unsigned long long s64 = (unsigned long long)odub + lagr_weight(e, lk);
@@ -733,17 +771,17 @@ struct LagrState {
   int stop;
 };
 
-// Walk from s to t along tight arcs of lbl (WHICH 1) or of lbw under
-// multiplier k (WHICH 3), by one wave; sums of len and delay, false if t is
-// not reached.
+// Walk from s to t along tight arcs of lbl (WHICH 1), of lbw under
+// multiplier k (WHICH 3) or of the four-multiplier search's field j (WHICH 4),
+// by one wave; sums of len and delay, false if t is not reached.
 template <int WHICH>
-__device__ bool walk_tight(const Ctx& c, int q, float k, double& sum_len, double& sum_delay) {
+__device__ bool walk_tight(const Ctx& c, int q, float k, double& sum_len, double& sum_delay, int j = 0) {
   const int lane = lane_id();
   const long long qb = (long long)q * c.n;
   const int t = c.q_dst[q];
   int v = c.q_src[q];
   float lv = c.lbl[qb + v].x;
-  uint32_t wv = __float_as_uint(c.lbl[qb + v].y);
+  uint32_t wv = WHICH == 4 ? *grid_field(c, qb + v, j) : __float_as_uint(c.lbl[qb + v].y);
   sum_len = 0.0;
   sum_delay = 0.0;
   for (int step = 0; v != t; step++) {
@@ -760,6 +798,7 @@ __device__ bool walk_tight(const Ctx& c, int q, float k, double& sum_len, double
       if (p < r1) {
         e = c.edges[p];
         lw = c.lbl[qb + e.x];
+        if (WHICH == 4) lw.y = __uint_as_float(*grid_field(c, qb + e.x, j));
         if (WHICH == 1) {
           tight = lw.x < FINF && __int_as_float(e.y) + lw.x == lv;
         } else {
@@ -892,6 +931,117 @@ __global__ void __launch_bounds__(1024) k_bound_order(Ctx c, int nq) {
   for (int i = threadIdx.x; i < nq; i += blockDim.x) c.q_order[i] = si[i];
 }
 
+// lagrange_bound = LAGR_GRID: the multiplier from one search under four
+// multipliers around the starting slope k0 (GRID_FRAC x k0, all relaxed in the
+// same rounds), the four multiplier-shortest paths walked by four waves at once,
+// then one hull step from the points nearest the bound on either side; the
+// multiplier with the largest bound at s is kept.  Two search passes instead of
+// one per multiplier tried.  Returns the rounds of the searches.
+constexpr int LAGR_GRID = 2;
+
+__device__ int lagr_grid(const Ctx& c, int q, LagrState& ls, float delta_len, float delta_delay, int* s_n,
+                         uint32_t* s_min, int& relaxed) {
+  __shared__ double s_glb[4];
+  __shared__ float s_gd[4], s_gl[4];
+  __shared__ int s_gok[4];
+  __shared__ float s_kh;
+  const int tid = threadIdx.x, wid = tid / WAVE;
+  const long long qb = (long long)q * c.n;
+  const int s = c.q_src[q];
+  const float bnd = c.q_bound[q];
+  const float k0 = ls.k;
+  const float4 lk4 = make_float4(fmaxf(1.f, rintf(0.25f * k0)), fmaxf(1.f, rintf(0.4f * k0)),
+                                 fmaxf(1.f, rintf(0.55f * k0)), fmaxf(1.f, rintf(0.75f * k0)));
+  const float lks[4] = {lk4.x, lk4.y, lk4.z, lk4.w};
+  // the four fields start unreached
+  const unsigned long long none2 = ((unsigned long long)LBW_NONE << 32) | LBW_NONE;
+  for (int j = tid; j < c.n; j += blockDim.x) {
+    c.pset[qb + j] = make_int2((int)LBW_NONE, (int)LBW_NONE);
+    c.cand_head[qb + j] = none2;
+  }
+  __threadfence();
+  __syncthreads();
+  int rounds = 0, rr = 0;
+  wg_search<4>(c, q, LSCALE * delta_len + lk4.x * delta_delay, s_n, s_min, relaxed, rr, 0.f, lk4);
+  rounds += rr;
+  __threadfence();
+  __syncthreads();
+  if (wid < 4) {
+    double wl = 0.0, wd = 0.0;
+    const float kw = wid == 0 ? lk4.x : wid == 1 ? lk4.y : wid == 2 ? lk4.z : lk4.w;
+    const uint32_t ws = *grid_field(c, qb + s, wid);
+    const bool ok = ws < LBW_NONE && walk_tight<4>(c, q, kw, wl, wd, wid);
+    if (lane_id() == 0) {
+      s_gok[wid] = ok ? 1 : 0;
+      s_glb[wid] = ws < LBW_NONE ? ((double)ws - (double)kw * (double)bnd) / (double)LSCALE : -1e300;
+      s_gd[wid] = (float)wd;
+      s_gl[wid] = (float)wl;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int jb = -1;
+    for (int j = 0; j < 4; j++) {
+      if (s_glb[j] > ls.best_lb) {
+        ls.best_lb = s_glb[j];
+        ls.best_k = lks[j];
+        jb = j;
+      }
+      if (s_gok[j]) {
+        if (s_gd[j] > bnd) {
+          if (s_gd[j] < ls.p1d) {
+            ls.p1d = s_gd[j];
+            ls.p1l = s_gl[j];
+          }
+        } else if (s_gd[j] > ls.p0d) {
+          ls.p0d = s_gd[j];
+          ls.p0l = s_gl[j];
+        }
+      }
+    }
+    ls.last_k = jb;  // here: the grid field holding best_k (-1: none)
+    float kh = 0.f;
+    if (ls.p1d > ls.p0d && ls.p0l > ls.p1l) kh = hull_multiplier(ls);
+    for (int j = 0; j < 4; j++)
+      if (kh == lks[j]) kh = 0.f;
+    s_kh = kh;
+  }
+  __syncthreads();
+  const float kh = s_kh;
+  float2* lq = c.lbl + qb;
+  bool from_grid = true;
+  if (kh > 0.f) {
+    // hull step into lbw (still unreached from the batch reset)
+    wg_search<3>(c, q, LSCALE * delta_len + kh * delta_delay, s_n, s_min, relaxed, rr, kh);
+    rounds += rr;
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+      const uint32_t ws = __float_as_uint(lq[s].y);
+      const double lb = ws < LBW_NONE ? ((double)ws - (double)kh * (double)bnd) / (double)LSCALE : -1e300;
+      if (lb >= ls.best_lb) {
+        ls.best_lb = lb;
+        ls.best_k = kh;
+        ls.last_k = -1.f;
+      }
+    }
+    __syncthreads();
+    from_grid = ls.last_k >= 0.f;
+  }
+  const int jb = (int)ls.last_k;
+  // lbw from the chosen grid field (or cleared when the hull step is kept but
+  // no bound was found), then the label search's words back to their reset
+  // values
+  for (int j = tid; j < c.n; j += blockDim.x) {
+    if (from_grid) lq[j].y = jb >= 0 ? __uint_as_float(*grid_field(c, qb + j, jb)) : __uint_as_float(LBW_NONE);
+    c.pset[qb + j] = make_int2(0, 0);
+    c.cand_head[qb + j] = ~0ull;
+  }
+  if (tid == 0 && from_grid && jb < 0) ls.best_k = 0.f;
+  __syncthreads();
+  return rounds;
+}
+
 __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float delta_delay, float delta_len) {
   __shared__ int s_n[2];
   __shared__ uint32_t s_min[2];
@@ -922,6 +1072,13 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
     if (threadIdx.x < WAVE) lagr_begin(c, q, ls);
     __syncthreads();
     float2* lq = c.lbl + (long long)q * c.n;
+    if (c.lagrange == LAGR_GRID) {
+      if (!ls.stop) r3 = lagr_grid(c, q, ls, delta_len, delta_delay, s_n, s_min, relaxed);
+      if (threadIdx.x == 0) c.q_lk[q] = ls.stop ? 0.f : ls.best_k;
+      wave_add(&c.ct->sssp_relaxed, relaxed);
+      if (threadIdx.x == 0) atomicMax(&c.ct->sround, r0 + r1 + r2 + r3);
+      return;
+    }
     for (int it = 0; it < c.lagrange && !ls.stop; it++) {
       const float k = ls.k;
       if (it > 0) {
