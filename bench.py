@@ -157,6 +157,7 @@ def main():
                     help="internal solvers on separate HIP streams (vine_options.concurrency); default: the "
                          "autotuned choice cached in vine_amd/models/autotune.json, else 3")
     ap.add_argument("--no-tuned", action="store_true", help="ignore the autotune cache (library launch defaults)")
+    ap.add_argument("--tune-cache", default=None, help="autotune cache file (default: the package's)")
     ap.add_argument("--use-graph", type=int, default=None, help="override: replay rounds as a HIP graph (0/1)")
     ap.add_argument("--near-windows", type=float, default=None, help="override: near-tier reach in windows")
     ap.add_argument("--hash-slots", type=int, default=None, help="override: LDS candidate table slots (0 = off)")
@@ -203,12 +204,13 @@ def main():
     base = dict(max_batch=args.batch, prune=S.PRUNE_BOUNDS, delta_policy=policy, delta_fixed=args.delta,
                 rounds_per_sync=16, **AT.BENCH_CAPACITIES)
     launch = {"concurrency": 3}
-    tuned = None if args.no_tuned else AT.cached_options(g, **base)
+    cache = {"cache": args.tune_cache} if args.tune_cache else {}
+    tuned = None if args.no_tuned else AT.cached_options(g, **cache, **base)
     source = "autotune cache" if tuned else "defaults"
     if not tuned and not args.no_tuned and policy != S.DELTA_MLP:
         # another window policy on the same graph and batch: the learned
         # window's launch shape, so that policies are compared at one shape
-        tuned = AT.cached_options(g, **dict(base, delta_policy=S.DELTA_MLP))
+        tuned = AT.cached_options(g, **cache, **dict(base, delta_policy=S.DELTA_MLP))
         source = "autotune cache (learned-window entry)" if tuned else source
     if tuned:
         launch = tuned

@@ -41,10 +41,11 @@ SPACE = ((("concurrency", "use_graph"), ((1, 0), (1, 1), (2, 0), (2, 1), (3, 0),
          (("labels_per_wave",), ((16,), (32,), (64,))),
          (("block_size",), ((64,), (128,), (256,))),
          (("grid_blocks",), ((256,), (384,), (768,))),
-         (("hash_slots",), ((0,), (64,), (256,), (1024,))))
+         (("hash_slots",), ((0,), (64,), (256,), (1024,))),
+         (("lagrange_bound",), ((2,), (1,), (4,))))
 KEYS = tuple(k for names, _ in SPACE for k in names)
 LIBRARY_DEFAULTS = {"concurrency": 1, "labels_per_wave": 64, "block_size": 128, "grid_blocks": 384, "use_graph": 0,
-                    "hash_slots": 0}
+                    "hash_slots": 0, "lagrange_bound": 2}
 
 
 def graph_key(g, max_batch, policy):
