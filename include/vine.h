@@ -91,7 +91,8 @@ typedef struct {
                                full grid_blocks grid; with time_kernels the first round of each check
                                is launched directly and timed, the graph holds the rest.
                                VINE_DELTA_AR launches directly (it sets the window from the host after
-                               every round).  0: direct launches */
+                               every round).  2: the same, but only once the frontier holds at most 32,768
+                               entries (the batch's tail).  0: direct launches */
   int32_t rounds_per_sync;  /* rounds between host checks of the frontier size */
   int32_t block_size;       /* threads per workgroup of the round kernels (64, 128, 192 or 256; anything
                                else is refused with VINE_E_ARG) */

@@ -288,15 +288,17 @@ def test_graph_replay(gpu, grid_case, policy):
 
 
 @pytest.mark.gpu
-def test_graph_replay_streams(gpu):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_graph_replay_streams(gpu, mode):
     """use_graph with three internal solvers (each captures on its own stream
-    from its own host thread), on the geometric sample, against the oracle."""
+    from its own host thread), on the geometric sample, against the oracle;
+    mode 2 replays only once the frontier is small."""
     g = W.knn(20000, seed=5)
     q = W.hop_queries(g, 30, hop_lo=5, hop_hi=80, f_permille=(1050, 1500), seed=5)
     ref = [O.cspp(g, q.src[i], q.dst[i], q.bound[i], prune_target=2) for i in range(len(q))]
-    print("graph replay, 3 streams", flush=True)
+    print("graph replay mode %d, 3 streams" % mode, flush=True)
     with S.Solver(g, max_batch=30, prune=2, delta_policy=S.DELTA_FIXED, delta_fixed=500.0, rounds_per_sync=8,
-                  use_graph=1, concurrency=3) as s:
+                  use_graph=mode, concurrency=3) as s:
         for rep in range(2):
             ans = s.solve(q.src, q.dst, q.bound)
             _compare(g, q, ref, ans)

@@ -418,8 +418,8 @@ vine_status validate_options(const vine_graph* g, vine_options& o) {
             "learned window and out-degrees <= %d (graph: %d)", dev::ARENA / 2, g->max_deg);
     return VINE_E_ARG;
   }
-  if (o.use_graph < 0 || o.use_graph > 1) {
-    set_error("vine_solver_create: use_graph must be 0 or 1");
+  if (o.use_graph < 0 || o.use_graph > 2) {
+    set_error("vine_solver_create: use_graph must be 0, 1 or 2");
     return VINE_E_ARG;
   }
   if (o.hash_slots && (o.hash_slots < 16 || o.hash_slots > dev::HASH_MAX_SLOTS ||
@@ -859,6 +859,11 @@ vine_status launch_round_any(vine_solver* s, int gsz, vine_stats* st, int sample
   }
 }
 
+// use_graph 2 replays rounds once the frontier (near tier, new labels, far
+// segments and far tier) holds at most this many entries: the tail of a batch,
+// where a few queries' rounds are short and launch gaps dominate
+constexpr long long GRAPH_TAIL_FRONTIER = 1 << 15;
+
 // use_graph: K rounds (expand, merge, pull/policy each) as one HIP graph.
 // Every round kernel reads its round number and frontier sizes from the
 // device counters, so the same K launches are valid for any K consecutive
@@ -1078,7 +1083,8 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
     long long fn_last = nb;
     unsigned long long exp_prev = 0;
     for (;;) {
-      if (graph) {
+      // use_graph 2: replay only in the tail, once the frontier is small
+      if (graph && (s->opt.use_graph == 1 || fn_last <= GRAPH_TAIL_FRONTIER)) {
         vine_status ls = launch_rounds_graph(s, K, st);
         if (ls != VINE_OK) return ls;
         rounds += K;
