@@ -479,8 +479,8 @@ void init_fields(vine_solver* s, const vine_graph* g, const vine_options& o) {
   s->lpw = o.labels_per_wave ? o.labels_per_wave : 64;
 }
 
-// Per-(query, vertex) state: pset, cand_head; with the bounds lbl, lbd, inq
-// and the two bound-search queue entries.
+// Per-(query, vertex) state: pset, cand_head; with the bounds lbl, lbd, the
+// bound searches' queue stamp and the HBM part of their two queues.
 size_t per_qv_bytes(const vine_options& o) {
   return sizeof(int2) + sizeof(unsigned long long) +
          (o.prune == 2 ? sizeof(float2) + sizeof(unsigned long long) + sizeof(uint32_t) + 2 * sizeof(int32_t) : 0);

@@ -115,7 +115,6 @@ struct Ctx {
   unsigned long long* lbd;  // packed (delay bits << 32 | len bits): lexicographic
                             // (delay, len) distance to t; the len part is the
                             // length of a minimum-delay path
-  uint32_t* inq;     // (query, vertex) in the bound-search frontier
   // pools
   float2* pool_cd;
   int32_t* pool_id;
@@ -160,6 +159,7 @@ struct Ctx {
   float avg_deg;
   float far_floor;    // smallest near-tier reach (key units)
   float far_mult;     // the near tier reaches far_mult windows past the front
+  uint32_t* inq;     // [B][n] bound searches: stamp of the round whose queue last took the vertex (0 = none)
   int32_t* wq;       // [B][2][n] bound-search frontier queues
   int32_t* q_order;  // [B] bound-search dispatch order (null: slot order)
   int4* wnear;       // workgroup engine: [B][2][wn_cap] near-tier records per query
@@ -534,8 +534,7 @@ __global__ void k_init_batch(Ctx c, int nq) {
 // query's workgroup retires early and a long one is never held up by others.
 // Each round relaxes the frontier vertices whose distance is within `delta`
 // of the round's smallest frontier distance and carries the rest, which keeps
-// re-relaxation close to Dijkstra's.  inq marks (query, vertex) pairs in the
-// frontier (one queue entry each, so a queue never exceeds n entries).
+// re-relaxation close to Dijkstra's.
 //   WHICH = 2: forward min delay fwd from s (on the graph itself, into lbl),
 //   limited to radius `bound`.  Run first when c.ellipse is set.
 //   WHICH = 0: delay to t on the transposed graph, limited to radius `bound`,
@@ -551,6 +550,51 @@ __global__ void k_init_batch(Ctx c, int nq) {
 //   in the bits of lbl.y (sums saturate at LBW_NONE, which is never stored);
 //   the window works on their float values.
 constexpr int BOUND_BLOCK = 512;
+// Frontier of a bound search: the first BQ_LDS entries of each round's queue
+// in LDS, the rest in the query's HBM queue.  A vertex enters the next
+// round's queue once: an LDS set of BQ_HASH slots, cleared every round,
+// records who is in it; a vertex whose BQ_PROBES probes find neither itself
+// nor a free slot (the set only fills up during a round, so that happens on
+// its every push of the round) is checked against its HBM stamp instead,
+// the round's number in the workgroup's count of rounds (`stamp`, never 0).
+// An entry holds only the vertex: its distance is read when it is taken, so
+// a vertex lowered after it was queued is relaxed with its newest distance.
+constexpr int BQ_LDS = 1024;
+constexpr int BQ_HASH = 2048;
+constexpr int BQ_PROBES = 8;
+
+struct BoundLds {
+  int n[2];          // entries in each queue
+  uint32_t kmin[2];  // smallest key queued in each (float bits)
+  uint32_t stamp;    // rounds of this workgroup so far, + 1
+  int q[2][BQ_LDS];
+  int hkey[BQ_HASH];  // vertices already in the next queue (-1: free)
+};
+
+__device__ __forceinline__ void bq_push(const Ctx& c, BoundLds& L, long long qb, int nxt, int32_t* hq, uint32_t stamp,
+                                        bool pred, int w) {
+  bool ins = pred;
+  if (pred) {
+    int h = (int)(((uint32_t)w * 2654435761u) >> 21) & (BQ_HASH - 1);
+    bool set = false;
+    for (int p = 0; p < BQ_PROBES; p++) {
+      const int k = atomicCAS(&L.hkey[h], -1, w);
+      if (k == -1 || k == w) {  // claimed (new in the queue) or found (queued already)
+        set = true;
+        ins = k == -1;
+        break;
+      }
+      h = (h + 1) & (BQ_HASH - 1);
+    }
+    if (!set) ins = atomicExch(&c.inq[qb + w], stamp) != stamp;
+  }
+  const int slot = wave_alloc32(&L.n[nxt], ins);
+  if (ins) {
+    if (slot < BQ_LDS) L.q[nxt][slot] = w;
+    else if (slot - BQ_LDS < c.n) hq[slot - BQ_LDS] = w;
+    else atomicOr(&c.ct->overflow, OVF_SSSP);
+  }
+}
 
 // distance field of search WHICH at (query, vertex) qv
 template <int WHICH>
@@ -570,15 +614,19 @@ __device__ __forceinline__ uint32_t load_agent_u32p(const uint32_t* p) {
 }
 
 template <int WHICH>
-__device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* s_min, int& relaxed, int& rounds,
+__device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& relaxed, int& rounds,
                           float lk = 0.f, float4 lk4 = make_float4(0.f, 0.f, 0.f, 0.f)) {
+  // arcs a lane relaxes per step: their loads and atomics are issued together
+  constexpr int R = 2;
   const float lks[4] = {lk4.x, lk4.y, lk4.z, lk4.w};
   const int n = c.n;
   const long long qb = (long long)q * n;
-  int32_t* que[2] = {c.wq + 2 * qb, c.wq + 2 * qb + n};
+  int32_t* que[2] = {c.wq + 2 * qb, c.wq + 2 * qb + n};  // HBM part of the two frontiers
   const float bnd = c.q_bound[q];
   const int tid = threadIdx.x;
   const int lane = lane_id();
+  const int nw = blockDim.x / WAVE;
+  const int wid = tid / WAVE;
   const int32_t* rp = (WHICH == 2) ? c.row_ptr : c.rrow_ptr;
   const int4* ep = (WHICH == 2) ? c.edges : c.redges;
   if (tid == 0) {
@@ -587,17 +635,17 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
     else if (WHICH == 4)
       for (int j = 0; j < 4; j++) *grid_field(c, qb + root, j) = 0u;
     else *bound_dist<WHICH>(c, qb + root) = 0.f;
-    c.inq[qb + root] = 1;
-    que[0][0] = root;
-    s_n[0] = 1;
-    s_n[1] = 0;
-    s_min[0] = __float_as_uint(0.f);
-    s_min[1] = __float_as_uint(FINF);
+    L.q[0][0] = root;
+    L.n[0] = 1;
+    L.n[1] = 0;
+    L.kmin[0] = __float_as_uint(0.f);
+    L.kmin[1] = __float_as_uint(FINF);
   }
+  for (int i = tid; i < BQ_HASH; i += blockDim.x) L.hkey[i] = -1;
   __syncthreads();
   int cur = 0;
   for (int round = 0;; round++) {
-    const int ncur = s_n[cur];
+    const int ncur = L.n[cur];
     if (ncur == 0) break;
     // every round settles the smallest frontier vertex for good, so rounds
     // are bounded by about n; this guard only stops a malformed run
@@ -607,140 +655,161 @@ __device__ void wg_search(const Ctx& c, int q, float delta, int* s_n, uint32_t* 
     }
     rounds = round + 1;
     const int nxt = cur ^ 1;
-    const float lim_key = __uint_as_float(s_min[cur]) + delta;
+    const float lim_key = __uint_as_float(L.kmin[cur]) + delta;
+    const uint32_t stamp = L.stamp;
     const int32_t* qin = que[cur];
     int32_t* qout = que[nxt];
-    for (int base = (tid / WAVE) * WAVE; base < ncur; base += blockDim.x) {
-      const int i = base + lane;
+    // entries are dealt to the waves in turn (entry lane * nw + wid of each
+    // block-sized slice), so a small frontier still spreads over every wave
+    for (int base = 0; base < ncur; base += blockDim.x) {
+      const int i = base + lane * nw + wid;
       int deg = 0, row = 0, u = 0;
       float du = 0.f, dl2 = 0.f;  // distance (delay for WHICH 0 and 2, len for 1); len tie key
       uint32_t dub = 0;           // WHICH 3: the integer distance (du is its float value)
       uint32_t dg[4] = {0u, 0u, 0u, 0u};  // WHICH 4: the four distances (du orders by the first)
       bool carry = false;
       if (i < ncur) {
-        u = qin[i];
+        u = i < BQ_LDS ? L.q[cur][i] : qin[i - BQ_LDS];
         const long long qv = qb + u;
-        if (WHICH == 0) du = hi_of(__hip_atomic_load(&c.lbd[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        else if (WHICH == 3) du = (float)__float_as_uint(load_agent(bound_dist<WHICH>(c, qv)));
-        else if (WHICH == 4) du = (float)load_agent_u32p(grid_field(c, qv, 0));
-        else du = load_agent(bound_dist<WHICH>(c, qv));
-        if (du > lim_key) {
-          carry = true;
+        // the vertex's current distance (atomics update it in L2) and its
+        // arcs' range, loaded together
+        if (WHICH == 0) {
+          const unsigned long long pk = __hip_atomic_load(&c.lbd[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          du = hi_of(pk);
+          dl2 = lo_of(pk);
+        } else if (WHICH == 3) {
+          dub = __float_as_uint(load_agent(bound_dist<WHICH>(c, qv)));
+          du = (float)dub;
+        } else if (WHICH == 4) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) dg[j] = load_agent_u32p(grid_field(c, qv, j));
+          du = (float)dg[0];
         } else {
-          // leave the frontier first, then read the final distance: a later
-          // decrease by another lane sees inq == 0 and re-queues the vertex.
-          // Only this workgroup touches the query's arrays, so workgroup
-          // ordering is enough (distances are read from L2, where the
-          // atomics land).
-          c.inq[qv] = 0;
-          __threadfence_block();
-          if (WHICH == 0) {
-            const unsigned long long pk = __hip_atomic_load(&c.lbd[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            du = hi_of(pk);
-            dl2 = lo_of(pk);
-          } else if (WHICH == 3) {
-            dub = __float_as_uint(load_agent(bound_dist<WHICH>(c, qv)));
-            du = (float)dub;
-          } else if (WHICH == 4) {
-#pragma unroll
-            for (int j = 0; j < 4; j++) dg[j] = load_agent_u32p(grid_field(c, qv, j));
-            du = (float)dg[0];
-          } else {
-            du = load_agent(bound_dist<WHICH>(c, qv));
-          }
-          row = rp[u];
-          deg = rp[u + 1] - row;
+          du = load_agent(bound_dist<WHICH>(c, qv));
         }
+        row = rp[u];
+        const int r1 = rp[u + 1];
+        carry = du > lim_key;
+        deg = carry ? 0 : r1 - row;
       }
-      const int cslot = wave_alloc32(&s_n[nxt], carry);
       if (carry) {
-        if (cslot < n) qout[cslot] = u;
-        else atomicOr(&c.ct->overflow, OVF_SSSP);
-        atomicMin(&s_min[nxt], __float_as_uint(du));
+        if (__float_as_uint(du) < L.kmin[nxt]) atomicMin(&L.kmin[nxt], __float_as_uint(du));
       }
-      wave_flat(deg, row, [&](int owner, int pos, bool act) {
-        const float odu = __shfl(du, owner, WAVE);
-        const float odl = __shfl(dl2, owner, WAVE);
-        const uint32_t odub = WHICH == 3 ? (uint32_t)__shfl((int)dub, owner, WAVE) : 0u;
-        uint32_t odg[4];
+      bq_push(c, L, qb, nxt, qout, stamp, carry, u);
+      // the wave's arcs, R per lane per step
+      int incl = deg;
 #pragma unroll
-        for (int j = 0; j < 4; j++) odg[j] = WHICH == 4 ? (uint32_t)__shfl((int)dg[j], owner, WAVE) : 0u;
-        bool push = false;
-        int w = 0;
-        if (act) {
-          relaxed++;
-          const int4 e = ep[pos];
-          w = e.x;
-          const long long wv = qb + w;
-          bool lowered = false;
-          float nd;
+      for (int o = 1; o < WAVE; o <<= 1) {
+        const int t = __shfl_up(incl, o, WAVE);
+        if (lane >= o) incl += t;
+      }
+      const int excl = incl - deg;
+      const int total = __shfl(incl, WAVE - 1, WAVE);
+      for (int k = 0; k < total; k += R * WAVE) {
+        int own[R], pos[R];
+        bool act[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int e = k + r * WAVE + lane;
+          act[r] = e < total;
+          const int ec = act[r] ? e : total - 1;
+          int lo = 0;
+#pragma unroll
+          for (int step = 32; step > 0; step >>= 1) {
+            const int v = __shfl(excl, lo + step, WAVE);
+            if (v <= ec) lo += step;
+          }
+          own[r] = lo;
+          pos[r] = __shfl(row, lo, WAVE) + (ec - __shfl(excl, lo, WAVE));
+        }
+        int4 ed[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) ed[r] = act[r] ? ep[pos[r]] : make_int4(0, 0, 0, 0);
+        // candidate distances, and the target's current value and region test
+        float nd[R];
+        bool go[R];
+        unsigned long long pk[R];
+        uint32_t ndg[R][4];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const float odu = __shfl(du, own[r], WAVE);
+          const float odl = __shfl(dl2, own[r], WAVE);
+          const long long wv = qb + ed[r].x;
+          go[r] = false;
+          nd[r] = FINF;
+          pk[r] = 0ull;
           if (WHICH == 2) {
-            nd = odu + __int_as_float(e.z);
-            float* dp = bound_dist<2>(c, wv);
-            if (nd <= bnd && nd < *dp) {
-              const unsigned old = atomicMin((unsigned*)dp, __float_as_uint(nd));
-              lowered = __float_as_uint(nd) < old;
-            }
+            nd[r] = odu + __int_as_float(ed[r].z);
+            go[r] = act[r] && nd[r] <= bnd && nd[r] < *bound_dist<2>(c, wv);
           } else if (WHICH == 1) {
-            nd = odu + __int_as_float(e.y);
-            float* dp = bound_dist<1>(c, wv);
-            if (lbd_of(c, wv) <= bnd && nd < *dp) {
-              const unsigned old = atomicMin((unsigned*)dp, __float_as_uint(nd));
-              lowered = __float_as_uint(nd) < old;
-            }
-          } else if (WHICH == 4) {
-            nd = FINF;
-            if (lbd_of(c, wv) <= bnd) {
-#pragma unroll
-              for (int j = 0; j < 4; j++) {
-                const unsigned long long s64 = (unsigned long long)odg[j] + lagr_weight(e, lks[j]);
-                const uint32_t ndb = s64 < (unsigned long long)LBW_NONE ? (uint32_t)s64 : LBW_NONE;
-                if (j == 0) nd = (float)ndb;
-                uint32_t* fp = grid_field(c, wv, j);
-                if (ndb < *fp) {
-                  const unsigned old = atomicMin(fp, ndb);
-                  lowered = lowered || ndb < old;
-                }
-              }
-            }
+            nd[r] = odu + __int_as_float(ed[r].y);
+            go[r] = act[r] && lbd_of(c, wv) <= bnd && nd[r] < *bound_dist<1>(c, wv);
           } else if (WHICH == 3) {
-            const unsigned long long s64 = (unsigned long long)odub + lagr_weight(e, lk);
-            const uint32_t ndb = s64 < (unsigned long long)LBW_NONE ? (uint32_t)s64 : LBW_NONE;
-            nd = (float)ndb;
-            unsigned* dp = (unsigned*)bound_dist<3>(c, wv);
-            if (lbd_of(c, wv) <= bnd && ndb < *dp) {
-              const unsigned old = atomicMin(dp, ndb);
-              lowered = ndb < old;
+            const uint32_t odub = (uint32_t)__shfl((int)dub, own[r], WAVE);
+            const unsigned long long s64 = (unsigned long long)odub + lagr_weight(ed[r], lk);
+            ndg[r][0] = s64 < (unsigned long long)LBW_NONE ? (uint32_t)s64 : LBW_NONE;
+            nd[r] = (float)ndg[r][0];
+            go[r] = act[r] && lbd_of(c, wv) <= bnd && ndg[r][0] < *(const uint32_t*)bound_dist<3>(c, wv);
+          } else if (WHICH == 4) {
+            go[r] = act[r] && lbd_of(c, wv) <= bnd;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+              const uint32_t od = (uint32_t)__shfl((int)dg[j], own[r], WAVE);
+              const unsigned long long s64 = (unsigned long long)od + lagr_weight(ed[r], lks[j]);
+              ndg[r][j] = s64 < (unsigned long long)LBW_NONE ? (uint32_t)s64 : LBW_NONE;
             }
+            nd[r] = (float)ndg[r][0];
           } else {
-            nd = odu + __int_as_float(e.z);
-            const unsigned long long pk = pack2(nd, odl + __int_as_float(e.y));
+            nd[r] = odu + __int_as_float(ed[r].z);
+            pk[r] = pack2(nd[r], odl + __int_as_float(ed[r].y));
             // with the forward search done, only vertices with fwd + lbd <=
             // bound: every vertex of a feasible s-t path lies there, and so
             // does every vertex of such a vertex's shortest path to t
             const float lim = c.ellipse ? bnd - c.lbl[wv].x : bnd;
-            if (nd <= lim && pk < c.lbd[wv]) {
-              const unsigned long long old = atomicMin(&c.lbd[wv], pk);
-              lowered = pk < old;
+            go[r] = act[r] && nd[r] <= lim && pk[r] < c.lbd[wv];
+          }
+        }
+        bool lowered[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const long long wv = qb + ed[r].x;
+          lowered[r] = false;
+          if (act[r]) relaxed++;
+          if (!go[r]) {
+          } else if (WHICH == 2 || WHICH == 1) {
+            const unsigned old = atomicMin((unsigned*)bound_dist<WHICH>(c, wv), __float_as_uint(nd[r]));
+            lowered[r] = __float_as_uint(nd[r]) < old;
+          } else if (WHICH == 3) {
+            const unsigned old = atomicMin((unsigned*)bound_dist<3>(c, wv), ndg[r][0]);
+            lowered[r] = ndg[r][0] < old;
+          } else if (WHICH == 4) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+              uint32_t* fp = grid_field(c, wv, j);
+              if (ndg[r][j] < *fp) {
+                const unsigned old = atomicMin(fp, ndg[r][j]);
+                lowered[r] = lowered[r] || ndg[r][j] < old;
+              }
             }
-          }
-          if (lowered) {
-            if (__float_as_uint(nd) < s_min[nxt]) atomicMin(&s_min[nxt], __float_as_uint(nd));
-            push = (atomicExch(&c.inq[wv], 1u) == 0u);
+          } else {
+            const unsigned long long old = atomicMin(&c.lbd[wv], pk[r]);
+            lowered[r] = pk[r] < old;
           }
         }
-        const int slot = wave_alloc32(&s_n[nxt], push);
-        if (push) {
-          if (slot < n) qout[slot] = w;
-          else atomicOr(&c.ct->overflow, OVF_SSSP);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          if (lowered[r] && __float_as_uint(nd[r]) < L.kmin[nxt]) atomicMin(&L.kmin[nxt], __float_as_uint(nd[r]));
+          bq_push(c, L, qb, nxt, qout, stamp, lowered[r], ed[r].x);
         }
-      });
+      }
     }
     __syncthreads();
     if (tid == 0) {
-      s_n[cur] = 0;
-      s_min[cur] = __float_as_uint(FINF);
+      L.n[cur] = 0;
+      L.kmin[cur] = __float_as_uint(FINF);
+      L.stamp++;
     }
+    for (int i = tid; i < BQ_HASH; i += blockDim.x) L.hkey[i] = -1;
     cur = nxt;
     __syncthreads();
   }
@@ -939,8 +1008,8 @@ __global__ void __launch_bounds__(1024) k_bound_order(Ctx c, int nq) {
 // one per multiplier tried.  Returns the rounds of the searches.
 constexpr int LAGR_GRID = 2;
 
-__device__ int lagr_grid(const Ctx& c, int q, LagrState& ls, float delta_len, float delta_delay, int* s_n,
-                         uint32_t* s_min, int& relaxed) {
+__device__ int lagr_grid(const Ctx& c, int q, LagrState& ls, float delta_len, float delta_delay, BoundLds& L,
+                         int& relaxed) {
   __shared__ double s_glb[4];
   __shared__ float s_gd[4], s_gl[4];
   __shared__ int s_gok[4];
@@ -962,7 +1031,7 @@ __device__ int lagr_grid(const Ctx& c, int q, LagrState& ls, float delta_len, fl
   __threadfence();
   __syncthreads();
   int rounds = 0, rr = 0;
-  wg_search<4>(c, q, LSCALE * delta_len + lk4.x * delta_delay, s_n, s_min, relaxed, rr, 0.f, lk4);
+  wg_search<4>(c, q, LSCALE * delta_len + lk4.x * delta_delay, L, relaxed, rr, 0.f, lk4);
   rounds += rr;
   __threadfence();
   __syncthreads();
@@ -1012,7 +1081,7 @@ __device__ int lagr_grid(const Ctx& c, int q, LagrState& ls, float delta_len, fl
   bool from_grid = true;
   if (kh > 0.f) {
     // hull step into lbw (still unreached from the batch reset)
-    wg_search<3>(c, q, LSCALE * delta_len + kh * delta_delay, s_n, s_min, relaxed, rr, kh);
+    wg_search<3>(c, q, LSCALE * delta_len + kh * delta_delay, L, relaxed, rr, kh);
     rounds += rr;
     __threadfence();
     __syncthreads();
@@ -1043,18 +1112,18 @@ __device__ int lagr_grid(const Ctx& c, int q, LagrState& ls, float delta_len, fl
 }
 
 __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float delta_delay, float delta_len) {
-  __shared__ int s_n[2];
-  __shared__ uint32_t s_min[2];
+  __shared__ BoundLds L;
   if ((int)blockIdx.x >= nq) return;
+  if (threadIdx.x == 0) L.stamp = 1;
   const int q = c.q_order ? c.q_order[blockIdx.x] : (int)blockIdx.x;
   int relaxed = 0, r0 = 0, r1 = 0, r2 = 0;
   if (c.ellipse) {
-    wg_search<2>(c, q, delta_delay, s_n, s_min, relaxed, r2);
+    wg_search<2>(c, q, delta_delay, L, relaxed, r2);
     // later reads of fwd must not hit lines cached while it was written
     __threadfence();
     __syncthreads();
   }
-  wg_search<0>(c, q, delta_delay, s_n, s_min, relaxed, r0);
+  wg_search<0>(c, q, delta_delay, L, relaxed, r0);
   __syncthreads();
   if (c.ellipse) {
     // lbl held fwd: back to +inf for the len search
@@ -1063,7 +1132,7 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
   }
   __threadfence();
   __syncthreads();
-  wg_search<1>(c, q, delta_len, s_n, s_min, relaxed, r1);
+  wg_search<1>(c, q, delta_len, L, relaxed, r1);
   int r3 = 0;
   if (c.lagrange) {
     __shared__ LagrState ls;
@@ -1073,7 +1142,7 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
     __syncthreads();
     float2* lq = c.lbl + (long long)q * c.n;
     if (c.lagrange == LAGR_GRID) {
-      if (!ls.stop) r3 = lagr_grid(c, q, ls, delta_len, delta_delay, s_n, s_min, relaxed);
+      if (!ls.stop) r3 = lagr_grid(c, q, ls, delta_len, delta_delay, L, relaxed);
       if (threadIdx.x == 0) c.q_lk[q] = ls.stop ? 0.f : ls.best_k;
       wave_add(&c.ct->sssp_relaxed, relaxed);
       if (threadIdx.x == 0) atomicMax(&c.ct->sround, r0 + r1 + r2 + r3);
@@ -1088,7 +1157,7 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
         __syncthreads();
       }
       int rr = 0;
-      wg_search<3>(c, q, LSCALE * delta_len + k * delta_delay, s_n, s_min, relaxed, rr, k);
+      wg_search<3>(c, q, LSCALE * delta_len + k * delta_delay, L, relaxed, rr, k);
       r3 += rr;
       __threadfence();
       __syncthreads();
@@ -1104,7 +1173,7 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
       __syncthreads();
       if (bk > 0.f) {
         int rr = 0;
-        wg_search<3>(c, q, LSCALE * delta_len + bk * delta_delay, s_n, s_min, relaxed, rr, bk);
+        wg_search<3>(c, q, LSCALE * delta_len + bk * delta_delay, L, relaxed, rr, bk);
         r3 += rr;
       }
     }

@@ -36,6 +36,17 @@ constexpr int WG_BLOCK = 256;
 constexpr int HSLOTS = 2048;  // LDS hash slots (vertices touched per round <= ARENA < HSLOTS)
 constexpr int ARENA = 1024;   // LDS candidate records per round; arcs of a label never split
 constexpr int WG_ROUND_GUARD = 1 << 21;  // rounds of one query (real searches need ~1e4)
+constexpr int WG_GROUPS = WG_BLOCK / MG;  // merge groups of a workgroup
+constexpr int FC = 128;       // front + candidates of one vertex merged at once (registers of a merge group)
+constexpr int FCC = 64;       // candidates of one vertex staged in LDS for that merge
+// Label ids are handed out from a block reserved per query (one global atomic
+// per block instead of one per inserted label).  A block is refilled at the
+// end of a round once fewer than ARENA ids are left (a round inserts at most
+// ARENA labels), with at least LID_MIN ids and about as many as the query has
+// used so far (at most LID_MAX); the ids left in a replaced or final block
+// stay unused.
+constexpr int LID_MIN = 2 * ARENA;
+constexpr int LID_MAX = 1 << 16;
 
 struct WgLds {
   int key[HSLOTS];    // vertex id, -1 = free
@@ -52,7 +63,27 @@ struct WgLds {
   int n_exp, n_ins;
   int more;           // far tier still holds records (decided by thread 0)
   long long exp_tot;
+  long long lid_base;  // label ids [lid_base + lid_n, lid_base + lid_lim) are free
+  long long lid_total; // ids reserved so far by this query
+  int lid_n, lid_lim;
+  float2 fcd[WG_GROUPS][FCC];  // merge: a vertex's candidates staged in LDS, (d, delay) ...
+  int fid[WG_GROUPS][FCC];     // ... and parent labels
 };
+
+// Reserve the next block of label ids (thread 0, between phases).
+__device__ __forceinline__ void wg_lid_refill(const Ctx& c, Counters* ct, WgLds& S) {
+  const long long want = min(max((long long)LID_MIN, S.lid_total), (long long)LID_MAX);
+  const long long base = (long long)atomicAdd(&ct->label_ctr, (unsigned long long)want);
+  S.lid_n = 0;
+  if (base + want > c.label_cap) {
+    atomicOr(&ct->overflow, OVF_LABEL);
+    S.lid_lim = 0;
+    return;
+  }
+  S.lid_base = base;
+  S.lid_lim = (int)want;
+  S.lid_total += want;
+}
 
 __device__ __forceinline__ int wg_hash(int v) { return (int)(((uint32_t)v * 2654435761u) >> 21); }
 
@@ -187,6 +218,239 @@ __device__ void wg_pull(const Ctx& c, Counters* ct, int q, WgLds& S, int4* near_
   }
 }
 
+// Fold all k candidates of vertex v into its Pareto front at once.  Entries
+// x < cnt are the front in the pool at off (sorted by d, delay strictly
+// falling), entries x >= cnt the candidates, staged in LDS at E/I[x - cnt]
+// in list order (parent label in I); m = cnt + k <= FC, k <= FCC.  Lane gl holds
+// entries gl + MG*r in registers.  The result
+// is the sequential one (candidates folded in list order, a front entry or
+// an earlier candidate winning a tie): an entry is dropped iff another one
+// is componentwise <= it and differs from it or precedes it, since the
+// dominance of two-criterion labels is a componentwise order.  Candidates
+// dropped this way get no label.  Survivors are placed by d (distinct among
+// them), the changed tail is written back to the pool (a new allocation if
+// the front outgrew it), new labels get ids from the query's block and join
+// the next near list.
+constexpr int NSL = FC / MG;
+
+template <typename Label_t>
+__device__ __forceinline__ void wg_merge_batch(const Ctx& c, Counters* ct, WgLds& S, int q, int v, long long qv,
+                                               int tq, float2 lb, float lbdv, float bnd, float lk, float2 best,
+                                               int4* near_nxt, int nxt, const float2* E, int32_t* I, uint32_t off,
+                                               int cnt, int cap, int k, int gl, int gbase,
+                                               unsigned long long gmask, int& my_ins, int& n_scan) {
+  const int m = cnt + k;
+  const int nsl = (m + MG - 1) / MG;
+  // front entries straight from the pool (all loads issued together),
+  // candidates from the stage
+  float xd[NSL], xl[NSL];
+  int32_t xid[NSL];  // front: label id; candidate: parent label
+  bool alive[NSL];
+#pragma unroll
+  for (int r = 0; r < NSL; r++) {
+    const int x = gl + MG * r;
+    alive[r] = x < m;
+    xd[r] = FINF;
+    xl[r] = FINF;
+    xid[r] = 0;
+    if (x < cnt) {
+      const float2 e = c.pool_cd[off + x];
+      xd[r] = e.x;
+      xl[r] = e.y;
+      xid[r] = c.pool_id[off + x];
+    } else if (x < m) {
+      const float2 e = E[x - cnt];
+      xd[r] = e.x;
+      xl[r] = e.y;
+      xid[r] = I[x - cnt];
+    }
+  }
+  // every candidate against every entry, in both directions
+  for (int j = cnt; j < m; j++) {
+    const float2 cj = E[j - cnt];  // the same address for all lanes
+    bool hit = false;              // some entry of this lane drops candidate j
+#pragma unroll
+    for (int r = 0; r < NSL; r++) {
+      if (r >= nsl) break;
+      const int x = gl + MG * r;
+      if (x < m && x != j) {
+        const bool same = xd[r] == cj.x && xl[r] == cj.y;
+        // x <= cj: drops j unless x is a later candidate of equal value
+        if (xd[r] <= cj.x && xl[r] <= cj.y && (!same || x < j)) hit = true;
+        // cj <= x: drops x unless x is a front entry or an earlier candidate of equal value
+        if (cj.x <= xd[r] && cj.y <= xl[r] && (!same || (x >= cnt && j < x))) alive[r] = false;
+      }
+    }
+    const bool dropped = (__ballot(hit) & gmask) != 0ull;
+    if (dropped) {
+#pragma unroll
+      for (int r = 0; r < NSL; r++)
+        if (gl + MG * r == j) alive[r] = false;
+    }
+  }
+  n_scan += k;
+  // survivor masks per slot (bit l: entry l + MG*r), front and candidates
+  uint32_t fm[NSL], cm[NSL];
+  int nf = 0, nc = 0;
+#pragma unroll
+  for (int r = 0; r < NSL; r++) {
+    fm[r] = 0u;
+    cm[r] = 0u;
+    if (r >= nsl) break;
+    const int x = gl + MG * r;
+    fm[r] = (uint32_t)((__ballot(alive[r] && x < cnt) & gmask) >> gbase);
+    cm[r] = (uint32_t)((__ballot(alive[r] && x >= cnt) & gmask) >> gbase);
+    nf += __popc(fm[r]);
+    nc += __popc(cm[r]);
+  }
+  // candidates' survival into the stage (sign bit of the parent field: free,
+  // parents are label ids >= 0), then the order statistics: for every held
+  // entry the surviving candidates with a smaller d, for every surviving
+  // candidate the surviving front entries with a smaller d
+  int less[NSL], nfl[NSL];
+#pragma unroll
+  for (int r = 0; r < NSL; r++) {
+    less[r] = 0;
+    nfl[r] = 0;
+  }
+  if (nc > 0) {
+#pragma unroll
+    for (int r = 0; r < NSL; r++) {
+      const int x = gl + MG * r;
+      if (x >= cnt && x < m && alive[r]) I[x - cnt] = xid[r] | (int32_t)0x80000000;
+    }
+    wave_lds_order();
+    for (int j = cnt; j < m; j++) {
+      const float2 cj = E[j - cnt];
+      const bool sj = I[j - cnt] < 0;
+      int nfj = 0;
+#pragma unroll
+      for (int r = 0; r < NSL; r++) {
+        if (r >= nsl) break;
+        const int x = gl + MG * r;
+        if (sj && cj.x < xd[r]) less[r]++;
+        nfj += __popc((uint32_t)((__ballot(alive[r] && x < cnt && xd[r] < cj.x) & gmask) >> gbase));
+      }
+#pragma unroll
+      for (int r = 0; r < NSL; r++)
+        if (gl + MG * r == j) nfl[r] = nfj;
+    }
+  }
+  const int newcnt = nf + nc;
+  // positions, the first changed one, label ranks
+  int pos[NSL], crank[NSL];
+  int first = newcnt;
+  {
+    int fbefore = 0, cbefore = 0;
+    const uint32_t below = (1u << gl) - 1u;
+#pragma unroll
+    for (int r = 0; r < NSL; r++) {
+      const int x = gl + MG * r;
+      const int fr = fbefore + __popc(fm[r] & below);
+      crank[r] = cbefore + __popc(cm[r] & below);
+      pos[r] = -1;
+      if (alive[r]) {
+        pos[r] = (x < cnt ? fr : nfl[r]) + less[r];
+        if (x >= cnt) first = min(first, pos[r]);
+      } else if (x < cnt) {
+        first = min(first, x);  // a front entry dropped: everything from it on may move
+      }
+      fbefore += __popc(fm[r]);
+      cbefore += __popc(cm[r]);
+    }
+  }
+#pragma unroll
+  for (int o = MG / 2; o > 0; o >>= 1) first = min(first, __shfl_xor(first, o, WAVE));
+  if (first >= newcnt && newcnt == cnt) return;  // nothing changed
+  // label ids for the surviving candidates
+  int lbase = 0;
+  if (gl == 0 && nc > 0) lbase = atomicAdd(&S.lid_n, nc);
+  lbase = __shfl(lbase, gbase, WAVE);
+  if (nc > 0 && lbase + nc > S.lid_lim) {
+    if (gl == 0) atomicOr(&ct->overflow, OVF_LABEL);
+    return;  // the solve fails
+  }
+  // a front that outgrew its allocation moves, written whole
+  if (newcnt > cap) {
+    int newcap = INIT_CAP;
+    while (newcap < 2 * newcnt) newcap <<= 1;
+    long long poff = 0;
+    if (gl == 0) poff = (long long)atomicAdd(&ct->pool_ctr, (unsigned long long)newcap);
+    poff = __shfl(poff, gbase, WAVE);
+    if (poff < 0 || (unsigned long long)(poff + newcap) > c.pool_cap || poff + newcap > 0xFFFFFFFFll) {
+      if (gl == 0) atomicOr(&ct->overflow, OVF_POOL);
+      return;
+    }
+    off = (uint32_t)poff;
+    cap = newcap;
+    first = 0;
+  }
+  float2 nb = make_float2(FINF, FINF);
+  int nbid = -1;
+#pragma unroll
+  for (int r = 0; r < NSL; r++) {
+    const int x = gl + MG * r;
+    bool push = false;
+    int4 nrec = make_int4(-1, 0, 0, 0);
+    float nkey = 0.f;
+    if (x < m) {
+      const int32_t idp = xid[r];  // front: label id; candidate: parent
+      if (x < cnt) {
+        if (!alive[r]) c.dead[idp] = 1;
+        else if (pos[r] >= first) {
+          c.pool_cd[off + pos[r]] = make_float2(xd[r], xl[r]);
+          c.pool_id[off + pos[r]] = idp;
+        }
+      } else if (alive[r]) {
+        const long long lid = S.lid_base + lbase + crank[r];
+        c.pool_cd[off + pos[r]] = make_float2(xd[r], xl[r]);
+        c.pool_id[off + pos[r]] = (int)lid;
+        nkey = lower_key(xd[r], xl[r], lb, lk, bnd);
+        c.labs[lid] = make_int4(v, __float_as_int(xd[r]), __float_as_int(xl[r]), idp);
+        c.lmeta[lid] = make_int2(q, __float_as_int(nkey));
+        c.dead[lid] = 0;
+        if (!(xd[r] < EXACT_LIMIT) || !(xl[r] < EXACT_LIMIT)) ct->range_flag = 1;
+        if (v == tq) {
+          if (xd[r] < nb.x) {
+            nb = make_float2(xd[r], xl[r]);
+            nbid = (int)lid;
+          }
+        } else {
+          const float nrd = xl[r] + lbdv;
+          if (!(nrd > bnd) && !lex_ge(nkey, nrd, best.x, best.y)) {
+            push = true;
+            nrec = make_int4((int)lid, q, __float_as_int(nkey), __float_as_int(nrd));
+          }
+        }
+      }
+    }
+    if (r < nsl) wg_carry(c, ct, q, S, near_nxt, nxt, push, nrec, nkey);
+  }
+  if (v == tq) {
+    // the new labels at t: the smallest d among them improves the best answer
+#pragma unroll
+    for (int o = MG / 2; o > 0; o >>= 1) {
+      const float od = __shfl_xor(nb.x, o, WAVE), ol = __shfl_xor(nb.y, o, WAVE);
+      const int oid = __shfl_xor(nbid, o, WAVE);
+      if (od < nb.x) {
+        nb = make_float2(od, ol);
+        nbid = oid;
+      }
+    }
+    if (gl == 0 && nbid >= 0) {
+      const float2 b = S.best;
+      if (nb.x < b.x || (nb.x == b.x && nb.y < b.y)) {
+        S.best = nb;
+        S.best_id = nbid;
+      }
+    }
+  }
+  if (gl == 0) {
+    my_ins += nc;
+    c.pset[qv] = pset_make(off, newcnt, cap);
+  }
+}
+
 template <typename Label_t, typename Constraint_t, typename Edge_t>
 __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
   constexpr int PRUNE = 2;
@@ -220,6 +484,8 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
     S.n_exp = 0;
     S.n_ins = 0;
     S.exp_tot = 0;
+    S.lid_total = 0;
+    wg_lid_refill(c, ct, S);
     // the source label (id q, written by k_init_batch) starts the near list
     const int s = c.q_src[q];
     const float key0 = __int_as_float(c.lmeta[q].y);
@@ -335,14 +601,21 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
     if (my_exp) atomicAdd(&S.n_exp, my_exp);
     __syncthreads();
 
-    // ---------------- merge: MG-lane groups over the touched vertices
+    // ---------------- merge: MG-lane groups over the touched vertices.  A
+    // vertex whose front and candidates fit FC entries (at most FCC
+    // candidates) is merged at once
+    // (wg_merge_batch); a larger one folds its candidates one at a time into
+    // the pool directly.
     const int nt = S.n_touched;
     int my_ins = 0;
     {
       const int gl = lane & (MG - 1);
       const int gbase = lane & ~(MG - 1);
       const unsigned long long gmask = ((1ull << MG) - 1ull) << gbase;
-      for (int g = tid / MG; g < nt; g += WG_BLOCK / MG) {
+      const int grp = tid / MG;
+      float2* const fcd = S.fcd[grp];
+      int32_t* const fid = S.fid[grp];
+      for (int g = grp; g < nt; g += WG_GROUPS) {
         const int h = S.touched[g];
         const int v = S.key[h];
         const long long qv = qb + v;
@@ -351,6 +624,23 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
         int cnt = pset_cnt(ps), cap = pset_cap(ps);
         const float2 lb = c.lbl[qv];
         const float lbdv = lbd_of(c, qv);
+        // count the candidates, staging them in list order while they fit
+        int ncnd = 0;
+        for (int kk = S.head[h]; kk >= 0; ncnd++) {
+          const int4 r = S.rec[kk];
+          kk = r.w;
+          if ((ncnd & (MG - 1)) == gl && cnt + ncnd < FC && ncnd < FCC) {
+            fcd[ncnd] = make_float2(__int_as_float(r.x), __int_as_float(r.y));
+            fid[ncnd] = r.z;
+          }
+        }
+        if (cnt + ncnd <= FC && ncnd <= FCC) {
+          wave_lds_order();
+          wg_merge_batch<Label_t>(c, ct, S, q, v, qv, tq, lb, lbdv, bnd, lk, best, near_nxt, nxt, fcd, fid, off, cnt,
+                                  cap, ncnd, gl, gbase, gmask, my_ins, n_scan);
+          wave_lds_order();  // the stage is reused by the group's next vertex
+          continue;
+        }
         for (int idx = S.head[h]; idx >= 0;) {
           const int4 r = S.rec[idx];
           idx = r.w;
@@ -410,17 +700,20 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
           long long poff = 0;
           if (grow && gl == 0) poff = (long long)atomicAdd(&ct->pool_ctr, (unsigned long long)newcap);
           poff = __shfl(poff, gbase, WAVE);
-          long long lid = wave_alloc64(&ct->label_ctr, acc && gl == 0);
-          lid = __shfl(lid, gbase, WAVE);
+          // the label id, from the query's reserved block
+          int lslot = 0;
+          if (acc && gl == 0) lslot = atomicAdd(&S.lid_n, 1);
+          lslot = __shfl(lslot, gbase, WAVE);
           bool fits = acc;
           if (grow && (poff < 0 || (unsigned long long)(poff + newcap) > c.pool_cap || poff + newcap > 0xFFFFFFFFll)) {
             if (gl == 0) atomicOr(&ct->overflow, OVF_POOL);
             fits = false;
           }
-          if (fits && lid >= c.label_cap) {
+          if (fits && lslot >= S.lid_lim) {
             if (gl == 0) atomicOr(&ct->overflow, OVF_LABEL);
             fits = false;
           }
+          const long long lid = S.lid_base + lslot;
           bool push = false;
           int4 nrec = make_int4(-1, 0, 0, 0);
           float nkey = 0.f;
@@ -549,6 +842,8 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
         S.n_touched = 0;
         S.n_near[cur] = 0;
         S.fmin[cur] = __float_as_uint(FINF);
+        // the next round inserts at most ARENA labels
+        if (S.lid_lim - S.lid_n < ARENA && S.lid_lim > 0) wg_lid_refill(c, ct, S);
       }
     }
     __syncthreads();
