@@ -299,14 +299,25 @@ def main():
     B_S, B_P = 24.0, 32.0  # bytes per bound-search relaxation / far record pulled
     # (HIP-event ms, launches timed, launches, algorithmic bytes of all launches)
     tr = agg["timed_rounds"]
-    kern = {
-        "expand": (agg["ms_expand"], tr, agg["expand_launches"], B_X * agg["expanded"] + B_E * agg["relaxed"]),
-        "merge": (agg["ms_merge"], tr, agg["merge_launches"], B_I * agg["inserted"]),
-        # a bound-search relaxation reads a 16-B arc and updates an 8-B distance
-        "bounds": (agg["ms_sssp"], agg["sssp_launches"], agg["sssp_launches"], B_S * agg["sssp_relaxed"]),
-        # a pulled far record is read once and written once into the near tier
-        "pull_policy": (agg["ms_pull"], tr, agg["expand_launches"], B_P * agg["far_pulled"]),
-    }
+    # a bound-search relaxation reads a 16-B arc and updates an 8-B distance
+    bounds_k = (agg["ms_sssp"], agg["sssp_launches"], agg["sssp_launches"], B_S * agg["sssp_relaxed"])
+    workgroup_engine = agg["merge_launches"] == 0 and agg["expand_launches"] > 0
+    if workgroup_engine:
+        # one persistent launch per batch runs every query's label search
+        # (k_search, each launch timed): expand and merge bytes together
+        kern = {
+            "search": (agg["ms_expand"], agg["expand_launches"], agg["expand_launches"],
+                       B_X * agg["expanded"] + B_E * agg["relaxed"] + B_I * agg["inserted"]),
+            "bounds": bounds_k,
+        }
+    else:
+        kern = {
+            "expand": (agg["ms_expand"], tr, agg["expand_launches"], B_X * agg["expanded"] + B_E * agg["relaxed"]),
+            "merge": (agg["ms_merge"], tr, agg["merge_launches"], B_I * agg["inserted"]),
+            "bounds": bounds_k,
+            # a pulled far record is read once and written once into the near tier
+            "pull_policy": (agg["ms_pull"], tr, agg["expand_launches"], B_P * agg["far_pulled"]),
+        }
     # dominant: the largest estimated time per step (timed share scaled to all launches)
     est = {k: v[0] * v[2] / max(v[1], 1) for k, v in kern.items()}
     dom = max(est, key=lambda k: est[k])
@@ -337,7 +348,8 @@ def main():
                 "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
                 "bytes_rule": {"expand": "20 B x labels expanded + 12 B x arcs relaxed",
                                "merge": "24 B x labels inserted", "bounds": "24 B x bound-search relaxations",
-                               "pull_policy": "32 B x far records pulled"}[dom],
+                               "pull_policy": "32 B x far records pulled",
+                               "search": "20 B x labels expanded + 12 B x arcs relaxed + 24 B x labels inserted"}[dom],
                 "algorithmic_bytes_per_launch": per_launch, "avg_launch_us": avg_s * 1e6,
                 "launches_per_step": launches / args.steps, "launches_timed": timed,
                 "kernel_ms_per_step_est": {k: v / args.steps for k, v in est.items()},
@@ -378,7 +390,7 @@ def main():
         for lbnd in (0, 2):
             with S.Solver(dg, max_batch=1, prune=S.PRUNE_BOUNDS, delta_policy=S.DELTA_FIXED,
                           delta_fixed=args.plain_delta, rounds_per_sync=16, label_capacity=200_000_000,
-                          lagrange_bound=lbnd) as ps:
+                          lagrange_bound=lbnd, engine=S.ENGINE_ROUNDS) as ps:
                 for name, qq in cases:
                     t1 = time.perf_counter()
                     a = ps.solve(qq.src, qq.dst, qq.bound, paths=False)
@@ -393,6 +405,17 @@ def main():
                                     and np.array_equal(a.feasible.astype(np.int32), feas_b[chk]))
                         r["same_answers"] = r.get("same_answers", True) and same
                     log("plain %s, lagrange_bound %d: %.1f queries/s" % (name, lbnd, len(qq) / dt))
+        # the same single-query solves in the persistent workgroup engine
+        # (reported beside it; the plain figure keeps the reference's
+        # grid-wide round structure)
+        with S.Solver(dg, max_batch=1, prune=S.PRUNE_BOUNDS, delta_policy=S.DELTA_FIXED,
+                      delta_fixed=args.plain_delta, label_capacity=200_000_000,
+                      engine=S.ENGINE_WORKGROUP) as ps:
+            for name, qq in cases:
+                t1 = time.perf_counter()
+                ps.solve(qq.src, qq.dst, qq.bound, paths=False)
+                res[name]["qps_workgroup_engine"] = len(qq) / (time.perf_counter() - t1)
+                log("plain %s, workgroup engine: %.1f queries/s" % (name, res[name]["qps_workgroup_engine"]))
         plain = res
 
     # CPU oracle on bounded samples (the GPU box's host cores)
@@ -417,8 +440,8 @@ def main():
                                "log-uniform [10,2000], bound U[1.05,2.0] x min delay" % (g.m, nq),
                    "batch": args.batch, "policy": args.policy, "delta": args.delta, "prune": "bounds",
                    "launch": launch, "launch_source": source,
-                   "plain": "1 query per solve, fixed window %g, random pairs bound 1.1x (%d queries), the faster of "
-                            "lagrange_bound 0 and 2" % (args.plain_delta, args.plain_queries),
+                   "plain": "1 query per solve, grid-wide rounds, fixed window %g, random pairs bound 1.1x (%d "
+                            "queries), the faster of lagrange_bound 0 and 2" % (args.plain_delta, args.plain_queries),
                    "parallelism": "single GPU"},
         "latency_us": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "work_per_step": {k: agg[k] / args.steps for k in ("rounds", "sssp_rounds", "scanned", "expanded", "relaxed",

@@ -71,11 +71,11 @@ enum {
 
 /* How the label search is scheduled. */
 enum {
-  VINE_ENGINE_AUTO = 0,      /* currently grid-wide rounds (the higher-throughput engine) */
+  VINE_ENGINE_AUTO = 0,      /* the workgroup engine where it applies, else grid-wide rounds */
   VINE_ENGINE_ROUNDS = 1,    /* grid-wide rounds: every query advances one round per kernel sequence */
-  VINE_ENGINE_WORKGROUP = 2  /* one workgroup per query runs its whole search: lower tail latency, lower
-                                throughput (needs VINE_PRUNE_BOUNDS, a fixed or learned window and
-                                out-degrees <= 512) */
+  VINE_ENGINE_WORKGROUP = 2  /* one persistent workgroup per query runs its whole search, rounds
+                                separated by workgroup barriers (needs VINE_PRUNE_BOUNDS, a fixed or
+                                learned window and out-degrees <= 512) */
 };
 
 typedef struct vine_graph vine_graph;

@@ -59,56 +59,68 @@ def _check(name, g, q, fx, a, idx=None):
     print("%s: %d answers bit-exact vs the oracle, paths valid" % (name, n), flush=True)
 
 
+ENGINES = [S.ENGINE_ROUNDS, S.ENGINE_WORKGROUP]
+
+
 @pytest.mark.gpu
 def test_ny_batch_full(gpu):
     """The benchmark's workload and configuration: 4,096 queries, learned
-    window, the cached launch shape (three streams)."""
+    window, the cached launch options; then the learned window in the other
+    engine and a fixed window in both, in one stream."""
     g, q, fx = _case("ny_batch")
     print("ny_batch: %d queries" % len(q), flush=True)
     base = AT.bench_base(4096)
     launch = AT.cached_options(g, **base) or {"concurrency": 3}
     dg = S.DeviceGraph(g)
-    a = _solve(dg, q, 4096, mlp=np.load(L.DEFAULT_MODEL), **{**base, **launch})
-    _check("ny_batch learned", g, q, fx, a)
-    # a fixed window in one stream
-    b = _solve(dg, q, 4096, delta_policy=S.DELTA_FIXED, delta_fixed=12.0, **AT.BENCH_CAPACITIES)
-    _check("ny_batch fixed 12", g, q, fx, b)
+    mlp = np.load(L.DEFAULT_MODEL)
+    a = _solve(dg, q, 4096, mlp=mlp, **{**base, **launch})
+    _check("ny_batch learned (bench options %s)" % launch, g, q, fx, a)
+    other = S.ENGINE_ROUNDS if launch.get("engine", S.ENGINE_AUTO) != S.ENGINE_ROUNDS else S.ENGINE_WORKGROUP
+    a2 = _solve(dg, q, 4096, mlp=mlp, **{**base, "engine": other})
+    _check("ny_batch learned engine %d" % other, g, q, fx, a2)
+    for eng in ENGINES:
+        b = _solve(dg, q, 4096, delta_policy=S.DELTA_FIXED, delta_fixed=12.0, engine=eng, **AT.BENCH_CAPACITIES)
+        _check("ny_batch fixed 12 engine %d" % eng, g, q, fx, b)
 
 
 @pytest.mark.gpu
-def test_ny_batch_plain_sample(gpu):
+@pytest.mark.parametrize("engine", ENGINES)
+def test_ny_batch_plain_sample(gpu, engine):
     """The plain single-query solver (one query per solve) on 96 queries of the
     batched workload drawn across its hop range."""
     g, q, fx = _case("ny_batch")
     idx = np.random.default_rng(3).choice(len(q), 96, replace=False)
     sub = q.subset(idx)
-    a = _solve(g, sub, 1, delta_policy=S.DELTA_FIXED, delta_fixed=12.0, label_capacity=200_000_000)
-    _check("ny_batch plain", g, q, fx, a, idx=idx)
+    a = _solve(g, sub, 1, delta_policy=S.DELTA_FIXED, delta_fixed=12.0, label_capacity=200_000_000, engine=engine)
+    _check("ny_batch plain engine %d" % engine, g, q, fx, a, idx=idx)
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["ny_1.1x", "ny_1.5x"])
-def test_ny_random_pairs_full(gpu, name):
+@pytest.mark.parametrize("engine", ENGINES)
+def test_ny_random_pairs_full(gpu, name, engine):
     g, q, fx = _case(name)
-    print("%s: %d queries" % (name, len(q)), flush=True)
+    print("%s: %d queries, engine %d" % (name, len(q), engine), flush=True)
     dg = S.DeviceGraph(g)
-    a = _solve(dg, q, 1000, delta_policy=S.DELTA_FIXED, delta_fixed=12.0)
+    a = _solve(dg, q, 1000, delta_policy=S.DELTA_FIXED, delta_fixed=12.0, engine=engine)
     _check(name + " fixed 12", g, q, fx, a)
-    b = _solve(dg, q, 1000, mlp=np.load(L.DEFAULT_MODEL), delta_policy=S.DELTA_MLP, delta_fixed=4.0, concurrency=2)
+    b = _solve(dg, q, 1000, mlp=np.load(L.DEFAULT_MODEL), delta_policy=S.DELTA_MLP, delta_fixed=4.0, concurrency=2,
+               engine=engine)
     _check(name + " learned", g, q, fx, b)
 
 
 @pytest.mark.gpu
-def test_ne_full_learned_vs_fixed(gpu):
+@pytest.mark.parametrize("engine", ENGINES)
+def test_ne_full_learned_vs_fixed(gpu, engine):
     """All 1,000 NE pairs at 1.2x: the learned window against a fixed one, in
     batches of 250 query slots; the learned window expands fewer labels."""
     g, q, fx = _case("ne_1.2x")
-    print("ne: n=%d m=%d, %d queries at 1.2x" % (g.n, g.m, len(q)), flush=True)
+    print("ne: n=%d m=%d, %d queries at 1.2x, engine %d" % (g.n, g.m, len(q), engine), flush=True)
     dg = S.DeviceGraph(g)
-    a = _solve(dg, q, 250, delta_policy=S.DELTA_FIXED, delta_fixed=12.0, label_capacity=300_000_000)
+    a = _solve(dg, q, 250, delta_policy=S.DELTA_FIXED, delta_fixed=12.0, label_capacity=300_000_000, engine=engine)
     _check("ne fixed 12", g, q, fx, a)
     b = _solve(dg, q, 250, mlp=np.load(L.DEFAULT_MODEL), delta_policy=S.DELTA_MLP, delta_fixed=4.0,
-               concurrency=3, label_capacity=300_000_000)
+               concurrency=3, label_capacity=300_000_000, engine=engine)
     _check("ne learned", g, q, fx, b)
     assert b.stats["expanded"] < a.stats["expanded"]
 
@@ -120,6 +132,10 @@ def test_rmat17_full(gpu):
           flush=True)
     assert fx["feasible"].all()  # pairs come from the largest component, bound >= min delay
     dg = S.DeviceGraph(g)
+    # out-degrees above 512: the workgroup engine does not apply, the default picks grid-wide rounds
+    with pytest.raises(S.VineError):
+        S.Solver(dg, max_batch=4, prune=S.PRUNE_BOUNDS, delta_policy=S.DELTA_FIXED, delta_fixed=20.0,
+                 engine=S.ENGINE_WORKGROUP)
     a = _solve(dg, q, 500, delta_policy=S.DELTA_FIXED, delta_fixed=20.0)
     _check("rmat17 fixed 20", g, q, fx, a)
     b = _solve(dg, q, 500, delta_policy=S.DELTA_INF)

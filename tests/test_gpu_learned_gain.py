@@ -5,6 +5,7 @@ include/AR.hpp driven by the AR(3) forecast, with the reference's MAX (free
 device memory / sizeof(label), algorithm/cspp.cuh:43-47).  Both runs use the
 same kernels, bounds and batch; only the window policy differs, so answers
 must be identical and the gain shows in labels expanded, rounds and time.
+Both run the grid-wide round engine, the only one the AR rule runs in.
 The network is the committed one (vine_amd/models/delta_mlp.npy), trained on
 NY-scale queries with seeds disjoint from these.
 """
@@ -23,7 +24,7 @@ def _run(dg, q, policy, delta, params=None):
     # on the host after every round, the learned window every 16 rounds, so
     # wall time would also count those waits; kernel time does not
     with S.Solver(dg, max_batch=len(q), prune=S.PRUNE_BOUNDS, delta_policy=policy, delta_fixed=delta,
-                  rounds_per_sync=16, time_kernels=1) as s:
+                  rounds_per_sync=16, time_kernels=1, engine=S.ENGINE_ROUNDS) as s:
         if params is not None:
             s.set_mlp(params)
         s.solve(q.src[:8], q.dst[:8], q.bound[:8], paths=False)  # warm-up (code objects, allocations)

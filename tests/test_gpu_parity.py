@@ -200,7 +200,8 @@ def test_block_size_refused(gpu, grid_case, block):
 def test_block_sizes_accepted(gpu, grid_case, block):
     g, q, ref = grid_case
     print("block_size %d" % block, flush=True)
-    with S.Solver(g, max_batch=25, prune=2, delta_policy=S.DELTA_FIXED, delta_fixed=300.0, block_size=block) as s:
+    with S.Solver(g, max_batch=25, prune=2, delta_policy=S.DELTA_FIXED, delta_fixed=300.0, block_size=block,
+                  engine=S.ENGINE_ROUNDS) as s:
         ans = s.solve(q.src, q.dst, q.bound)
     _compare(g, q, ref, ans)
 
@@ -266,7 +267,8 @@ def test_graph_replay(gpu, grid_case, policy):
     from vine_amd import learn as L
     g, q, ref = grid_case
     delta = {S.DELTA_FIXED: 300.0, S.DELTA_MLP: 4.0, S.DELTA_INF: 0.0}[policy]
-    kw = dict(max_batch=25, prune=2, delta_policy=policy, delta_fixed=delta, rounds_per_sync=4, time_kernels=1)
+    kw = dict(max_batch=25, prune=2, delta_policy=policy, delta_fixed=delta, rounds_per_sync=4, time_kernels=1,
+              engine=S.ENGINE_ROUNDS)
     print("graph replay policy=%d" % policy, flush=True)
     with S.Solver(g, **kw) as d:
         if policy == S.DELTA_MLP:
@@ -298,7 +300,7 @@ def test_graph_replay_streams(gpu, mode):
     ref = [O.cspp(g, q.src[i], q.dst[i], q.bound[i], prune_target=2) for i in range(len(q))]
     print("graph replay mode %d, 3 streams" % mode, flush=True)
     with S.Solver(g, max_batch=30, prune=2, delta_policy=S.DELTA_FIXED, delta_fixed=500.0, rounds_per_sync=8,
-                  use_graph=mode, concurrency=3) as s:
+                  use_graph=mode, concurrency=3, engine=S.ENGINE_ROUNDS) as s:
         for rep in range(2):
             ans = s.solve(q.src, q.dst, q.bound)
             _compare(g, q, ref, ans)
@@ -315,7 +317,8 @@ def test_lds_candidate_table(gpu, grid_case, slots):
     g, q, ref = grid_case
     print("LDS candidate table, %d slots" % slots, flush=True)
     for policy, delta in ((S.DELTA_FIXED, 300.0), (S.DELTA_MLP, 4.0), (S.DELTA_INF, 0.0)):
-        with S.Solver(g, max_batch=25, prune=2, delta_policy=policy, delta_fixed=delta, hash_slots=slots) as s:
+        with S.Solver(g, max_batch=25, prune=2, delta_policy=policy, delta_fixed=delta, hash_slots=slots,
+                      engine=S.ENGINE_ROUNDS) as s:
             if policy == S.DELTA_MLP:
                 s.set_mlp(np.load(L.DEFAULT_MODEL))
             ans = s.solve(q.src, q.dst, q.bound)
@@ -337,7 +340,7 @@ def test_lds_candidate_table_knn(gpu):
     print("LDS candidate table, knn20k", flush=True)
     for prune, slots in ((2, 64), (2, 1024), (1, 256)):
         with S.Solver(g, max_batch=16, prune=prune, delta_policy=S.DELTA_FIXED, delta_fixed=500.0,
-                      hash_slots=slots) as s:
+                      hash_slots=slots, engine=S.ENGINE_ROUNDS) as s:
             ans = s.solve(q.src, q.dst, q.bound)
         _compare(g, q, ref, ans)
     with S.Solver(g, max_batch=30, prune=0, hash_slots=32) as s:

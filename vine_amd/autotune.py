@@ -9,14 +9,17 @@ later run loads the choice instead of searching again:
                                   max_batch=4096, delta_policy=S.DELTA_MLP, ...)
     solver = S.Solver(dg, **opts)
 
-Searched, one coordinate at a time from the library defaults: the number of
-internal solvers on separate HIP streams (concurrency), frontier entries a
-wave takes per expand step (labels_per_wave), threads per workgroup of the
-round kernels (block_size), workgroups per round kernel (grid_blocks),
-whether the rounds between host checks replay as one captured HIP graph
-(use_graph) and the slots of the first-level candidate table in LDS
-(hash_slots).  Every configuration must return the same answers as the
-first one (they only change the launch shape); a difference raises.
+Searched, one coordinate at a time from the library defaults: the engine
+(one persistent workgroup per query, or grid-wide rounds) together with the
+number of internal solvers on separate HIP streams (concurrency) and, for
+grid-wide rounds, whether the rounds between host checks replay as one
+captured HIP graph (use_graph); then, for grid-wide rounds only, frontier
+entries a wave takes per expand step (labels_per_wave), threads per
+workgroup of the round kernels (block_size), workgroups per round kernel
+(grid_blocks) and the slots of the first-level candidate table in LDS
+(hash_slots); last the Lagrangian multiplier mode of the bound searches
+(lagrange_bound).  Every configuration must return the same answers as the
+first one (they only change the schedule); a difference raises.
 
     python -m vine_amd.autotune ny_batch        # tune the bench workload, update the cache
 """
@@ -35,17 +38,28 @@ from . import workload as W
 
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models", "autotune.json")
 # coordinates searched one after another; a coordinate may set several options
-# together (streams and graph replay interact: replay removes the launch gaps
-# that other streams' kernels otherwise fill)
-SPACE = ((("concurrency", "use_graph"), ((1, 0), (1, 1), (2, 0), (2, 1), (3, 0), (3, 1))),
+# together (the engine, streams and graph replay interact: replay removes the
+# launch gaps that other streams' kernels otherwise fill, and the workgroup
+# engine has no launches per round to replay)
+SPACE = ((("engine", "concurrency", "use_graph"),
+          ((S.ENGINE_WORKGROUP, 1, 0), (S.ENGINE_WORKGROUP, 2, 0), (S.ENGINE_WORKGROUP, 3, 0),
+           (S.ENGINE_ROUNDS, 1, 0), (S.ENGINE_ROUNDS, 1, 1), (S.ENGINE_ROUNDS, 2, 0), (S.ENGINE_ROUNDS, 2, 1),
+           (S.ENGINE_ROUNDS, 3, 0), (S.ENGINE_ROUNDS, 3, 1))),
          (("labels_per_wave",), ((16,), (32,), (64,))),
          (("block_size",), ((64,), (128,), (256,))),
          (("grid_blocks",), ((256,), (384,), (768,))),
          (("hash_slots",), ((0,), (64,), (256,), (1024,))),
          (("lagrange_bound",), ((2,), (1,), (4,))))
+# options of the grid-wide round kernels only (not searched for the workgroup engine)
+ROUNDS_ONLY = ("labels_per_wave", "block_size", "grid_blocks", "hash_slots")
 KEYS = tuple(k for names, _ in SPACE for k in names)
-LIBRARY_DEFAULTS = {"concurrency": 1, "labels_per_wave": 64, "block_size": 128, "grid_blocks": 384, "use_graph": 0,
-                    "hash_slots": 0, "lagrange_bound": 2}
+LIBRARY_DEFAULTS = {"engine": S.ENGINE_AUTO, "concurrency": 1, "labels_per_wave": 64, "block_size": 128,
+                    "grid_blocks": 384, "use_graph": 0, "hash_slots": 0, "lagrange_bound": 2}
+
+
+def applies(names, opts):
+    """Whether the coordinate `names` changes anything under `opts`."""
+    return not (opts.get("engine") == S.ENGINE_WORKGROUP and all(k in ROUNDS_ONLY for k in names))
 
 
 def graph_key(g, max_batch, policy):
@@ -83,6 +97,8 @@ def search(dg, q, base, mlp=None, reps=1, log=None):
     tried = []
     best_t = None
     for names, values in SPACE:
+        if not applies(names, cur):
+            continue
         scores = {}
         for v in values:
             opts = dict(cur, **dict(zip(names, v)))

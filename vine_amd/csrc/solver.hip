@@ -475,7 +475,7 @@ void init_fields(vine_solver* s, const vine_graph* g, const vine_options& o) {
   const bool wg_ok = o.prune == VINE_PRUNE_BOUNDS &&
                      (o.delta_policy == VINE_DELTA_FIXED || o.delta_policy == VINE_DELTA_MLP) &&
                      g->max_deg <= dev::ARENA / 2;
-  s->wg = wg_ok && o.engine == VINE_ENGINE_WORKGROUP;
+  s->wg = wg_ok && (o.engine == VINE_ENGINE_WORKGROUP || o.engine == VINE_ENGINE_AUTO);
   s->lpw = o.labels_per_wave ? o.labels_per_wave : 64;
 }
 

@@ -14,10 +14,13 @@
 //              candidate goes into an LDS per-vertex hash table: open
 //              addressing on the vertex id, the slot heading a linked list
 //              of arena records;
-//   merge   -- 16-lane groups over the touched slots fold each vertex's
-//              candidates into its sorted Pareto front (k_merge's logic);
-//              live new labels join the next near list;
-//   policy  -- thread 0: the query's 8 features, an optional trace row and,
+//   merge   -- 16-lane groups over the touched vertices merge each
+//              vertex's candidates into its sorted Pareto front a chunk at a
+//              time (every pair compared in registers, survivors placed by
+//              d, the changed tail written back once; a front of FC entries
+//              or more takes the rest one at a time in the pool); live new
+//              labels join the next near list;
+//   policy  -- wave 0: the query's 8 features, an optional trace row and,
 //              for VINE_DELTA_MLP, the next window from the network.
 // All per-query state (near lists, fronts, label records of the query, far
 // tier, best) is written and read by this workgroup alone, i.e. through one
@@ -32,7 +35,7 @@
 namespace vine {
 namespace dev {
 
-constexpr int WG_BLOCK = 256;
+constexpr int WG_BLOCK = 512;
 constexpr int HSLOTS = 2048;  // LDS hash slots (vertices touched per round <= ARENA < HSLOTS)
 constexpr int ARENA = 1024;   // LDS candidate records per round; arcs of a label never split
 constexpr int WG_ROUND_GUARD = 1 << 21;  // rounds of one query (real searches need ~1e4)
@@ -230,14 +233,14 @@ __device__ void wg_pull(const Ctx& c, Counters* ct, int q, WgLds& S, int4* near_
 // dropped this way get no label.  Survivors are placed by d (distinct among
 // them), the changed tail is written back to the pool (a new allocation if
 // the front outgrew it), new labels get ids from the query's block and join
-// the next near list.
+// the next near list.  off, cnt and cap return the new front's header.
 constexpr int NSL = FC / MG;
 
 template <typename Label_t>
 __device__ __forceinline__ void wg_merge_batch(const Ctx& c, Counters* ct, WgLds& S, int q, int v, long long qv,
                                                int tq, float2 lb, float lbdv, float bnd, float lk, float2 best,
-                                               int4* near_nxt, int nxt, const float2* E, int32_t* I, uint32_t off,
-                                               int cnt, int cap, int k, int gl, int gbase,
+                                               int4* near_nxt, int nxt, const float2* E, int32_t* I, uint32_t& off,
+                                               int& cnt, int& cap, int k, int gl, int gbase,
                                                unsigned long long gmask, int& my_ins, int& n_scan) {
   const int m = cnt + k;
   const int nsl = (m + MG - 1) / MG;
@@ -449,6 +452,7 @@ __device__ __forceinline__ void wg_merge_batch(const Ctx& c, Counters* ct, WgLds
     my_ins += nc;
     c.pset[qv] = pset_make(off, newcnt, cap);
   }
+  cnt = newcnt;
 }
 
 template <typename Label_t, typename Constraint_t, typename Edge_t>
@@ -601,11 +605,10 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
     if (my_exp) atomicAdd(&S.n_exp, my_exp);
     __syncthreads();
 
-    // ---------------- merge: MG-lane groups over the touched vertices.  A
-    // vertex whose front and candidates fit FC entries (at most FCC
-    // candidates) is merged at once
-    // (wg_merge_batch); a larger one folds its candidates one at a time into
-    // the pool directly.
+    // ---------------- merge: MG-lane groups over the touched vertices.
+    // Candidates are merged into a vertex's front a chunk at a time
+    // (wg_merge_batch); once the front has FC entries, the rest are folded
+    // one at a time in the pool.
     const int nt = S.n_touched;
     int my_ins = 0;
     {
@@ -624,24 +627,28 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
         int cnt = pset_cnt(ps), cap = pset_cap(ps);
         const float2 lb = c.lbl[qv];
         const float lbdv = lbd_of(c, qv);
-        // count the candidates, staging them in list order while they fit
-        int ncnd = 0;
-        for (int kk = S.head[h]; kk >= 0; ncnd++) {
-          const int4 r = S.rec[kk];
-          kk = r.w;
-          if ((ncnd & (MG - 1)) == gl && cnt + ncnd < FC && ncnd < FCC) {
-            fcd[ncnd] = make_float2(__int_as_float(r.x), __int_as_float(r.y));
-            fid[ncnd] = r.z;
+        // the candidates in list order, folded in chunks that fit the
+        // registers (front + chunk <= FC) and the stage (chunk <= FCC)
+        int kk = S.head[h];
+        while (kk >= 0 && cnt < FC) {
+          const int room = min(FCC, FC - cnt);
+          int k = 0;
+          for (; kk >= 0 && k < room; k++) {
+            const int4 r = S.rec[kk];
+            kk = r.w;
+            if ((k & (MG - 1)) == gl) {
+              fcd[k] = make_float2(__int_as_float(r.x), __int_as_float(r.y));
+              fid[k] = r.z;
+            }
           }
-        }
-        if (cnt + ncnd <= FC && ncnd <= FCC) {
           wave_lds_order();
           wg_merge_batch<Label_t>(c, ct, S, q, v, qv, tq, lb, lbdv, bnd, lk, best, near_nxt, nxt, fcd, fid, off, cnt,
-                                  cap, ncnd, gl, gbase, gmask, my_ins, n_scan);
-          wave_lds_order();  // the stage is reused by the group's next vertex
-          continue;
+                                  cap, k, gl, gbase, gmask, my_ins, n_scan);
+          wave_lds_order();  // the stage is reused by the next chunk
         }
-        for (int idx = S.head[h]; idx >= 0;) {
+        if (kk < 0) continue;
+        // a front of FC entries or more: the rest folded one at a time in the pool
+        for (int idx = kk; idx >= 0;) {
           const int4 r = S.rec[idx];
           idx = r.w;
           const float cd = __int_as_float(r.x), cdl = __int_as_float(r.y);
