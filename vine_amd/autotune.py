@@ -18,7 +18,7 @@ entries a wave takes per expand step (labels_per_wave), threads per
 workgroup of the round kernels (block_size), workgroups per round kernel
 (grid_blocks) and the slots of the first-level candidate table in LDS
 (hash_slots); last the Lagrangian multiplier mode of the bound searches
-(lagrange_bound).  Every configuration must return the same answers as the
+(lagrange_bound) and the near tier's reach in windows (near_windows).  Every configuration must return the same answers as the
 first one (they only change the schedule); a difference raises.
 
     python -m vine_amd.autotune ny_batch        # tune the bench workload, update the cache
@@ -49,12 +49,13 @@ SPACE = ((("engine", "concurrency", "use_graph"),
          (("block_size",), ((64,), (128,), (256,))),
          (("grid_blocks",), ((256,), (384,), (768,))),
          (("hash_slots",), ((0,), (64,), (256,), (1024,))),
-         (("lagrange_bound",), ((2,), (1,), (4,))))
+         (("lagrange_bound",), ((2,), (1,), (4,))),
+         (("near_windows",), ((1.0,), (0.5,), (0.25,))))
 # options of the grid-wide round kernels only (not searched for the workgroup engine)
 ROUNDS_ONLY = ("labels_per_wave", "block_size", "grid_blocks", "hash_slots")
 KEYS = tuple(k for names, _ in SPACE for k in names)
 LIBRARY_DEFAULTS = {"engine": S.ENGINE_AUTO, "concurrency": 1, "labels_per_wave": 64, "block_size": 128,
-                    "grid_blocks": 384, "use_graph": 0, "hash_slots": 0, "lagrange_bound": 2}
+                    "grid_blocks": 384, "use_graph": 0, "hash_slots": 0, "lagrange_bound": 2, "near_windows": 1.0}
 
 
 def applies(names, opts):

@@ -284,22 +284,29 @@ vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, cons
   dst = pdst.data();
   len = plen.data();
   delay = pdelay.data();
-  // packed records {dst, len, delay, 0}; transposed copy for the bound search
+  // packed records {dst, len, delay, dst's arc range}; transposed copy for
+  // the bound search (its records carry the ranges of the transposed CSR)
   std::vector<int4> e(std::max<int64_t>(m, 1)), re(std::max<int64_t>(m, 1));
   std::vector<int32_t> rrow(n + 1, 0);
   for (int64_t p = 0; p < m; p++) rrow[dst[p] + 1]++;
   for (int32_t v = 0; v < n; v++) rrow[v + 1] += rrow[v];
   std::vector<int32_t> fill(rrow.begin(), rrow.end() - 1);
+  auto row_info = [m](const int32_t* rp, int32_t v) -> int32_t {
+    const int64_t first = rp[v], deg = rp[v + 1] - rp[v];
+    if (m >= (1LL << 24) || deg >= 255) return (int32_t)dev::ROW_UNKNOWN;
+    return (int32_t)(((uint32_t)first << 8) | (uint32_t)deg);
+  };
   for (int32_t u = 0; u < n; u++)
     for (int32_t p = row_ptr[u]; p < row_ptr[u + 1]; p++) {
       int4 rec;
       rec.x = dst[p];
       std::memcpy(&rec.y, &len[p], 4);
       std::memcpy(&rec.z, &delay[p], 4);
-      rec.w = 0;
+      rec.w = row_info(row_ptr, dst[p]);
       e[p] = rec;
       int4 rr = rec;
       rr.x = u;
+      rr.w = row_info(rrow.data(), u);
       re[fill[dst[p]]++] = rr;
     }
   auto* g = new vine_graph();

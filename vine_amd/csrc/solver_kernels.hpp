@@ -41,6 +41,11 @@ constexpr int MAX_BLOCK = 256;
 constexpr int MAX_WAVES = MAX_BLOCK / WAVE;
 static_assert(MAX_BLOCK % WAVE == 0 && MAX_WAVES >= 1, "round kernels run whole waves");
 constexpr float FINF = __builtin_huge_valf();
+// The fourth word of an arc record: the target's own arc range in the same
+// CSR, first arc << 8 | out-degree, where that fits (fewer than 2^24 arcs and
+// an out-degree below 255); ROW_UNKNOWN otherwise (read the row pointers).
+constexpr uint32_t ROW_UNKNOWN = 0xFFFFFFFFu;
+__device__ __forceinline__ bool row_known(uint32_t ri) { return ri != ROW_UNKNOWN; }
 constexpr float EXACT_LIMIT = 16777216.0f;  // 2^24
 constexpr int INIT_CAP = 4;                 // first Pareto-set allocation (entries)
 constexpr int TS_CAP = 1 << 20;             // per-round timestamps kept
@@ -557,8 +562,13 @@ constexpr int BOUND_BLOCK = 512;
 // nor a free slot (the set only fills up during a round, so that happens on
 // its every push of the round) is checked against its HBM stamp instead,
 // the round's number in the workgroup's count of rounds (`stamp`, never 0).
-// An entry holds only the vertex: its distance is read when it is taken, so
-// a vertex lowered after it was queued is relaxed with its newest distance.
+// An LDS entry carries the vertex's arc range, from the arc that reached it,
+// and its key when queued, which decides whether the round takes it (a later
+// decrease found the vertex queued already, so the key can be stale high:
+// such an entry is carried once and queued again with its current key); the
+// distance it is relaxed with is read when it is taken, so a vertex lowered
+// after it was queued is relaxed with its newest distance.  Those reads and
+// the arc reads are issued together.  HBM entries hold only the vertex.
 constexpr int BQ_LDS = 1024;
 constexpr int BQ_HASH = 2048;
 constexpr int BQ_PROBES = 8;
@@ -567,12 +577,12 @@ struct BoundLds {
   int n[2];          // entries in each queue
   uint32_t kmin[2];  // smallest key queued in each (float bits)
   uint32_t stamp;    // rounds of this workgroup so far, + 1
-  int q[2][BQ_LDS];
+  int4 q[2][BQ_LDS];  // {vertex, its arc range (row_info), key bits when queued, 0}
   int hkey[BQ_HASH];  // vertices already in the next queue (-1: free)
 };
 
 __device__ __forceinline__ void bq_push(const Ctx& c, BoundLds& L, long long qb, int nxt, int32_t* hq, uint32_t stamp,
-                                        bool pred, int w) {
+                                        bool pred, int w, uint32_t ri, uint32_t keybits) {
   bool ins = pred;
   if (pred) {
     int h = (int)(((uint32_t)w * 2654435761u) >> 21) & (BQ_HASH - 1);
@@ -590,7 +600,7 @@ __device__ __forceinline__ void bq_push(const Ctx& c, BoundLds& L, long long qb,
   }
   const int slot = wave_alloc32(&L.n[nxt], ins);
   if (ins) {
-    if (slot < BQ_LDS) L.q[nxt][slot] = w;
+    if (slot < BQ_LDS) L.q[nxt][slot] = make_int4(w, (int)ri, (int)keybits, 0);
     else if (slot - BQ_LDS < c.n) hq[slot - BQ_LDS] = w;
     else atomicOr(&c.ct->overflow, OVF_SSSP);
   }
@@ -635,7 +645,7 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
     else if (WHICH == 4)
       for (int j = 0; j < 4; j++) *grid_field(c, qb + root, j) = 0u;
     else *bound_dist<WHICH>(c, qb + root) = 0.f;
-    L.q[0][0] = root;
+    L.q[0][0] = make_int4(root, (int)ROW_UNKNOWN, (int)__float_as_uint(0.f), 0);
     L.n[0] = 1;
     L.n[1] = 0;
     L.kmin[0] = __float_as_uint(0.f);
@@ -668,11 +678,12 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
       uint32_t dub = 0;           // WHICH 3: the integer distance (du is its float value)
       uint32_t dg[4] = {0u, 0u, 0u, 0u};  // WHICH 4: the four distances (du orders by the first)
       bool carry = false;
+      int4 ent = make_int4(0, (int)ROW_UNKNOWN, 0, 0);
       if (i < ncur) {
-        u = i < BQ_LDS ? L.q[cur][i] : qin[i - BQ_LDS];
+        ent = i < BQ_LDS ? L.q[cur][i] : make_int4(qin[i - BQ_LDS], (int)ROW_UNKNOWN, 0, 1);
+        u = ent.x;
         const long long qv = qb + u;
-        // the vertex's current distance (atomics update it in L2) and its
-        // arcs' range, loaded together
+        // the vertex's current distance (atomics update it in L2)
         if (WHICH == 0) {
           const unsigned long long pk = __hip_atomic_load(&c.lbd[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           du = hi_of(pk);
@@ -687,15 +698,18 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
         } else {
           du = load_agent(bound_dist<WHICH>(c, qv));
         }
-        row = rp[u];
-        const int r1 = rp[u + 1];
-        carry = du > lim_key;
-        deg = carry ? 0 : r1 - row;
+        // taken by its key when queued (HBM entries: by the distance)
+        carry = (ent.w ? du : __int_as_float(ent.z)) > lim_key;
+        if (!carry) {
+          if (row_known((uint32_t)ent.y)) {
+            row = (int)((uint32_t)ent.y >> 8);
+            deg = ent.y & 255;
+          } else {
+            row = rp[u];
+            deg = rp[u + 1] - row;
+          }
+        }
       }
-      if (carry) {
-        if (__float_as_uint(du) < L.kmin[nxt]) atomicMin(&L.kmin[nxt], __float_as_uint(du));
-      }
-      bq_push(c, L, qb, nxt, qout, stamp, carry, u);
       // the wave's arcs, R per lane per step
       int incl = deg;
 #pragma unroll
@@ -770,10 +784,12 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
           }
         }
         bool lowered[R];
+        uint32_t nkey[R];  // key of a lowered target (WHICH 4: its first field after the update)
 #pragma unroll
         for (int r = 0; r < R; r++) {
           const long long wv = qb + ed[r].x;
           lowered[r] = false;
+          nkey[r] = __float_as_uint(nd[r]);
           if (act[r]) relaxed++;
           if (!go[r]) {
           } else if (WHICH == 2 || WHICH == 1) {
@@ -783,14 +799,20 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
             const unsigned old = atomicMin((unsigned*)bound_dist<3>(c, wv), ndg[r][0]);
             lowered[r] = ndg[r][0] < old;
           } else if (WHICH == 4) {
+            uint32_t f0 = LBW_NONE;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
               uint32_t* fp = grid_field(c, wv, j);
-              if (ndg[r][j] < *fp) {
+              const uint32_t cf = *fp;
+              uint32_t nf = cf;
+              if (ndg[r][j] < cf) {
                 const unsigned old = atomicMin(fp, ndg[r][j]);
                 lowered[r] = lowered[r] || ndg[r][j] < old;
+                nf = min(old, ndg[r][j]);
               }
+              if (j == 0) f0 = nf;
             }
+            nkey[r] = __float_as_uint((float)f0);
           } else {
             const unsigned long long old = atomicMin(&c.lbd[wv], pk[r]);
             lowered[r] = pk[r] < old;
@@ -798,10 +820,13 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
-          if (lowered[r] && __float_as_uint(nd[r]) < L.kmin[nxt]) atomicMin(&L.kmin[nxt], __float_as_uint(nd[r]));
-          bq_push(c, L, qb, nxt, qout, stamp, lowered[r], ed[r].x);
+          if (lowered[r] && nkey[r] < L.kmin[nxt]) atomicMin(&L.kmin[nxt], nkey[r]);
+          bq_push(c, L, qb, nxt, qout, stamp, lowered[r], ed[r].x, (uint32_t)ed[r].w, nkey[r]);
         }
       }
+      // carried entries, queued again with their current key
+      if (carry && __float_as_uint(du) < L.kmin[nxt]) atomicMin(&L.kmin[nxt], __float_as_uint(du));
+      bq_push(c, L, qb, nxt, qout, stamp, carry, u, (uint32_t)ent.y, __float_as_uint(du));
     }
     __syncthreads();
     if (tid == 0) {
@@ -1202,12 +1227,18 @@ __device__ __forceinline__ int far_fit_class(int n) {
   return k;
 }
 
+// LOCAL: only the calling workgroup appends to and reads this query's far
+// tier (the workgroup engine), so its own order suffices; otherwise waves on
+// every CU do, and the records must reach memory before the link.
+template <bool LOCAL>
 __device__ __forceinline__ void far_link(const Ctx& c, int* headp, int q, int fb, int blk, int cls, int count,
                                          int expected) {
   const int head = (blk << 4) | cls;
   for (;;) {
     c.fhd[blk] = make_int2(count, expected);
-    __threadfence();  // records and header before the link
+    // records and header before the link
+    if (LOCAL) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    else __threadfence();
     const int prev = atomicCAS(headp, expected, head);
     if (prev == expected) break;
     expected = prev;
@@ -1216,6 +1247,7 @@ __device__ __forceinline__ void far_link(const Ctx& c, int* headp, int q, int fb
   if (expected < 0) atomicOr(&c.far_occ[(long long)q * NBW + (fb >> 6)], 1ull << (fb & 63));
 }
 
+template <bool LOCAL = false>
 __device__ __forceinline__ void far_append(const Ctx& c, Counters* ct, int q, int fb, const int4& crec4) {
   int* headp = &c.far_head[(long long)q * NBK + fb];
   int hv = __hip_atomic_load(headp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1232,7 +1264,7 @@ __device__ __forceinline__ void far_append(const Ctx& c, Counters* ct, int q, in
         const int ncls = cls + 1;
         const int nb = atomicAdd(&ct->far_chunks, 1 << ncls);
         if (nb < 0 || nb > c.far_cap - (1 << ncls)) atomicOr(&ct->overflow, OVF_FAR);
-        else far_link(c, headp, q, fb, nb, ncls, 0, hv);
+        else far_link<LOCAL>(c, headp, q, fb, nb, ncls, 0, hv);
       }
     } else {
       const int hv2 = __hip_atomic_load(headp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1256,7 +1288,7 @@ __device__ __forceinline__ void far_append(const Ctx& c, Counters* ct, int q, in
       if (lane == leader) atomicOr(&ct->overflow, OVF_FAR);
     } else {
       if (mine) c.frec[(long long)blk * FU + __popcll(mm & ((1ull << lane) - 1ull))] = crec4;
-      if (lane == leader) far_link(c, headp, q, fb, blk, ncls, n_same, hv);
+      if (lane == leader) far_link<LOCAL>(c, headp, q, fb, blk, ncls, n_same, hv);
     }
     if (mine) done = true;
     pend &= ~mm;

@@ -50,6 +50,13 @@ constexpr int FCC = 64;       // candidates of one vertex staged in LDS for that
 // stay unused.
 constexpr int LID_MIN = 2 * ARENA;
 constexpr int LID_MAX = 1 << 16;
+// Pareto-pool entries likewise: a block per query, refilled at the end of a
+// round once fewer than POOL_LOW entries are left (POOL_MIN, then about as
+// many as the query has reserved so far, at most POOL_MAX); an allocation the
+// block cannot take goes to the global counter.
+constexpr int POOL_LOW = 4096;
+constexpr long long POOL_MIN = 8192;
+constexpr long long POOL_MAX = 1 << 20;
 
 struct WgLds {
   int key[HSLOTS];    // vertex id, -1 = free
@@ -69,6 +76,9 @@ struct WgLds {
   long long lid_base;  // label ids [lid_base + lid_n, lid_base + lid_lim) are free
   long long lid_total; // ids reserved so far by this query
   int lid_n, lid_lim;
+  long long pool_base;  // pool entries [pool_base + pool_n, pool_base + pool_lim) are free
+  long long pool_total; // entries reserved so far by this query
+  int pool_n, pool_lim;
   float2 fcd[WG_GROUPS][FCC];  // merge: a vertex's candidates staged in LDS, (d, delay) ...
   int fid[WG_GROUPS][FCC];     // ... and parent labels
 };
@@ -86,6 +96,26 @@ __device__ __forceinline__ void wg_lid_refill(const Ctx& c, Counters* ct, WgLds&
   S.lid_base = base;
   S.lid_lim = (int)want;
   S.lid_total += want;
+}
+
+// Reserve the next block of pool entries (thread 0, between phases).
+__device__ __forceinline__ void wg_pool_refill(const Ctx& c, Counters* ct, WgLds& S) {
+  const long long want = min(max(POOL_MIN, S.pool_total), POOL_MAX);
+  const long long base = (long long)atomicAdd(&ct->pool_ctr, (unsigned long long)want);
+  S.pool_n = 0;
+  S.pool_lim = 0;
+  if (base < 0 || (unsigned long long)(base + want) > c.pool_cap || base + want > 0xFFFFFFFFll) return;
+  S.pool_base = base;
+  S.pool_lim = (int)want;
+  S.pool_total += want;
+}
+
+// n pool entries for one front (one lane): from the query's block, else from
+// the global counter (the caller checks that range against the capacity).
+__device__ __forceinline__ long long wg_pool_alloc(Counters* ct, WgLds& S, int n) {
+  const int o = atomicAdd(&S.pool_n, n);
+  if (o + n <= S.pool_lim) return S.pool_base + o;
+  return (long long)atomicAdd(&ct->pool_ctr, (unsigned long long)n);
 }
 
 __device__ __forceinline__ int wg_hash(int v) { return (int)(((uint32_t)v * 2654435761u) >> 21); }
@@ -118,11 +148,11 @@ __device__ __forceinline__ void wg_carry(const Ctx& c, Counters* ct, int q, WgLd
       fb = max(fb, nfn);
     }
   }
+  // far_cnt counts the wave's far entries with one atomic
+  const unsigned long long mf = __ballot(pred && to_far && nfn < NBK);
+  if (mf && lane_id() == __ffsll((long long)mf) - 1) atomicAdd(&c.far_cnt[q], __popcll(mf));
   if (pred && to_far) {
-    if (nfn < NBK) {
-      atomicAdd(&c.far_cnt[q], 1);
-      far_append(c, ct, q, fb, rec);
-    }
+    if (nfn < NBK) far_append<true>(c, ct, q, fb, rec);
     else atomicOr(&ct->overflow, OVF_CARRY);
   }
 }
@@ -378,7 +408,7 @@ __device__ __forceinline__ void wg_merge_batch(const Ctx& c, Counters* ct, WgLds
     int newcap = INIT_CAP;
     while (newcap < 2 * newcnt) newcap <<= 1;
     long long poff = 0;
-    if (gl == 0) poff = (long long)atomicAdd(&ct->pool_ctr, (unsigned long long)newcap);
+    if (gl == 0) poff = wg_pool_alloc(ct, S, newcap);
     poff = __shfl(poff, gbase, WAVE);
     if (poff < 0 || (unsigned long long)(poff + newcap) > c.pool_cap || poff + newcap > 0xFFFFFFFFll) {
       if (gl == 0) atomicOr(&ct->overflow, OVF_POOL);
@@ -490,6 +520,8 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
     S.exp_tot = 0;
     S.lid_total = 0;
     wg_lid_refill(c, ct, S);
+    S.pool_total = 0;
+    wg_pool_refill(c, ct, S);
     // the source label (id q, written by k_init_batch) starts the near list
     const int s = c.q_src[q];
     const float key0 = __int_as_float(c.lmeta[q].y);
@@ -705,7 +737,7 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
           }
           // groups run divergently here: the leader allocates alone
           long long poff = 0;
-          if (grow && gl == 0) poff = (long long)atomicAdd(&ct->pool_ctr, (unsigned long long)newcap);
+          if (grow && gl == 0) poff = wg_pool_alloc(ct, S, newcap);
           poff = __shfl(poff, gbase, WAVE);
           // the label id, from the query's reserved block
           int lslot = 0;
@@ -851,6 +883,7 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
         S.fmin[cur] = __float_as_uint(FINF);
         // the next round inserts at most ARENA labels
         if (S.lid_lim - S.lid_n < ARENA && S.lid_lim > 0) wg_lid_refill(c, ct, S);
+        if (S.pool_lim - S.pool_n < POOL_LOW) wg_pool_refill(c, ct, S);
       }
     }
     __syncthreads();
