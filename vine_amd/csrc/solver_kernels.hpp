@@ -626,8 +626,10 @@ __device__ __forceinline__ uint32_t load_agent_u32p(const uint32_t* p) {
 template <int WHICH>
 __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& relaxed, int& rounds,
                           float lk = 0.f, float4 lk4 = make_float4(0.f, 0.f, 0.f, 0.f)) {
-  // arcs a lane relaxes per step: their loads and atomics are issued together
-  constexpr int R = 2;
+  // arcs a lane relaxes per step (their loads and atomics issued together);
+  // two measured no faster and take registers that the other stream's label
+  // search needs to share a CU
+  constexpr int R = 1;
   const float lks[4] = {lk4.x, lk4.y, lk4.z, lk4.w};
   const int n = c.n;
   const long long qb = (long long)q * n;

@@ -40,7 +40,7 @@ constexpr int HSLOTS = 2048;  // LDS hash slots (vertices touched per round <= A
 constexpr int ARENA = 1024;   // LDS candidate records per round; arcs of a label never split
 constexpr int WG_ROUND_GUARD = 1 << 21;  // rounds of one query (real searches need ~1e4)
 constexpr int WG_GROUPS = WG_BLOCK / MG;  // merge groups of a workgroup
-constexpr int FC = 128;       // front + candidates of one vertex merged at once (registers of a merge group)
+constexpr int FC = 64;        // front + candidates of one vertex merged at once (registers of a merge group)
 constexpr int FCC = 64;       // candidates of one vertex staged in LDS for that merge
 // Label ids are handed out from a block reserved per query (one global atomic
 // per block instead of one per inserted label).  A block is refilled at the
