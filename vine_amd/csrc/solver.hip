@@ -667,6 +667,7 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
   c.ts = s->d_ts;
   c.ct = s->d_ct;
   c.ellipse = o.bound_ball ? 0 : 1;
+  c.fwd_in_head = s->wg ? 1 : 0;
   return VINE_OK;
 }
 
@@ -1004,7 +1005,14 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
   const long long BN = (long long)s->B * c.n;
   // reset per-(query, vertex) headers and counters
   HIP_TRY(hipMemsetAsync(s->d_pset, 0, sizeof(int2) * BN, s->stream));
-  HIP_TRY(hipMemsetAsync(s->d_cand_head, 0xFF, sizeof(unsigned long long) * BN, s->stream));
+  if (s->wg && s->opt.prune == 2) {
+    // the workgroup engine's label search never reads candidate lists: the
+    // words hold the forward search's distances (+inf to start)
+    vine_status fs = fill_f32(s, (float*)s->d_cand_head, dev::FINF, 2 * BN);
+    if (fs != VINE_OK) return fs;
+  } else {
+    HIP_TRY(hipMemsetAsync(s->d_cand_head, 0xFF, sizeof(unsigned long long) * BN, s->stream));
+  }
   HIP_TRY(hipMemsetAsync(s->d_far_head, 0xFF, sizeof(int32_t) * s->B * dev::NBK, s->stream));
   HIP_TRY(hipMemsetAsync(s->d_far_occ, 0, sizeof(unsigned long long) * s->B * dev::NBW, s->stream));
   std::memset(s->h_ct, 0, sizeof(Counters));

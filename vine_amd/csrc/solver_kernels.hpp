@@ -172,7 +172,8 @@ struct Ctx {
   unsigned long long* q_fin;  // [B] device clock when the query's search ended (workgroup engine)
   unsigned long long* ts;  // [TS_CAP] per-round merge-end wall clock
   Counters* ct;
-  int32_t ellipse;  // reverse bound searches restricted to fwd + dist <= bound (fwd in lbl)
+  int32_t ellipse;  // reverse bound searches restricted to fwd + dist <= bound
+  int32_t fwd_in_head;  // fwd kept in cand_head's low word (workgroup engine), else in lbl
   // scheduler: per-query round statistics, MLP, feature trace
   int32_t stats_on;
   int32_t policy;     // VINE_DELTA_*
@@ -607,9 +608,16 @@ __device__ __forceinline__ void bq_push(const Ctx& c, BoundLds& L, long long qb,
 }
 
 // distance field of search WHICH at (query, vertex) qv
+// The forward search's distance fwd: in lbl's len word, or with fwd_in_head
+// in the low word of cand_head, which the workgroup engine's label search
+// never uses (then lbl needs no reset before the len search).
+__device__ __forceinline__ float* fwd_ptr(const Ctx& c, long long qv) {
+  return c.fwd_in_head ? (float*)&c.cand_head[qv] : &c.lbl[qv].x;
+}
+
 template <int WHICH>
 __device__ __forceinline__ float* bound_dist(const Ctx& c, long long qv) {
-  return WHICH == 3 ? &c.lbl[qv].y : &c.lbl[qv].x;
+  return WHICH == 2 ? fwd_ptr(c, qv) : WHICH == 3 ? &c.lbl[qv].y : &c.lbl[qv].x;
 }
 
 // WHICH = 4 keeps four Lagrangian distances per (query, vertex) in the label
@@ -781,7 +789,7 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
             // with the forward search done, only vertices with fwd + lbd <=
             // bound: every vertex of a feasible s-t path lies there, and so
             // does every vertex of such a vertex's shortest path to t
-            const float lim = c.ellipse ? bnd - c.lbl[wv].x : bnd;
+            const float lim = c.ellipse ? bnd - *fwd_ptr(c, wv) : bnd;
             go[r] = act[r] && nd[r] <= lim && pk[r] < c.lbd[wv];
           }
         }
@@ -1152,7 +1160,7 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
   }
   wg_search<0>(c, q, delta_delay, L, relaxed, r0);
   __syncthreads();
-  if (c.ellipse) {
+  if (c.ellipse && !c.fwd_in_head) {
     // lbl held fwd: back to +inf for the len search
     float2* l = c.lbl + (long long)q * c.n;
     for (int j = threadIdx.x; j < c.n; j += blockDim.x) l[j].x = FINF;
