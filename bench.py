@@ -55,12 +55,15 @@ def cpu_info():
 
 def cpu_baseline(args, g, qb, fx, cost_b, dly_b, feas_b):
     """The oracle timed on the host: one core (queries in sequence) and all
-    threads (one query per thread).  Rows: the grid workload's 100 queries with
-    the reference's own search (restated cspp_cpu, no pruning); a sample of the
-    batched workload with the restated search plus its lower-bound pruning,
-    drawn from the queries that search finishes (its longest queries take
-    minutes to hours: whole-frontier rounds re-expand labels many times over);
-    the label-setting oracle on a uniform sample of the batched workload."""
+    threads (one query per thread), with per-query latencies.  Rows: the grid
+    workload's 100 queries with the reference's own search (restated cspp_cpu,
+    no pruning); samples of the NY-scale random pairs at bounds 1.1x and 1.5x
+    and of the batched workload with the restated search plus its lower-bound
+    pruning, drawn from the queries that search finishes within the limit its
+    fixture records (its longest queries take minutes to hours: whole-frontier
+    rounds re-expand labels many times over), so that no sampled query reaches
+    the cap; the label-setting oracle on a uniform sample of the batched
+    workload."""
     import numpy as np
     from concurrent.futures import ThreadPoolExecutor
 
@@ -71,71 +74,111 @@ def cpu_baseline(args, g, qb, fx, cost_b, dly_b, feas_b):
     threads = max(1, min(16, nproc or 1))
 
     def timed(fn, idx, nthreads):
+        """Wall seconds over idx, and (output, seconds) of every query."""
+        def one(i):
+            t1 = time.perf_counter()
+            r = fn(i)
+            return r, time.perf_counter() - t1
         t1 = time.perf_counter()
         if nthreads == 1:
-            out = [fn(i) for i in idx]
+            out = [one(i) for i in idx]
         else:
             with ThreadPoolExecutor(nthreads) as ex:
-                out = list(ex.map(fn, idx))
-        return time.perf_counter() - t1, out
+                out = list(ex.map(one, idx))
+        return time.perf_counter() - t1, [o[0] for o in out], [o[1] for o in out]
+
+    def ms(secs):
+        a = 1e3 * np.asarray(secs, np.float64)
+        return {"mean": float(a.mean()), "p50": float(np.percentile(a, 50)), "p99": float(np.percentile(a, 99))}
+
+    def restated(gr, q, cost, dly, feas):
+        """Restated cspp_cpu with lower-bound pruning; None at the cap, else
+        whether the answer equals the expected one."""
+        def fn(i):
+            try:
+                r = O.cspp(gr, q.src[i], q.dst[i], q.bound[i], prune_target=2, time_limit=args.cpu_limit)
+            except O.TimeLimit:
+                log("cpu: query %d stopped at the %.0f s cap" % (i, args.cpu_limit))
+                return None
+            return (r.feasible == bool(feas[i])) and (not r.feasible or (
+                np.float32(r.cost) == cost[i] and np.float32(r.delay) == dly[i]))
+        return fn
+
+    def sample_row(fn, idx, pool_note):
+        t_n, out, sec = timed(fn, idx, threads)
+        sub = idx[: max(1, len(idx) // 4)]
+        t_1, out1, sec1 = timed(fn, sub, 1)
+        done = [o for o in out + out1 if o is not None]
+        return {"queries": len(idx), "qps_all": len(idx) / t_n, "qps_1core": len(sub) / t_1,
+                "queries_1core": len(sub), "per_query_ms": ms(sec), "per_query_ms_1core": ms(sec1),
+                "finished": len(done), "runs": len(out) + len(out1), "agree": bool(all(done)),
+                "drawn": pool_note}
 
     res = {"cpu_model": model, "nproc": nproc, "threads": threads}
     # grid workload, reference search
     gg, qg = W.preset("grid64")
     one = lambda i: O.cspp(gg, qg.src[i], qg.dst[i], qg.bound[i])
-    t_1, _ = timed(one, range(len(qg)), 1)
-    t_n, _ = timed(one, range(len(qg)), threads)
-    res["grid64_reference_search"] = {"queries": len(qg), "qps_1core": len(qg) / t_1, "qps_all": len(qg) / t_n}
+    t_1, _, s1 = timed(one, range(len(qg)), 1)
+    t_n, _, sn = timed(one, range(len(qg)), threads)
+    res["grid64_reference_search"] = {"queries": len(qg), "qps_1core": len(qg) / t_1, "qps_all": len(qg) / t_n,
+                                      "per_query_ms_1core": ms(s1), "per_query_ms": ms(sn)}
     log("cpu grid64: %.1f / %.1f queries/s (1 core / %d threads)" % (len(qg) / t_1, len(qg) / t_n, threads))
 
     rng = np.random.default_rng(1)
-    pool = np.arange(len(qb))
-    flagged = fx is not None and int((fx["restated"] == 1).sum()) >= args.cpu_queries
-    if flagged:
-        pool = np.nonzero(fx["restated"] == 1)[0]
+
+    def pool_of(f, nq):
+        """Queries the restated search finished within the fixture's limit
+        (all queries where the fixture holds no such flags)."""
+        if f is not None and int((f["restated"] == 1).sum()) >= 8:
+            pool = np.nonzero(f["restated"] == 1)[0]
+            return pool, ("from the %d of %d queries that the restated search finishes within %.0f s each "
+                          "(the rest take longer, so the rate is an upper bound for the whole workload)"
+                          % (len(pool), nq, float(f["restated_limit_s"])))
+        return np.arange(nq), "uniformly from all %d queries" % nq
+
+    # NY-scale random pairs, bound 1.1x and 1.5x min delay
+    for name, mult in (("ny_1.1x", 11), ("ny_1.5x", 15)):
+        fp = os.path.join(ROOT, "tests", "golden", "answers_%s.npz" % name)
+        if not os.path.exists(fp):
+            continue
+        f2 = np.load(fp)
+        q2 = W.random_queries(g, 1000, (mult, 10), seed=0)
+        if not (np.array_equal(f2["src"], q2.src) and np.array_equal(f2["bound"], q2.bound)):
+            continue
+        pool, note = pool_of(f2, len(q2))
+        idx = rng.choice(pool, min(args.cpu_queries, len(pool)), replace=False)
+        row = sample_row(restated(g, q2, f2["cost"], f2["delay"], f2["feasible"]), idx, note)
+        res["random_pairs_" + name[3:]] = row
+        log("cpu %s: %.2f / %.2f queries/s (1 core / %d threads), %d/%d finished" %
+            (name, row["qps_1core"], row["qps_all"], threads, row["finished"], row["runs"]))
+
+    # the batched workload
+    pool, note = pool_of(fx, len(qb))
     idx = rng.choice(pool, min(args.cpu_queries, len(pool)), replace=False)
-
-    def restated(i):
-        t1 = time.perf_counter()
-        try:
-            r = O.cspp(g, qb.src[i], qb.dst[i], qb.bound[i], prune_target=2, time_limit=args.cpu_limit)
-        except O.TimeLimit:
-            log("cpu: query %d stopped at the %.0f s cap" % (i, args.cpu_limit))
-            return None
-        log("cpu: query %d done in %.1f s" % (i, time.perf_counter() - t1))
-        return (r.feasible == bool(feas_b[i])) and (not r.feasible or (
-            np.float32(r.cost) == cost_b[i] and np.float32(r.delay) == dly_b[i]))
-
-    t_n, out = timed(restated, idx, threads)
-    done = [o for o in out if o is not None]
-    sub = idx[: max(1, len(idx) // 4)]
-    t_1, out1 = timed(restated, sub, 1)
-    value = len(idx) / t_n
-    if flagged:
-        drawn = ("drawn (seed 1) from the %d of %d that the restated cspp_cpu search with lower-bound pruning "
-                 "finishes within %.0f s each (the other queries take minutes to hours each with this search, so "
-                 "the rate is an upper bound for the workload)" % (len(pool), len(qb), float(fx["restated_limit_s"])))
-    else:
-        drawn = ("drawn (seed 1) uniformly from all %d, with the restated cspp_cpu search plus lower-bound "
-                 "pruning; a query stopped at the cap counts at the cap, so the rate is an upper bound" % len(qb))
-    res.update({"value": value, "unit": "queries/s", "cores": threads, "kind": "port",
-                "single_core_value": len(sub) / t_1,
-                "finished": len(done), "finished_agree_with_gpu": bool(all(done)),
-                "sample": "%d queries of the batched workload %s; one query per thread on %d threads "
-                          "(single_core_value: %d of them in sequence on one core); %d/%d finished under the "
-                          "%.0f s cap, every finished answer equal to the GPU's: %s"
-                          % (len(idx), drawn, threads, len(sub), len(done), len(idx), args.cpu_limit,
-                             bool(all(done)))})
-    log("cpu batched sample: %.2f queries/s on %d threads, %.2f on one core" % (value, threads, len(sub) / t_1))
+    row = sample_row(restated(g, qb, cost_b, dly_b, feas_b), idx, note)
+    capped = row["runs"] - row["finished"]
+    res.update({"value": row["qps_all"], "unit": "queries/s", "cores": threads, "kind": "port",
+                "single_core_value": row["qps_1core"], "finished": row["finished"], "runs": row["runs"],
+                "finished_agree_with_gpu": row["agree"], "per_query_ms": row["per_query_ms"],
+                "per_query_ms_1core": row["per_query_ms_1core"],
+                "sample": "%d queries of the batched workload drawn (seed 1) %s, with the restated cspp_cpu "
+                          "search plus lower-bound pruning; one query per thread on %d threads "
+                          "(single_core_value: %d of them in sequence on one core); %d of %d runs finished "
+                          "under the %.0f s cap%s, every finished answer equal to the GPU's: %s"
+                          % (len(idx), note, threads, row["queries_1core"], row["finished"], row["runs"],
+                             args.cpu_limit, "" if capped == 0 else " (a run stopped at the cap counts at "
+                             "the cap)", row["agree"])})
+    log("cpu batched sample: %.2f queries/s on %d threads, %.2f on one core" %
+        (row["qps_all"], threads, row["qps_1core"]))
 
     # label-setting oracle: every query finishes, uniform sample
     uidx = rng.choice(len(qb), min(4 * args.cpu_queries, len(qb)), replace=False)
     ls = lambda i: O.cspp_astar(g, qb.src[i], qb.dst[i], qb.bound[i])
-    t_n, outl = timed(ls, uidx, threads)
+    t_n, outl, sl = timed(ls, uidx, threads)
     agree = all((r.feasible == bool(feas_b[i])) and (not r.feasible or (
         np.float32(r.cost) == cost_b[i] and np.float32(r.delay) == dly_b[i])) for r, i in zip(outl, uidx))
     res["label_setting"] = {"queries": len(uidx), "qps_all": len(uidx) / t_n, "threads": threads,
-                            "agree_with_gpu": agree,
+                            "per_query_ms": ms(sl), "agree_with_gpu": agree,
                             "note": "uniform sample of the batched workload; lexicographic bi-criteria A* with "
                                     "exact reverse-Dijkstra bounds (two full Dijkstras per query dominate)"}
     log("cpu label-setting: %.2f queries/s on %d threads" % (len(uidx) / t_n, threads))

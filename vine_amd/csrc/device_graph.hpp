@@ -16,6 +16,9 @@ struct vine_graph {
   float max_len = 0.f;
   float max_delay = 0.f;
   bool integer_weights = true;  // every len and delay an integer below 2^24 (path sums exact)
+  // every arc u->v has a twin v->u of no larger delay, so the min delay from
+  // t to any v is at most the min delay from v to t
+  bool twin_delay = false;
   int device = 0;
   // vertices are renumbered for locality (layout.cpp): perm[caller id] =
   // internal id, d_inv the inverse; every array below uses internal ids

@@ -330,6 +330,20 @@ vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, cons
         delay[p] >= 16777216.f)
       g->integer_weights = false;
   }
+  {
+    // twins: the smallest delay of each (tail, head) pair, looked up reversed
+    std::vector<std::pair<uint64_t, float>> arcs((size_t)m);
+    for (int32_t u = 0; u < n; u++)
+      for (int32_t p = row_ptr[u]; p < row_ptr[u + 1]; p++)
+        arcs[p] = {((uint64_t)(uint32_t)u << 32) | (uint32_t)dst[p], delay[p]};
+    std::sort(arcs.begin(), arcs.end());
+    g->twin_delay = true;
+    for (int64_t p = 0; p < m && g->twin_delay; p++) {
+      const uint64_t rk = (arcs[p].first << 32) | (arcs[p].first >> 32);
+      auto it = std::lower_bound(arcs.begin(), arcs.end(), std::make_pair(rk, -1.f));
+      g->twin_delay = it != arcs.end() && it->first == rk && it->second <= arcs[p].second;
+    }
+  }
   hipError_t err = hipGetDevice(&g->device);
   size_t eb = sizeof(int4) * std::max<int64_t>(m, 1);
   if (err == hipSuccess) err = hipMalloc(&g->d_row, sizeof(int32_t) * (n + 1));
@@ -668,6 +682,7 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
   c.ct = s->d_ct;
   c.ellipse = o.bound_ball ? 0 : 1;
   c.fwd_in_head = s->wg ? 1 : 0;
+  c.fwd_cut = g->twin_delay && g->integer_weights ? 1 : 0;
   return VINE_OK;
 }
 

@@ -174,6 +174,7 @@ struct Ctx {
   Counters* ct;
   int32_t ellipse;  // reverse bound searches restricted to fwd + dist <= bound
   int32_t fwd_in_head;  // fwd kept in cand_head's low word (workgroup engine), else in lbl
+  int32_t fwd_cut;      // twin arcs and integer weights: fwd stops at (bound + fwd(t)) / 2
   // scheduler: per-query round statistics, MLP, feature trace
   int32_t stats_on;
   int32_t policy;     // VINE_DELTA_*
@@ -542,7 +543,14 @@ __global__ void k_init_batch(Ctx c, int nq) {
 // of the round's smallest frontier distance and carries the rest, which keeps
 // re-relaxation close to Dijkstra's.
 //   WHICH = 2: forward min delay fwd from s (on the graph itself, into lbl),
-//   limited to radius `bound`.  Run first when c.ellipse is set.
+//   limited to radius `bound`.  Run first when c.ellipse is set.  With
+//   c.fwd_cut the radius shrinks to (bound + D) / 2, D = fwd(t): a twin arc
+//   of no larger delay for every arc makes the min delay from t to v at most
+//   lbd(v), so fwd(v) <= D + lbd(v), and a vertex with fwd + lbd <= bound has
+//   fwd <= (bound + D) / 2.  The current fwd(t) is an upper bound on D, so
+//   the radius from it is safe at every round, and integer delays (below
+//   2^24, path sums exact) make the floor of the half exact.  A vertex left
+//   with a larger tentative fwd is outside the region all the same.
 //   WHICH = 0: delay to t on the transposed graph, limited to radius `bound`,
 //   or with c.ellipse to fwd + lbd <= bound.  A label at u has delay >=
 //   fwd(u), so outside that region every label fails the delay test and
@@ -649,6 +657,7 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
   const int wid = tid / WAVE;
   const int32_t* rp = (WHICH == 2) ? c.row_ptr : c.rrow_ptr;
   const int4* ep = (WHICH == 2) ? c.edges : c.redges;
+  const int tq = c.q_dst[q];
   if (tid == 0) {
     const int root = (WHICH == 2) ? c.q_src[q] : c.q_dst[q];
     if (WHICH == 0) c.lbd[qb + root] = pack2(0.f, 0.f);
@@ -676,6 +685,11 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
     rounds = round + 1;
     const int nxt = cur ^ 1;
     const float lim_key = __uint_as_float(L.kmin[cur]) + delta;
+    float flim = bnd;  // WHICH 2: radius of the forward search this round
+    if (WHICH == 2 && c.fwd_cut && bnd < 16777216.f) {
+      const float dt = fminf(load_agent(fwd_ptr(c, qb + tq)), bnd);
+      flim = (float)floor(0.5 * ((double)bnd + (double)dt));
+    }
     const uint32_t stamp = L.stamp;
     const int32_t* qin = que[cur];
     int32_t* qout = que[nxt];
@@ -764,7 +778,7 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
           pk[r] = 0ull;
           if (WHICH == 2) {
             nd[r] = odu + __int_as_float(ed[r].z);
-            go[r] = act[r] && nd[r] <= bnd && nd[r] < *bound_dist<2>(c, wv);
+            go[r] = act[r] && nd[r] <= flim && nd[r] < *bound_dist<2>(c, wv);
           } else if (WHICH == 1) {
             nd[r] = odu + __int_as_float(ed[r].y);
             go[r] = act[r] && lbd_of(c, wv) <= bnd && nd[r] < *bound_dist<1>(c, wv);
