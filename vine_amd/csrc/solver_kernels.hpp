@@ -577,7 +577,8 @@ constexpr int BOUND_BLOCK = 512;
 // such an entry is carried once and queued again with its current key); the
 // distance it is relaxed with is read when it is taken, so a vertex lowered
 // after it was queued is relaxed with its newest distance.  Those reads and
-// the arc reads are issued together.  HBM entries hold only the vertex.
+// the arc reads are issued together.  HBM entries hold only the vertex and
+// are taken in the round that reads them.
 constexpr int BQ_LDS = 1024;
 constexpr int BQ_HASH = 2048;
 constexpr int BQ_PROBES = 8;
@@ -722,8 +723,10 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
         } else {
           du = load_agent(bound_dist<WHICH>(c, qv));
         }
-        // taken by its key when queued (HBM entries: by the distance)
-        carry = (ent.w ? du : __int_as_float(ent.z)) > lim_key;
+        // taken by its key when queued, so the arc reads need not wait for
+        // the distance; HBM entries keep no key and are taken (relaxing a
+        // vertex early only costs work: it is queued again when lowered)
+        carry = !ent.w && __int_as_float(ent.z) > lim_key;
         if (!carry) {
           if (row_known((uint32_t)ent.y)) {
             row = (int)((uint32_t)ent.y >> 8);
