@@ -90,6 +90,9 @@ struct vine_solver {
   float2* d_lbl = nullptr;  // {lbl, lbw} per (query, vertex)
   float* d_qlk = nullptr;
   unsigned long long* d_lbd = nullptr;
+  float* d_fwd = nullptr;
+  uint32_t* d_ell = nullptr;
+  int32_t ell_words = 0;
   uint32_t* d_inq = nullptr;
   int32_t* d_wq = nullptr;
   int32_t* d_qorder = nullptr;
@@ -500,11 +503,14 @@ void init_fields(vine_solver* s, const vine_graph* g, const vine_options& o) {
   s->lpw = o.labels_per_wave ? o.labels_per_wave : 64;
 }
 
-// Per-(query, vertex) state: pset, cand_head; with the bounds lbl, lbd, the
-// bound searches' queue stamp and the HBM part of their two queues.
+// Per-(query, vertex) state: pset, cand_head; with the bounds lbl, lbd, fwd,
+// the bound searches' queue stamp and the HBM part of their two queues (and
+// a bit of the region bitmap, counted as a byte).
 size_t per_qv_bytes(const vine_options& o) {
   return sizeof(int2) + sizeof(unsigned long long) +
-         (o.prune == 2 ? sizeof(float2) + sizeof(unsigned long long) + sizeof(uint32_t) + 2 * sizeof(int32_t) : 0);
+         (o.prune == 2 ? sizeof(float2) + sizeof(unsigned long long) + sizeof(float) + sizeof(uint32_t) +
+                             2 * sizeof(int32_t) + 1
+                       : 0);
 }
 
 struct Caps {
@@ -560,6 +566,9 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
     ALLOC(s->d_lbl, sizeof(float2) * BN);
     ALLOC(s->d_qlk, sizeof(float) * B);
     ALLOC(s->d_lbd, sizeof(unsigned long long) * BN);
+    ALLOC(s->d_fwd, sizeof(float) * BN);
+    s->ell_words = (int32_t)((g->n + 31) / 32);
+    ALLOC(s->d_ell, sizeof(uint32_t) * s->ell_words * B);
     ALLOC(s->d_inq, sizeof(uint32_t) * BN);
     ALLOC(s->d_wq, sizeof(int32_t) * 2 * BN);
     ALLOC(s->d_qorder, sizeof(int32_t) * B);
@@ -646,6 +655,9 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
   c.q_lk = s->d_qlk;
   c.lagrange = o.prune == 2 ? o.lagrange_bound : 0;
   c.lbd = s->d_lbd;
+  c.fwd = s->d_fwd;
+  c.ell = s->d_ell;
+  c.ell_words = s->ell_words;
   c.inq = s->d_inq;
   c.wq = s->d_wq;
   c.q_order = nullptr;  // set per batch when the slots are sorted
@@ -681,7 +693,6 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
   c.ts = s->d_ts;
   c.ct = s->d_ct;
   c.ellipse = o.bound_ball ? 0 : 1;
-  c.fwd_in_head = s->wg ? 1 : 0;
   c.fwd_cut = g->twin_delay && g->integer_weights ? 1 : 0;
   return VINE_OK;
 }
@@ -1020,14 +1031,9 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
   const long long BN = (long long)s->B * c.n;
   // reset per-(query, vertex) headers and counters
   HIP_TRY(hipMemsetAsync(s->d_pset, 0, sizeof(int2) * BN, s->stream));
-  if (s->wg && s->opt.prune == 2) {
-    // the workgroup engine's label search never reads candidate lists: the
-    // words hold the forward search's distances (+inf to start)
-    vine_status fs = fill_f32(s, (float*)s->d_cand_head, dev::FINF, 2 * BN);
-    if (fs != VINE_OK) return fs;
-  } else {
-    HIP_TRY(hipMemsetAsync(s->d_cand_head, 0xFF, sizeof(unsigned long long) * BN, s->stream));
-  }
+  // the workgroup engine never reads candidate lists (the four-multiplier
+  // pass sets the words it uses itself)
+  if (!s->wg) HIP_TRY(hipMemsetAsync(s->d_cand_head, 0xFF, sizeof(unsigned long long) * BN, s->stream));
   HIP_TRY(hipMemsetAsync(s->d_far_head, 0xFF, sizeof(int32_t) * s->B * dev::NBK, s->stream));
   HIP_TRY(hipMemsetAsync(s->d_far_occ, 0, sizeof(unsigned long long) * s->B * dev::NBW, s->stream));
   std::memset(s->h_ct, 0, sizeof(Counters));
@@ -1055,7 +1061,9 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
   if (s->opt.prune == 2) {
     vine_status fs = fill_f32(s, (float*)s->d_lbd, dev::FINF, 2 * BN);
     if (fs == VINE_OK) fs = fill_f32(s, (float*)s->d_lbl, dev::FINF, 2 * BN);
+    if (fs == VINE_OK && c.ellipse) fs = fill_f32(s, s->d_fwd, dev::FINF, BN);
     if (fs != VINE_OK) return fs;
+    HIP_TRY(hipMemsetAsync(s->d_ell, 0, sizeof(uint32_t) * s->ell_words * s->B, s->stream));
     HIP_TRY(hipMemsetAsync(s->d_inq, 0, sizeof(uint32_t) * BN, s->stream));
     vine_status ss = run_bounds(s, nb, st);
     if (ss != VINE_OK) return ss;

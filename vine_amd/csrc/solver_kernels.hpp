@@ -173,7 +173,9 @@ struct Ctx {
   unsigned long long* ts;  // [TS_CAP] per-round merge-end wall clock
   Counters* ct;
   int32_t ellipse;  // reverse bound searches restricted to fwd + dist <= bound
-  int32_t fwd_in_head;  // fwd kept in cand_head's low word (workgroup engine), else in lbl
+  float* fwd;           // [B][n] forward search's min delay from s (+inf: not reached)
+  uint32_t* ell;        // [B][ell_words] bit v: lbd(v) is finite (v lies in the bound region)
+  int32_t ell_words;
   int32_t fwd_cut;      // twin arcs and integer weights: fwd stops at (bound + fwd(t)) / 2
   // scheduler: per-query round statistics, MLP, feature trace
   int32_t stats_on;
@@ -397,6 +399,8 @@ __device__ __forceinline__ unsigned long long pack2(float hi, float lo) {
 __device__ __forceinline__ float hi_of(unsigned long long p) { return __uint_as_float((uint32_t)(p >> 32)); }
 __device__ __forceinline__ float lo_of(unsigned long long p) { return __uint_as_float((uint32_t)p); }
 __device__ __forceinline__ float lbd_of(const Ctx& c, long long qv) { return hi_of(c.lbd[qv]); }
+// lbd before its search reaches the vertex: both words +inf
+constexpr unsigned long long LBD_NONE = 0x7F8000007F800000ull;
 
 // Key and lower bound on the final cost of any feasible completion of a label
 // (d, delay) at a vertex with bounds lw = {lbl, lbw}, for a query with
@@ -542,7 +546,7 @@ __global__ void k_init_batch(Ctx c, int nq) {
 // Each round relaxes the frontier vertices whose distance is within `delta`
 // of the round's smallest frontier distance and carries the rest, which keeps
 // re-relaxation close to Dijkstra's.
-//   WHICH = 2: forward min delay fwd from s (on the graph itself, into lbl),
+//   WHICH = 2: forward min delay fwd from s (on the graph itself, into fwd),
 //   limited to radius `bound`.  Run first when c.ellipse is set.  With
 //   c.fwd_cut the radius shrinks to (bound + D) / 2, D = fwd(t): a twin arc
 //   of no larger delay for every arc makes the min delay from t to v at most
@@ -558,7 +562,9 @@ __global__ void k_init_batch(Ctx c, int nq) {
 //   distance lexicographic; its len part is the length of a min-delay path.
 //   WHICH = 1: len to t, restricted to vertices with lbd <= bound: only those
 //   can lie on a feasible completion, so the restricted distance is a valid
-//   lower bound for every label that passes the delay test.
+//   lower bound for every label that passes the delay test.  The region is
+//   the set of vertices the search of lbd reached (it only lowers lbd to
+//   values within the bound), kept as a bitmap (in_region).
 //   WHICH = 3: the Lagrangian distance lbw to t under the integer weights
 //   LSCALE * len + lk * delay, with the same restriction, as unsigned integers
 //   in the bits of lbl.y (sums saturate at LBW_NONE, which is never stored);
@@ -617,11 +623,16 @@ __device__ __forceinline__ void bq_push(const Ctx& c, BoundLds& L, long long qb,
 }
 
 // distance field of search WHICH at (query, vertex) qv
-// The forward search's distance fwd: in lbl's len word, or with fwd_in_head
-// in the low word of cand_head, which the workgroup engine's label search
-// never uses (then lbl needs no reset before the len search).
-__device__ __forceinline__ float* fwd_ptr(const Ctx& c, long long qv) {
-  return c.fwd_in_head ? (float*)&c.cand_head[qv] : &c.lbl[qv].x;
+// The forward search's distance fwd, 4 B per (query, vertex): a 128-B line
+// holds 32 vertices' values.
+__device__ __forceinline__ float* fwd_ptr(const Ctx& c, long long qv) { return &c.fwd[qv]; }
+
+// Whether v lies in query q's bound region, i.e. was reached by the search
+// of lbd (whose values are all <= the bound): one bit per vertex, set when
+// lbd(v) first drops from +inf, so the later searches' region test reads a
+// line that covers 1,024 vertices instead of v's 8-B lbd.
+__device__ __forceinline__ bool in_region(const Ctx& c, int q, int v) {
+  return (c.ell[(long long)q * c.ell_words + (v >> 5)] >> (v & 31)) & 1u;
 }
 
 template <int WHICH>
@@ -661,7 +672,10 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
   const int tq = c.q_dst[q];
   if (tid == 0) {
     const int root = (WHICH == 2) ? c.q_src[q] : c.q_dst[q];
-    if (WHICH == 0) c.lbd[qb + root] = pack2(0.f, 0.f);
+    if (WHICH == 0) {
+      c.lbd[qb + root] = pack2(0.f, 0.f);
+      atomicOr(&c.ell[(long long)q * c.ell_words + (root >> 5)], 1u << (root & 31));
+    }
     else if (WHICH == 4)
       for (int j = 0; j < 4; j++) *grid_field(c, qb + root, j) = 0u;
     else *bound_dist<WHICH>(c, qb + root) = 0.f;
@@ -784,15 +798,15 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
             go[r] = act[r] && nd[r] <= flim && nd[r] < *bound_dist<2>(c, wv);
           } else if (WHICH == 1) {
             nd[r] = odu + __int_as_float(ed[r].y);
-            go[r] = act[r] && lbd_of(c, wv) <= bnd && nd[r] < *bound_dist<1>(c, wv);
+            go[r] = act[r] && in_region(c, q, ed[r].x) && nd[r] < *bound_dist<1>(c, wv);
           } else if (WHICH == 3) {
             const uint32_t odub = (uint32_t)__shfl((int)dub, own[r], WAVE);
             const unsigned long long s64 = (unsigned long long)odub + lagr_weight(ed[r], lk);
             ndg[r][0] = s64 < (unsigned long long)LBW_NONE ? (uint32_t)s64 : LBW_NONE;
             nd[r] = (float)ndg[r][0];
-            go[r] = act[r] && lbd_of(c, wv) <= bnd && ndg[r][0] < *(const uint32_t*)bound_dist<3>(c, wv);
+            go[r] = act[r] && in_region(c, q, ed[r].x) && ndg[r][0] < *(const uint32_t*)bound_dist<3>(c, wv);
           } else if (WHICH == 4) {
-            go[r] = act[r] && lbd_of(c, wv) <= bnd;
+            go[r] = act[r] && in_region(c, q, ed[r].x);
 #pragma unroll
             for (int j = 0; j < 4; j++) {
               const uint32_t od = (uint32_t)__shfl((int)dg[j], own[r], WAVE);
@@ -843,6 +857,8 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
           } else {
             const unsigned long long old = atomicMin(&c.lbd[wv], pk[r]);
             lowered[r] = pk[r] < old;
+            if (lowered[r] && old == LBD_NONE)
+              atomicOr(&c.ell[(long long)q * c.ell_words + (ed[r].x >> 5)], 1u << (ed[r].x & 31));
           }
         }
 #pragma unroll
@@ -1177,11 +1193,6 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
   }
   wg_search<0>(c, q, delta_delay, L, relaxed, r0);
   __syncthreads();
-  if (c.ellipse && !c.fwd_in_head) {
-    // lbl held fwd: back to +inf for the len search
-    float2* l = c.lbl + (long long)q * c.n;
-    for (int j = threadIdx.x; j < c.n; j += blockDim.x) l[j].x = FINF;
-  }
   __threadfence();
   __syncthreads();
   wg_search<1>(c, q, delta_len, L, relaxed, r1);
