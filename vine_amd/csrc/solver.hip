@@ -87,7 +87,8 @@ struct vine_solver {
   // per (query, vertex)
   int2* d_pset = nullptr;
   unsigned long long* d_cand_head = nullptr;
-  float2* d_lbl = nullptr;  // {lbl, lbw} per (query, vertex)
+  float* d_lbl = nullptr;  // lbl per (query, vertex)
+  float* d_lbw = nullptr;  // lbw per (query, vertex)
   float* d_qlk = nullptr;
   unsigned long long* d_lbd = nullptr;
   float* d_fwd = nullptr;
@@ -563,7 +564,8 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
   ALLOC(s->d_pset, sizeof(int2) * BN);
   ALLOC(s->d_cand_head, sizeof(unsigned long long) * BN);
   if (o.prune == 2) {
-    ALLOC(s->d_lbl, sizeof(float2) * BN);
+    ALLOC(s->d_lbl, sizeof(float) * BN);
+    ALLOC(s->d_lbw, sizeof(float) * BN);
     ALLOC(s->d_qlk, sizeof(float) * B);
     ALLOC(s->d_lbd, sizeof(unsigned long long) * BN);
     ALLOC(s->d_fwd, sizeof(float) * BN);
@@ -652,6 +654,7 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
   c.pset = s->d_pset;
   c.cand_head = s->d_cand_head;
   c.lbl = s->d_lbl;
+  c.lbw = s->d_lbw;
   c.q_lk = s->d_qlk;
   c.lagrange = o.prune == 2 ? o.lagrange_bound : 0;
   c.lbd = s->d_lbd;
@@ -1060,7 +1063,8 @@ vine_status solve_batch(vine_solver* s, int nb, int q0, vine_stats* st) {
   }
   if (s->opt.prune == 2) {
     vine_status fs = fill_f32(s, (float*)s->d_lbd, dev::FINF, 2 * BN);
-    if (fs == VINE_OK) fs = fill_f32(s, (float*)s->d_lbl, dev::FINF, 2 * BN);
+    if (fs == VINE_OK) fs = fill_f32(s, s->d_lbl, dev::FINF, BN);
+    if (fs == VINE_OK) fs = fill_f32(s, s->d_lbw, dev::FINF, BN);
     if (fs == VINE_OK && c.ellipse) fs = fill_f32(s, s->d_fwd, dev::FINF, BN);
     if (fs != VINE_OK) return fs;
     HIP_TRY(hipMemsetAsync(s->d_ell, 0, sizeof(uint32_t) * s->ell_words * s->B, s->stream));

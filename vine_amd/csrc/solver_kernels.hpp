@@ -111,10 +111,12 @@ struct Ctx {
   // per (query, vertex) candidate list head: chunk index << 32 | records
   // claimed in that chunk (may exceed CCHUNK; ~0 = empty)
   unsigned long long* cand_head;
-  // {lbl, lbw}: lbl = length to t (restricted to lbd <= bound); lbw = the
-  // Lagrangian distance to t under weights LSCALE * len + q_lk[q] * delay
-  // (same restriction; +inf where not searched or q_lk[q] = 0)
-  float2* lbl;
+  // lbl = length to t (restricted to lbd <= bound); lbw = the Lagrangian
+  // distance to t under weights LSCALE * len + q_lk[q] * delay (same
+  // restriction; +inf where not searched or q_lk[q] = 0), as integer bits.
+  // Two 4-B arrays: a bound search's line holds 32 vertices' values.
+  float* lbl;
+  float* lbw;
   float* q_lk;       // [B] Lagrange multiplier x LSCALE of the query (0 = none)
   int32_t lagrange;  // multipliers tried by the Lagrangian bound searches (0: no lbw)
   unsigned long long* lbd;  // packed (delay bits << 32 | len bits): lexicographic
@@ -399,6 +401,8 @@ __device__ __forceinline__ unsigned long long pack2(float hi, float lo) {
 __device__ __forceinline__ float hi_of(unsigned long long p) { return __uint_as_float((uint32_t)(p >> 32)); }
 __device__ __forceinline__ float lo_of(unsigned long long p) { return __uint_as_float((uint32_t)p); }
 __device__ __forceinline__ float lbd_of(const Ctx& c, long long qv) { return hi_of(c.lbd[qv]); }
+// {lbl, lbw} at (query, vertex) qv, as the label keys take them
+__device__ __forceinline__ float2 lbl_pair(const Ctx& c, long long qv) { return make_float2(c.lbl[qv], c.lbw[qv]); }
 // lbd before its search reaches the vertex: both words +inf
 constexpr unsigned long long LBD_NONE = 0x7F8000007F800000ull;
 
@@ -483,7 +487,7 @@ __global__ void k_init_batch(Ctx c, int nq) {
     int s = c.q_src[q];
     long long qs = (long long)q * c.n + s;
     float key = 0.f;
-    if (PRUNE == 2) key = lower_key(0.f, 0.f, c.lbl[qs], c.q_lk[q], c.q_bound[q]);
+    if (PRUNE == 2) key = lower_key(0.f, 0.f, lbl_pair(c, qs), c.q_lk[q], c.q_bound[q]);
     c.labs[q] = make_int4(s, __float_as_int(0.f), __float_as_int(0.f), -1);
     c.lmeta[q] = make_int2(q, __float_as_int(key));
     c.dead[q] = 0;
@@ -637,7 +641,7 @@ __device__ __forceinline__ bool in_region(const Ctx& c, int q, int v) {
 
 template <int WHICH>
 __device__ __forceinline__ float* bound_dist(const Ctx& c, long long qv) {
-  return WHICH == 2 ? fwd_ptr(c, qv) : WHICH == 3 ? &c.lbl[qv].y : &c.lbl[qv].x;
+  return WHICH == 2 ? fwd_ptr(c, qv) : WHICH == 3 ? &c.lbw[qv] : &c.lbl[qv];
 }
 
 // WHICH = 4 keeps four Lagrangian distances per (query, vertex) in the label
@@ -917,8 +921,8 @@ __device__ bool walk_tight(const Ctx& c, int q, float k, double& sum_len, double
   const long long qb = (long long)q * c.n;
   const int t = c.q_dst[q];
   int v = c.q_src[q];
-  float lv = c.lbl[qb + v].x;
-  uint32_t wv = WHICH == 4 ? *grid_field(c, qb + v, j) : __float_as_uint(c.lbl[qb + v].y);
+  float lv = c.lbl[qb + v];
+  uint32_t wv = WHICH == 4 ? *grid_field(c, qb + v, j) : __float_as_uint(c.lbw[qb + v]);
   sum_len = 0.0;
   sum_delay = 0.0;
   for (int step = 0; v != t; step++) {
@@ -934,7 +938,7 @@ __device__ bool walk_tight(const Ctx& c, int q, float k, double& sum_len, double
       bool tight = false;
       if (p < r1) {
         e = c.edges[p];
-        lw = c.lbl[qb + e.x];
+        lw = lbl_pair(c, qb + e.x);
         if (WHICH == 4) lw.y = __uint_as_float(*grid_field(c, qb + e.x, j));
         if (WHICH == 1) {
           tight = lw.x < FINF && __int_as_float(e.y) + lw.x == lv;
@@ -973,7 +977,7 @@ __device__ void lagr_begin(const Ctx& c, int q, LagrState& ls) {
   const long long qb = (long long)q * c.n;
   const int s = c.q_src[q];
   const float bnd = c.q_bound[q];
-  const float L1 = c.lbl[qb + s].x;
+  const float L1 = c.lbl[qb + s];
   const unsigned long long pk = c.lbd[qb + s];
   const float D0 = hi_of(pk), UB = lo_of(pk);
   double wl = 0.0, wd = 0.0;
@@ -999,7 +1003,7 @@ __device__ void lagr_step(const Ctx& c, int q, LagrState& ls) {
   const int s = c.q_src[q];
   const float bnd = c.q_bound[q];
   const float k = ls.k;
-  const uint32_t ws = __float_as_uint(c.lbl[qb + s].y);
+  const uint32_t ws = __float_as_uint(c.lbw[qb + s]);
   double wl = 0.0, wd = 0.0;
   const bool ok = ws < LBW_NONE && walk_tight<3>(c, q, k, wl, wd);
   if (lane_id() == 0) {
@@ -1145,7 +1149,7 @@ __device__ int lagr_grid(const Ctx& c, int q, LagrState& ls, float delta_len, fl
   }
   __syncthreads();
   const float kh = s_kh;
-  float2* lq = c.lbl + qb;
+  float* lq = c.lbw + qb;
   bool from_grid = true;
   if (kh > 0.f) {
     // hull step into lbw (still unreached from the batch reset)
@@ -1154,7 +1158,7 @@ __device__ int lagr_grid(const Ctx& c, int q, LagrState& ls, float delta_len, fl
     __threadfence();
     __syncthreads();
     if (tid == 0) {
-      const uint32_t ws = __float_as_uint(lq[s].y);
+      const uint32_t ws = __float_as_uint(lq[s]);
       const double lb = ws < LBW_NONE ? ((double)ws - (double)kh * (double)bnd) / (double)LSCALE : -1e300;
       if (lb >= ls.best_lb) {
         ls.best_lb = lb;
@@ -1170,7 +1174,7 @@ __device__ int lagr_grid(const Ctx& c, int q, LagrState& ls, float delta_len, fl
   // no bound was found), then the label search's words back to their reset
   // values
   for (int j = tid; j < c.n; j += blockDim.x) {
-    if (from_grid) lq[j].y = jb >= 0 ? __uint_as_float(*grid_field(c, qb + j, jb)) : __uint_as_float(LBW_NONE);
+    if (from_grid) lq[j] = jb >= 0 ? __uint_as_float(*grid_field(c, qb + j, jb)) : __uint_as_float(LBW_NONE);
     c.pset[qb + j] = make_int2(0, 0);
     c.cand_head[qb + j] = ~0ull;
   }
@@ -1203,7 +1207,7 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
     __syncthreads();
     if (threadIdx.x < WAVE) lagr_begin(c, q, ls);
     __syncthreads();
-    float2* lq = c.lbl + (long long)q * c.n;
+    float* lq = c.lbw + (long long)q * c.n;
     if (c.lagrange == LAGR_GRID) {
       if (!ls.stop) r3 = lagr_grid(c, q, ls, delta_len, delta_delay, L, relaxed);
       if (threadIdx.x == 0) c.q_lk[q] = ls.stop ? 0.f : ls.best_k;
@@ -1215,7 +1219,7 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
       const float k = ls.k;
       if (it > 0) {
         // the last search's lbw is replaced
-        for (int j = threadIdx.x; j < c.n; j += blockDim.x) lq[j].y = __uint_as_float(LBW_NONE);
+        for (int j = threadIdx.x; j < c.n; j += blockDim.x) lq[j] = __uint_as_float(LBW_NONE);
         __threadfence();
         __syncthreads();
       }
@@ -1231,7 +1235,7 @@ __global__ void __launch_bounds__(BOUND_BLOCK) k_bounds(Ctx c, int nq, float del
     if (bk != ls.last_k) {
       // lbw in memory belongs to another multiplier: cleared, and searched
       // again under the best one if there is one
-      for (int j = threadIdx.x; j < c.n; j += blockDim.x) lq[j].y = __uint_as_float(LBW_NONE);
+      for (int j = threadIdx.x; j < c.n; j += blockDim.x) lq[j] = __uint_as_float(LBW_NONE);
       __threadfence();
       __syncthreads();
       if (bk > 0.f) {
@@ -1692,7 +1696,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_expand(Ctx c, int lpw) {
         const int2 ps = c.pset[qv];
         hw = __hip_atomic_load(&c.cand_head[qv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const float lbdv = PRUNE == 2 ? lbd_of(c, qv) : 0.f;
-        const float2 lw = PRUNE == 2 ? c.lbl[qv] : make_float2(0.f, FINF);
+        const float2 lw = PRUNE == 2 ? lbl_pair(c, qv) : make_float2(0.f, FINF);
         if (ok && PRUNE == 1 && son.vid != c.q_dst[oq] && lex_ge(son.d, son.delay, b.x, b.y)) ok = false;
         if (ok && PRUNE == 2) {
           const float rd = son.delay + lbdv;
@@ -1837,7 +1841,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_merge(Ctx c) {
       cap = pset_cap(ps);
       tq = c.q_dst[q];
       if (PRUNE == 2) {
-        lb = c.lbl[qv];
+        lb = lbl_pair(c, qv);
         qlk = c.q_lk[q];
         qbnd = c.q_bound[q];
       }

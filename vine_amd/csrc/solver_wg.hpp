@@ -611,7 +611,7 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
         const long long qv = qb + son.vid;
         const float srd = son.delay + lbd_of(c, qv);
         if (srd > C.delay_L) return;
-        if (lex_ge(lower_key(son.d, son.delay, c.lbl[qv], lk, bnd), srd, best.x, best.y)) return;
+        if (lex_ge(lower_key(son.d, son.delay, lbl_pair(c, qv), lk, bnd), srd, best.x, best.y)) return;
         const int2 ps = c.pset[qv];
         if (front_dominates<Label_t>(c.pool_cd + (uint32_t)ps.x, pset_cnt(ps), son.vid, son)) return;
         n_cand++;
@@ -657,7 +657,7 @@ __global__ void __launch_bounds__(WG_BLOCK) k_search(Ctx c, int nq) {
         const int2 ps = c.pset[qv];
         uint32_t off = (uint32_t)ps.x;
         int cnt = pset_cnt(ps), cap = pset_cap(ps);
-        const float2 lb = c.lbl[qv];
+        const float2 lb = lbl_pair(c, qv);
         const float lbdv = lbd_of(c, qv);
         // the candidates in list order, folded in chunks that fit the
         // registers (front + chunk <= FC) and the stage (chunk <= FCC)
