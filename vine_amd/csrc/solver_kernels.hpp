@@ -571,7 +571,7 @@ __global__ void k_init_batch(Ctx c, int nq) {
 //   values within the bound), kept as a bitmap (in_region).
 //   WHICH = 3: the Lagrangian distance lbw to t under the integer weights
 //   LSCALE * len + lk * delay, with the same restriction, as unsigned integers
-//   in the bits of lbl.y (sums saturate at LBW_NONE, which is never stored);
+//   in the bits of lbw (sums saturate at LBW_NONE, which is never stored);
 //   the window works on their float values.
 constexpr int BOUND_BLOCK = 512;
 // Frontier of a bound search: the first BQ_LDS entries of each round's queue
