@@ -3,9 +3,9 @@
 Workload (one "step"): 4,096 queries on the 264,346-vertex 3-NN geometric
 graph, hop distances log-uniform in [10, 2000] (truncated at the graph's hop
 diameter), bounds floor(U[1.05, 2.0] x min delay), solved by the batched
-solver in one pass (all queries share every round launch) with the learned
-per-query window (vine_amd/models/delta_mlp.npy), split over the internal
-solvers on separate HIP streams and the launch shape that the autotune cache
+solver in one pass with the learned per-query window
+(vine_amd/models/delta_mlp.npy), with the engine, internal solvers (HIP
+streams) and launch shape that the autotune cache
 (vine_amd/models/autotune.json) holds for this graph and batch.  Inputs are in HBM before
 the timed region; answers stay in HBM.  Beside it: the plain single-query
 solver (one query per solve, fixed window) on the random-pair sweeps (bound
