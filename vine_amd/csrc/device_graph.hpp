@@ -29,4 +29,8 @@ struct vine_graph {
   int4* d_edges = nullptr;
   int32_t* d_rrow = nullptr;
   int4* d_redges = nullptr;
+  // min delays from four landmarks to every vertex (-1: unreachable), for the
+  // forward bound search's region test; null unless every delay is an
+  // integer and every such distance is below 2^24 (exact as floats)
+  float4* d_lmk = nullptr;
 };

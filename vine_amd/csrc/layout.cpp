@@ -110,6 +110,56 @@ std::vector<int32_t> locality_order(int32_t n, const int32_t* row, const int32_t
   return order;
 }
 
+namespace {
+
+// Min delays from src (-1: unreachable), by Dijkstra with a binary heap;
+// sums in double, so integer delays stay exact.
+void dijkstra(int32_t n, const int32_t* row, const int32_t* dst, const float* delay, int32_t src,
+              std::vector<double>& d) {
+  d.assign((size_t)n, -1.0);
+  std::vector<std::pair<double, int32_t>> heap;
+  auto cmp = [](const std::pair<double, int32_t>& a, const std::pair<double, int32_t>& b) { return a.first > b.first; };
+  d[src] = 0.0;
+  heap.push_back({0.0, src});
+  while (!heap.empty()) {
+    std::pop_heap(heap.begin(), heap.end(), cmp);
+    const auto [du, u] = heap.back();
+    heap.pop_back();
+    if (du > d[u]) continue;
+    for (int32_t p = row[u]; p < row[u + 1]; p++) {
+      const int32_t w = dst[p];
+      const double nd = du + (double)delay[p];
+      if (d[w] < 0.0 || nd < d[w]) {
+        d[w] = nd;
+        heap.push_back({nd, w});
+        std::push_heap(heap.begin(), heap.end(), cmp);
+      }
+    }
+  }
+}
+
+}  // namespace
+
+void landmark_distances(int32_t n, const int32_t* row, const int32_t* dst, const float* delay, int K,
+                        std::vector<float>& out) {
+  out.assign((size_t)n * 4, -1.f);
+  std::vector<double> d, nearest;
+  dijkstra(n, row, dst, delay, 0, d);
+  int32_t next = 0;
+  for (int32_t v = 0; v < n; v++)
+    if (d[v] > d[next]) next = v;
+  nearest.assign((size_t)n, -1.0);
+  for (int j = 0; j < K; j++) {
+    dijkstra(n, row, dst, delay, next, d);
+    for (int32_t v = 0; v < n; v++) {
+      out[(size_t)v * 4 + j] = (float)d[v];
+      if (d[v] >= 0.0 && (nearest[v] < 0.0 || d[v] < nearest[v])) nearest[v] = d[v];
+    }
+    for (int32_t v = 0; v < n; v++)
+      if (nearest[v] > nearest[next]) next = v;
+  }
+}
+
 }  // namespace vine
 
 extern "C" int32_t vine_locality_order(int32_t n, const int32_t* row_ptr, const int32_t* dst, int32_t* out_order) {

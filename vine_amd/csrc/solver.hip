@@ -348,6 +348,15 @@ vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, cons
       g->twin_delay = it != arcs.end() && it->first == rk && it->second <= arcs[p].second;
     }
   }
+  std::vector<float> lmk;
+  if (g->integer_weights && m > 0) {
+    vine::landmark_distances(n, row_ptr, dst, delay, 4, lmk);
+    for (float x : lmk)
+      if (!(x < 16777216.f)) {
+        lmk.clear();
+        break;
+      }
+  }
   hipError_t err = hipGetDevice(&g->device);
   size_t eb = sizeof(int4) * std::max<int64_t>(m, 1);
   if (err == hipSuccess) err = hipMalloc(&g->d_row, sizeof(int32_t) * (n + 1));
@@ -362,6 +371,9 @@ vine_status vine_graph_create(int32_t n, int64_t m, const int32_t* row_ptr, cons
   if (err == hipSuccess) err = hipMalloc(&g->d_inv, sizeof(int32_t) * n);
   if (err == hipSuccess) err = hipMemcpy(g->d_perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy(g->d_inv, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
+  if (err == hipSuccess && !lmk.empty()) err = hipMalloc(&g->d_lmk, sizeof(float4) * n);
+  if (err == hipSuccess && !lmk.empty())
+    err = hipMemcpy(g->d_lmk, lmk.data(), sizeof(float4) * n, hipMemcpyHostToDevice);
   if (err != hipSuccess) {
     set_error("vine_graph_create: %s", hipGetErrorString(err));
     vine_graph_destroy(g);
@@ -377,6 +389,7 @@ vine_status vine_graph_destroy(vine_graph* g) {
   if (g->d_rrow) (void)hipFree(g->d_rrow);
   if (g->d_edges) (void)hipFree(g->d_edges);
   if (g->d_redges) (void)hipFree(g->d_redges);
+  if (g->d_lmk) (void)hipFree(g->d_lmk);
   if (g->d_perm) (void)hipFree(g->d_perm);
   if (g->d_inv) (void)hipFree(g->d_inv);
   delete g;
@@ -697,6 +710,7 @@ vine_status alloc_buffers(vine_solver* s, const Caps& k) {
   c.ct = s->d_ct;
   c.ellipse = o.bound_ball ? 0 : 1;
   c.fwd_cut = g->twin_delay && g->integer_weights ? 1 : 0;
+  c.lmk = g->d_lmk;
   return VINE_OK;
 }
 

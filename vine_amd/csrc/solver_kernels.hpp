@@ -179,6 +179,7 @@ struct Ctx {
   uint32_t* ell;        // [B][ell_words] bit v: lbd(v) is finite (v lies in the bound region)
   int32_t ell_words;
   int32_t fwd_cut;      // twin arcs and integer weights: fwd stops at (bound + fwd(t)) / 2
+  const float4* lmk;    // [n] min delays from four landmarks (-1: unreachable); null: no landmark test
   // scheduler: per-query round statistics, MLP, feature trace
   int32_t stats_on;
   int32_t policy;     // VINE_DELTA_*
@@ -631,6 +632,23 @@ __device__ __forceinline__ void bq_push(const Ctx& c, BoundLds& L, long long qb,
 // holds 32 vertices' values.
 __device__ __forceinline__ float* fwd_ptr(const Ctx& c, long long qv) { return &c.fwd[qv]; }
 
+// A lower bound on the min delay from v to t from the landmark delays lv
+// (of v) and lt (of t): d(v, t) >= d(L, t) - d(L, v) on any graph, and with
+// twin arcs (delays symmetric) also >= d(L, v) - d(L, t).  Both are
+// consistent (h(u) <= delay(u, v) + h(v) along an arc), and exact on integer
+// delays below 2^24.
+__device__ __forceinline__ float landmark_bound(float4 lv, float4 lt, bool twin) {
+  const float a[4] = {lt.x, lt.y, lt.z, lt.w}, b[4] = {lv.x, lv.y, lv.z, lv.w};
+  float h = 0.f;
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    if (a[j] >= 0.f && b[j] >= 0.f) {
+      h = fmaxf(h, a[j] - b[j]);
+      if (twin) h = fmaxf(h, b[j] - a[j]);
+    }
+  return h;
+}
+
 // Whether v lies in query q's bound region, i.e. was reached by the search
 // of lbd (whose values are all <= the bound): one bit per vertex, set when
 // lbd(v) first drops from +inf, so the later searches' region test reads a
@@ -674,6 +692,12 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
   const int32_t* rp = (WHICH == 2) ? c.row_ptr : c.rrow_ptr;
   const int4* ep = (WHICH == 2) ? c.edges : c.redges;
   const int tq = c.q_dst[q];
+  // the forward search queues only vertices v with fwd(v) + h(v) <= bound,
+  // h the landmark bound on the delay from v to t: every vertex of the
+  // region fwd + lbd <= bound passes, and (h being consistent) so does every
+  // vertex of its shortest path from s, so fwd stays exact on the region
+  const bool use_lmk = WHICH == 2 && c.lmk != nullptr;
+  const float4 lmt = use_lmk ? c.lmk[tq] : make_float4(-1.f, -1.f, -1.f, -1.f);
   if (tid == 0) {
     const int root = (WHICH == 2) ? c.q_src[q] : c.q_dst[q];
     if (WHICH == 0) {
@@ -864,6 +888,11 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
             if (lowered[r] && old == LBD_NONE)
               atomicOr(&c.ell[(long long)q * c.ell_words + (ed[r].x >> 5)], 1u << (ed[r].x & 31));
           }
+        }
+        if (use_lmk) {
+#pragma unroll
+          for (int r = 0; r < R; r++)
+            if (lowered[r]) lowered[r] = nd[r] + landmark_bound(c.lmk[ed[r].x], lmt, c.fwd_cut != 0) <= bnd;
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
