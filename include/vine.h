@@ -292,6 +292,15 @@ int64_t vine_dop_sequence(const int64_t* sizes, int n, double max_labels, int64_
  * share cache lines of the per-(query, vertex) arrays.  Host only. */
 int32_t vine_locality_order(int32_t n, const int32_t* row_ptr, const int32_t* dst, int32_t* out_order);
 
+/* The landmark delays vine_graph_create keeps for the forward bound search
+ * (no counterpart in the reference): out[v * 4 + j] = min delay from landmark
+ * j to v (-1: unreachable), four landmarks chosen farthest-first (the vertex
+ * farthest from vertex 0, then each time the vertex farthest from its nearest
+ * landmark).  The search bounds the delay from v to t below by
+ * d(L, t) - d(L, v), and on graphs with twin arcs also by d(L, v) - d(L, t).
+ * Host only. */
+int32_t vine_landmark_delays(int32_t n, const int32_t* row_ptr, const int32_t* dst, const float* delay, float* out);
+
 /* ---- Other label families ------------------------------------------------
  * The reference compiles one label family at a time (CSP in include/label.hpp:
  * 46-169, 248-461 and include/constraint.hpp:21-80, 106-186); here each is a

@@ -187,6 +187,58 @@ def test_window_labels_are_batch_aware():
     assert cost == 100 + 100 + 4000 + 50.0 * 100
 
 
+def _landmarks(g):
+    L = _lib.lib()
+    fn = L.vine_landmark_delays
+    fn.restype = ctypes.c_int32
+    fn.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    row = np.ascontiguousarray(g.row_ptr, np.int32)
+    dst = np.ascontiguousarray(g.dst, np.int32)
+    dly = np.ascontiguousarray(g.delay, np.float32)
+    out = np.zeros((g.n, 4), np.float32)
+    assert fn(g.n, row.ctypes.data, dst.ctypes.data, dly.ctypes.data, out.ctypes.data) == 0
+    return out
+
+
+@pytest.mark.parametrize("kind", ["knn", "rmat"])
+def test_landmark_delays_are_shortest_delays_and_bound_the_rest(kind):
+    """The landmark delays the forward bound search prunes with
+    (vine_landmark_delays, kept by vine_graph_create): each column is scipy's
+    Dijkstra from a landmark (the vertex at delay 0), the landmarks are
+    distinct, and the bound the search takes on the delay from v to t,
+    max_j d(L_j, t) - d(L_j, v) (and, on the symmetric k-NN graph, also
+    d(L_j, v) - d(L_j, t)), never exceeds scipy's delay from v to t.  A bound
+    above the true delay would prune vertices of feasible paths."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import dijkstra
+    g = W.knn(20000, seed=5) if kind == "knn" else W.rmat(scale=12, edge_factor=6, seed=2)
+    lm = _landmarks(g)
+    A = csr_matrix((g.delay.astype(np.float64), g.dst, g.row_ptr), shape=(g.n, g.n))
+    seeds = []
+    for j in range(4):
+        zero = np.nonzero(lm[:, j] == 0.0)[0]
+        assert len(zero) >= 1
+        s = int(zero[0])
+        seeds.append(s)
+        d = dijkstra(A, indices=s)
+        want = np.where(np.isfinite(d), d, -1.0)
+        assert np.array_equal(lm[:, j].astype(np.float64), want), j
+    assert len(set(seeds)) == 4
+    twin = kind == "knn"
+    rng = np.random.default_rng(0)
+    targets = rng.choice(g.n, 12, replace=False)
+    to_t = dijkstra(A.T.tocsr(), indices=targets)  # delay from every v to each target
+    for k, t in enumerate(targets):
+        a, b = lm[t][None, :].astype(np.float64), lm.astype(np.float64)
+        ok = (a >= 0) & (b >= 0)
+        h = np.where(ok, a - b, 0.0)
+        if twin:
+            h = np.maximum(h, np.where(ok, b - a, 0.0))
+        h = np.maximum(h.max(axis=1), 0.0)
+        reach = np.isfinite(to_t[k])
+        assert np.all(h[reach] <= to_t[k][reach]), (kind, t)
+
+
 def test_locality_order_is_a_permutation_that_packs_neighbours():
     """The internal vertex order (vine_locality_order, used by vine_graph_create):
     a permutation, each component contiguous, and on the road-like k-NN graph

@@ -168,3 +168,12 @@ extern "C" int32_t vine_locality_order(int32_t n, const int32_t* row_ptr, const 
   std::copy(o.begin(), o.end(), out_order);
   return VINE_OK;
 }
+
+extern "C" int32_t vine_landmark_delays(int32_t n, const int32_t* row_ptr, const int32_t* dst, const float* delay,
+                                        float* out) {
+  if (n <= 0 || !row_ptr || !out || (row_ptr[n] > 0 && (!dst || !delay))) return VINE_E_ARG;
+  std::vector<float> o;
+  vine::landmark_distances(n, row_ptr, dst, delay, 4, o);
+  std::copy(o.begin(), o.end(), out);
+  return VINE_OK;
+}
