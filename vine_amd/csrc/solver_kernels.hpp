@@ -589,7 +589,7 @@ constexpr int BOUND_BLOCK = 512;
 // distance it is relaxed with is read when it is taken, so a vertex lowered
 // after it was queued is relaxed with its newest distance.  Those reads and
 // the arc reads are issued together.  HBM entries hold only the vertex and
-// are taken in the round that reads them.
+// are taken or carried by their distance (within four windows).
 constexpr int BQ_LDS = 1024;
 constexpr int BQ_HASH = 2048;
 constexpr int BQ_PROBES = 8;
@@ -740,6 +740,8 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
     // block-sized slice), so a small frontier still spreads over every wave
     for (int base = 0; base < ncur; base += blockDim.x) {
       const int i = base + lane * nw + wid;
+      // the wave's last entry index is past the LDS part (and exists)
+      const bool hbm_wave = min(base + (WAVE - 1) * nw + wid, ncur - 1) >= BQ_LDS;
       int deg = 0, row = 0, u = 0;
       float du = 0.f, dl2 = 0.f;  // distance (delay for WHICH 0 and 2, len for 1); len tie key
       uint32_t dub = 0;           // WHICH 3: the integer distance (du is its float value)
@@ -766,9 +768,14 @@ __device__ void wg_search(const Ctx& c, int q, float delta, BoundLds& L, int& re
           du = load_agent(bound_dist<WHICH>(c, qv));
         }
         // taken by its key when queued, so the arc reads need not wait for
-        // the distance; HBM entries keep no key and are taken (relaxing a
-        // vertex early only costs work: it is queued again when lowered)
-        carry = !ent.w && __int_as_float(ent.z) > lim_key;
+        // the distance.  HBM entries keep no key and go by their distance,
+        // within four windows: a frontier that overflows the LDS part is
+        // worked through in fewer rounds, and the overflow never runs far
+        // ahead of the window (taking every HBM entry made a search with a
+        // 512-entry LDS part six times slower).  The test on them is in the
+        // waves that hold any (wave-uniform), so the others never wait.
+        if (hbm_wave) carry = ent.w ? du > __uint_as_float(L.kmin[cur]) + 4.f * delta : __int_as_float(ent.z) > lim_key;
+        else carry = __int_as_float(ent.z) > lim_key;
         if (!carry) {
           if (row_known((uint32_t)ent.y)) {
             row = (int)((uint32_t)ent.y >> 8);
